@@ -1,0 +1,186 @@
+"""ctypes binding of the native library ``libctpwa_amd.so`` (C ABI in include/ctpwa_amd.h).
+
+The library is built in-tree by ``ctpwa_amd.build`` (or ``__graft_entry__.build()``).
+There is no fallback: if the library is missing, importing a GPU entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libctpwa_amd.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_ll = ctypes.c_longlong
+
+# every symbol include/ctpwa_amd.h declares
+EXPORTS = [
+    "ctpwa_create", "ctpwa_destroy", "ctpwa_dims", "ctpwa_param_info", "ctpwa_nll_grad",
+    "ctpwa_hessian", "ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_time_eval_kernel",
+    "ctpwa_footprint_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
+    "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
+]
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NativeError("native library %s is not built; run `python -m ctpwa_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    L.ctpwa_create.restype = ctypes.c_int
+    L.ctpwa_create.argtypes = [ctypes.c_char_p, _vp, _ll, _vp, _vp, _ll, _vp, _vp, _ll, _vp,
+                               ctypes.c_int, ctypes.POINTER(_vp)]
+    L.ctpwa_destroy.argtypes = [_vp]
+    L.ctpwa_destroy.restype = None
+    L.ctpwa_dims.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    L.ctpwa_param_info.argtypes = [_vp, _dp, _dp, _dp]
+    for f in ("ctpwa_nll_grad", "ctpwa_hessian"):
+        getattr(L, f).argtypes = [_vp, _vp, _vp, _vp]
+    for f in ("ctpwa_intensity", "ctpwa_amplitudes"):
+        getattr(L, f).argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+    L.ctpwa_time_eval_kernel.argtypes = [_vp, _vp, ctypes.c_int, _vp, _dp]
+    L.ctpwa_footprint_bytes.restype = _ll
+    L.ctpwa_footprint_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll]
+    L.ctpwa_layout.argtypes = [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll), ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.ctpwa_kernel_source.restype = ctypes.c_char_p
+    L.ctpwa_kernel_source.argtypes = [_vp]
+    L.ctpwa_generate_source.restype = _ll
+    L.ctpwa_generate_source.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _ll]
+    L.ctpwa_compile_check.restype = _ll
+    L.ctpwa_compile_check.argtypes = [ctypes.c_char_p]
+    L.ctpwa_last_error.restype = ctypes.c_char_p
+    L.ctpwa_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return lib().ctpwa_last_error().decode(errors="replace")
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise NativeError("%s failed: %s" % (what, last_error()))
+
+
+def generate_source(plan_text: str) -> str:
+    n = lib().ctpwa_generate_source(plan_text.encode(), None, 0)
+    if n < 0:
+        raise NativeError(last_error())
+    buf = ctypes.create_string_buffer(int(n))
+    lib().ctpwa_generate_source(plan_text.encode(), buf, n)
+    return buf.value.decode()
+
+
+def compile_check(plan_text: str) -> int:
+    n = lib().ctpwa_compile_check(plan_text.encode())
+    if n < 0:
+        raise NativeError(last_error())
+    return int(n)
+
+
+def footprint_bytes(plan_text: str, n_phsp: int, n_data: int, n_bkg: int) -> int:
+    n = lib().ctpwa_footprint_bytes(plan_text.encode(), n_phsp, n_data, n_bkg)
+    if n < 0:
+        raise NativeError(last_error())
+    return int(n)
+
+
+def _addr(a):
+    if a is None:
+        return None
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Handle:
+    """Owns one native likelihood (device buffers, generated kernel)."""
+
+    def __init__(self, plan_text: str, phsp, data, bkg=None, w_phsp=None, w_data=None, w_bkg=None,
+                 device: int = 0):
+        import numpy as np
+        L = lib()
+        arrs = []
+        for a in (phsp, w_phsp, data, w_data, bkg, w_bkg):
+            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
+        phsp, w_phsp, data, w_data, bkg, w_bkg = arrs
+        n = [0 if x is None else x.shape[0] for x in (phsp, data, bkg)]
+        for w, k in ((w_phsp, 0), (w_data, 1), (w_bkg, 2)):
+            if w is not None and w.shape[0] != n[k]:
+                raise ValueError("weight array length does not match its sample")
+        h = ctypes.c_void_p()
+        rc = L.ctpwa_create(plan_text.encode(), _addr(phsp), n[0], _addr(w_phsp), _addr(data), n[1],
+                            _addr(w_data), _addr(bkg), n[2], _addr(w_bkg), device, ctypes.byref(h))
+        check(rc, "ctpwa_create")
+        self.h = h
+        self.device = device
+        a, f, t, p = (ctypes.c_int() for _ in range(4))
+        check(L.ctpwa_dims(h, ctypes.byref(a), ctypes.byref(f), ctypes.byref(t), ctypes.byref(p)), "ctpwa_dims")
+        self.n_amp, self.n_free, self.n_theta, self.n_pol = a.value, f.value, t.value, p.value
+        self.n_events = n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ctpwa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_params(self) -> int:
+        return 2 * self.n_free + self.n_theta
+
+    def param_info(self):
+        import numpy as np
+        init, lo, hi = (np.zeros(self.n_theta) for _ in range(3))
+        check(lib().ctpwa_param_info(self.h, init.ctypes.data_as(_dp), lo.ctypes.data_as(_dp),
+                                     hi.ctypes.data_as(_dp)), "ctpwa_param_info")
+        return init, lo, hi
+
+    def nll_grad(self, params_ptr: int, out_ptr: int, stream: int):
+        check(lib().ctpwa_nll_grad(self.h, ctypes.c_void_p(params_ptr), ctypes.c_void_p(out_ptr),
+                                   ctypes.c_void_p(stream)), "ctpwa_nll_grad")
+
+    def hessian(self, params_ptr: int, out_ptr: int, stream: int):
+        check(lib().ctpwa_hessian(self.h, ctypes.c_void_p(params_ptr), ctypes.c_void_p(out_ptr),
+                                  ctypes.c_void_p(stream)), "ctpwa_hessian")
+
+    def intensity(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
+        check(lib().ctpwa_intensity(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
+                                    ctypes.c_void_p(stream)), "ctpwa_intensity")
+
+    def amplitudes(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
+        check(lib().ctpwa_amplitudes(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
+                                     ctypes.c_void_p(stream)), "ctpwa_amplitudes")
+
+    def time_eval_kernel(self, params_ptr: int, iters: int, stream: int) -> float:
+        ms = ctypes.c_double()
+        check(lib().ctpwa_time_eval_kernel(self.h, ctypes.c_void_p(params_ptr), iters, ctypes.c_void_p(stream),
+                                           ctypes.byref(ms)), "ctpwa_time_eval_kernel")
+        return ms.value
+
+    def layout(self):
+        n3 = (_ll * 3)()
+        b = _ll()
+        nu, ni, nw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().ctpwa_layout(self.h, n3, ctypes.byref(b), ctypes.byref(nu), ctypes.byref(ni),
+                                 ctypes.byref(nw)), "ctpwa_layout")
+        return dict(n_events=list(n3), bytes_per_eval=b.value, n_sl_tensors=nu.value,
+                    n_invariants=ni.value, n_workgroups=nw.value)
+
+    def kernel_source(self) -> str:
+        return lib().ctpwa_kernel_source(self.h).decode()

@@ -1,0 +1,213 @@
+"""``ctpwa.analysis`` drop-in: the reference's Python boundary over the native library.
+
+Method names, argument meaning, layouts and error behaviour follow the reference's
+pybind11 module (src/main.cu:90-131) and ``class analysis`` (src/Analysis.cu:1268-4832):
+
+* ``getNLL(params)``: params is a 1-D float64 tensor [2n + P] on the analysis device,
+  laid out [Re p_0..n-1, Im p_0..n-1, theta_0..P-1]; returns a 0-dim float64 tensor
+  whose backward gives the analytic gradient (NLLFunction, src/Analysis.cu:625-1261).
+  A wrong dtype or device raises RuntimeError (TORCH_CHECK, src/Analysis.cu:1296-1301).
+* ``getHessian(params)``: [(2n+P), (2n+P)] float64 on the device, grouped layout.
+* ``getNVector() / getNFreeTheta() / getNParams() / getParamNames() /
+  getFreeResParams() / getFitMode() / setFitMode() / getNPolarizations() /
+  getAmplitudeNames() / isValid() / reCalcAmp(theta)``.
+
+All device work runs on the current torch stream; nothing synchronises the host
+except reading results.
+"""
+from __future__ import annotations
+
+import os
+from typing import Dict, Optional
+
+import numpy as np
+import torch
+
+from . import _native
+from .config import ConfigParser
+from .model import Plan, build_plan
+from .phasespace import load_sample
+
+
+def _read_weights(entry, base_dir: str, n: int) -> Optional[np.ndarray]:
+    """``*_weights``: ``[x]`` a constant, ``[dat, file]`` one value per line
+    (src/Analysis.cu:292-495)."""
+    if entry is None:
+        return None
+    if len(entry) == 1:
+        return np.full(n, float(entry[0]))
+    path = entry[1]
+    if not os.path.isabs(path):
+        path = os.path.join(base_dir, path)
+    if entry[0].lower() == "npy":
+        w = np.load(path, allow_pickle=False).astype(np.float64).reshape(-1)
+    else:
+        with open(path) as f:
+            w = np.array(f.read().split(), dtype=np.float64)
+    if w.shape[0] < n:
+        raise ValueError("%s holds %d weights for %d events" % (path, w.shape[0], n))
+    return w[:n]
+
+
+class NLLFunction(torch.autograd.Function):
+    """Forward runs the fused NLL+gradient; backward scales the stored gradient."""
+
+    @staticmethod
+    def forward(ctx, params, ana):
+        out = ana._nll_grad_tensor(params)
+        ctx.save_for_backward(out[1:])
+        return out[0].clone()
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return g * grad_out, None
+
+
+class analysis:
+    """One likelihood on one device.
+
+    ``samples`` (an extension for callers that hold events in memory) maps ``phsp``,
+    ``data``, ``bkg`` to [N, nfinal, 4] float64 arrays and ``*_weights`` to [N] arrays;
+    when given, the ``Data`` section's files are not read.
+    """
+
+    def __init__(self, config_file: str = "config.yml", samples: Optional[Dict[str, np.ndarray]] = None,
+                 device: Optional[int] = None):
+        self._valid = False
+        self._fit_mode = 1
+        self.config_file = config_file
+        self.cfg = ConfigParser(config_file)
+        self.plan: Plan = build_plan(self.cfg, self._fit_mode)
+        nf = len(self.plan.final_order)
+        base = self.cfg.base_dir
+        df = self.cfg.data_files
+        s = dict(samples or {})
+        for key in ("phsp", "data", "bkg"):
+            if key not in s and key in df:
+                s[key] = load_sample(df[key], base, nf)
+        if s.get("phsp") is None:
+            raise RuntimeError("config has no phase-space sample")
+        if s.get("data") is None:
+            raise RuntimeError("config has no data sample")
+        n = {k: (0 if s.get(k) is None else s[k].shape[0]) for k in ("phsp", "data", "bkg")}
+        w = {}
+        for key in ("phsp", "data", "bkg"):
+            wk = key + "_weights"
+            if wk in s and s[wk] is not None:
+                w[key] = np.broadcast_to(np.asarray(s[wk], dtype=np.float64), (n[key],))
+            elif wk in df and n[key] > 0:
+                w[key] = _read_weights(df[wk], base, n[key])
+            else:
+                w[key] = None
+        if self.cfg.precision not in ("auto", "double", "float"):
+            raise RuntimeError("precision must be auto, float or double")
+        if device is None:
+            device = torch.cuda.current_device() if torch.cuda.is_available() else 0
+        self.device = torch.device("cuda", device)
+        self.n_events = n
+        self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
+                                 w["phsp"], w["data"], w["bkg"], device=device)
+        self._valid = True
+
+    # ---- reference API -------------------------------------------------------
+    def isValid(self) -> bool:
+        return self._valid
+
+    def getNLL(self, params: torch.Tensor) -> torch.Tensor:
+        self._check_params(params)
+        return NLLFunction.apply(params, self)
+
+    def getHessian(self, params: torch.Tensor) -> torch.Tensor:
+        self._check_params(params)
+        p = params.detach().contiguous()
+        n = self.getNParams()
+        H = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        self._h.hessian(p.data_ptr(), H.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
+        return H
+
+    def getNVector(self) -> int:
+        return self.plan.n_free
+
+    def getNFreeTheta(self) -> int:
+        return self.plan.n_theta
+
+    def getNParams(self) -> int:
+        return self.plan.n_params()
+
+    def getParamNames(self):
+        return list(self.plan.param_names)
+
+    def getAmplitudeNames(self):
+        return list(self.plan.amp_names)
+
+    def getNPolarizations(self) -> int:
+        return self.plan.npol
+
+    def getFreeResParams(self) -> torch.Tensor:
+        """[3, P] float64 on the CPU: initial value, lower and upper bound."""
+        init, lo, hi = self._h.param_info()
+        return torch.tensor(np.stack([init, lo, hi]), dtype=torch.float64)
+
+    def getFitMode(self) -> int:
+        return self._fit_mode
+
+    def setFitMode(self, mode: int) -> None:
+        if mode != 1:
+            raise RuntimeError("only fit mode 1 (vspace) is built")
+        self._fit_mode = mode
+
+    def reCalcAmp(self, theta) -> None:
+        """The reference recomputes its stored amplitudes at ``theta``
+        (src/AmpGen.cu:2344-2726).  Here amplitudes are evaluated from the theta part of
+        the parameter vector inside every call, so there is nothing to recompute; the
+        argument is validated only."""
+        t = torch.as_tensor(theta, dtype=torch.float64)
+        if t.numel() != self.plan.n_theta:
+            raise RuntimeError("theta must have %d entries" % self.plan.n_theta)
+
+    # ---- extensions ----------------------------------------------------------
+    def getIntensity(self, params: torch.Tensor, cls: int = 1) -> torch.Tensor:
+        """Per-event intensity of a sample (0 phsp, 1 data, 2 bkg)."""
+        self._check_params(params)
+        n = self.n_events[("phsp", "data", "bkg")[cls]]
+        out = torch.empty(n, dtype=torch.float64, device=self.device)
+        if n:
+            self._h.intensity(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
+                              torch.cuda.current_stream(self.device).cuda_stream)
+        return out
+
+    def getAmplitudes(self, params: torch.Tensor, cls: int = 1) -> torch.Tensor:
+        """Per-event amplitudes [N, nPol, nAmp] complex128 (couplings not applied)."""
+        self._check_params(params)
+        n = self.n_events[("phsp", "data", "bkg")[cls]]
+        out = torch.empty((n, self.plan.npol, self.plan.n_amp, 2), dtype=torch.float64, device=self.device)
+        if n:
+            self._h.amplitudes(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
+                               torch.cuda.current_stream(self.device).cuda_stream)
+        return torch.view_as_complex(out)
+
+    def kernelTimeMs(self, params: torch.Tensor, iters: int = 20) -> float:
+        self._check_params(params)
+        return self._h.time_eval_kernel(params.detach().contiguous().data_ptr(), iters,
+                                        torch.cuda.current_stream(self.device).cuda_stream)
+
+    def layout(self):
+        return self._h.layout()
+
+    # ---- internals -----------------------------------------------------------
+    def _check_params(self, params: torch.Tensor):
+        if not isinstance(params, torch.Tensor):
+            raise RuntimeError("params must be a torch.Tensor")
+        if params.dtype != torch.float64:
+            raise RuntimeError("params must be float64")
+        if not params.is_cuda or params.device != self.device:
+            raise RuntimeError("params must be on %s" % self.device)
+        if params.dim() != 1 or params.numel() != self.getNParams():
+            raise RuntimeError("params must be 1-D with %d entries" % self.getNParams())
+
+    def _nll_grad_tensor(self, params: torch.Tensor) -> torch.Tensor:
+        p = params.detach().contiguous()
+        out = torch.empty(1 + self.getNParams(), dtype=torch.float64, device=self.device)
+        self._h.nll_grad(p.data_ptr(), out.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
+        return out

@@ -1,0 +1,703 @@
+// C ABI of the native likelihood library: handle construction (plan -> device
+// layout -> one-time SL/invariant kernels -> generated fused kernel via hipRTC) and
+// the per-evaluation entry points.  See include/ctpwa_amd.h for the contract.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ctpwa_amd.h"
+#include "device.hpp"
+#include "model.hpp"
+
+namespace ctpwa {
+std::string generate_eval_kernel(const Model_& M);
+}
+
+extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctpwa::DTopo*, int, const ctpwa::DNode*,
+                                         const ctpwa::DKin*, int, const ctpwa::DU*, int, const double*, const int*, int,
+                                         const ctpwa::DInv*, int, double2*, double*, long long, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
+
+using namespace ctpwa;
+
+static thread_local std::string g_err;
+
+static int fail(const std::string& msg) {
+    g_err = msg;
+    return 1;
+}
+
+#define HIPCHECK(x)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host angular-momentum algebra for the Clebsch-Gordan x Y_lm table
+// ---------------------------------------------------------------------------
+// factorial with the reference's semantics (src/Amplitude.cu:5-91): exact for
+// integers and half-integers up to 11.5, Stirling with two correction terms for
+// integers >= 12
+static double host_fact(double n) {
+    if (n < 0) return 0.0;
+    double ri = std::round(n);
+    bool is_int = std::fabs(n - ri) < 1e-10;
+    if (is_int && ri <= 11) {
+        double f = 1.0;
+        for (int i = 2; i <= (int)ri; ++i) f *= i;
+        return f;
+    }
+    if (is_int) {
+        int m = (int)ri;
+        return std::sqrt(2.0 * M_PI * m) * std::pow(m / M_E, m) * (1.0 + 1.0 / (12.0 * m) + 1.0 / (288.0 * m * m));
+    }
+    if (std::fabs(2 * n - std::round(2 * n)) < 1e-10 && n <= 11.5) {
+        int k = (int)std::round(n - 0.5);
+        double df = 1.0;
+        for (int i = 1; i <= 2 * k + 1; i += 2) df *= i;
+        return df / std::pow(2.0, k + 1) * std::sqrt(M_PI);
+    }
+    return std::tgamma(n);
+}
+
+// <j1 m1 j2 m2 | J M> (Racah), arguments as 2j+1 and 2m
+static double host_cg(int dj1, int m1x2, int dj2, int m2x2, int dJ, int Mx2) {
+    double j1 = (dj1 - 1) / 2.0, j2 = (dj2 - 1) / 2.0, J = (dJ - 1) / 2.0;
+    double m1 = m1x2 / 2.0, m2 = m2x2 / 2.0, M = Mx2 / 2.0;
+    if (std::fabs(M - m1 - m2) > 1e-10) return 0.0;
+    if (J < std::fabs(j1 - j2) || J > j1 + j2) return 0.0;
+    if (std::fabs(m1) > j1 || std::fabs(m2) > j2 || std::fabs(M) > J) return 0.0;
+    double pre = std::sqrt((2 * J + 1) * host_fact(j1 + j2 - J) * host_fact(j1 - j2 + J) * host_fact(-j1 + j2 + J) /
+                           host_fact(j1 + j2 + J + 1));
+    pre *= std::sqrt(host_fact(J + M) * host_fact(J - M) * host_fact(j1 - m1) * host_fact(j1 + m1) *
+                     host_fact(j2 - m2) * host_fact(j2 + m2));
+    int kmin = (int)std::ceil(std::max(0.0, std::max(j2 - J - m1, j1 - J + m2)));
+    int kmax = (int)std::floor(std::min(j1 + j2 - J, std::min(j1 - m1, j2 + m2)));
+    double s = 0;
+    for (int k = kmin; k <= kmax; ++k) {
+        double den = host_fact(k) * host_fact(j1 + j2 - J - k) * host_fact(j1 - m1 - k) * host_fact(j2 + m2 - k) *
+                     host_fact(J - j2 + m1 + k) * host_fact(J - j1 - m2 + k);
+        if (std::fabs(den) > 1e-15) s += ((k & 1) ? -1.0 : 1.0) / den;
+    }
+    return pre * s;
+}
+
+// table entry for amp[s][s1][s2] = qs^L / sqrt(2j+1) <j1 s1 j2 s2|S s1+s2> <S s1+s2 L M|j s>
+//   N_LM phase(M) P_L^|M|(cos) e^{-iM phi}:  everything but qs^L, P and the phase factor
+static void node_cg_table(const PNode& n, int S, int L, std::vector<double>& out) {
+    for (int si = 0; si < n.dj; ++si)
+        for (int s1i = 0; s1i < n.dj1; ++s1i)
+            for (int s2i = 0; s2i < n.dj2; ++s2i) {
+                int s = n.dj - 1 - 2 * si, s1 = n.dj1 - 1 - 2 * s1i, s2 = n.dj2 - 1 - 2 * s2i;
+                int Mx2 = s - s1 - s2;
+                double v = 0.0;
+                if (Mx2 % 2 == 0) {
+                    int M = Mx2 / 2, am = std::abs(M);
+                    if (am <= L) {
+                        double cg1 = host_cg(n.dj1, s1, n.dj2, s2, S, s1 + s2);
+                        double cg2 = host_cg(S, s1 + s2, 2 * L + 1, Mx2, n.dj, s);
+                        double norm = (L == 0) ? 0.5 * std::sqrt(1.0 / M_PI)
+                                               : std::sqrt((2.0 * L + 1.0) * host_fact(L - am) / (4.0 * M_PI * host_fact(L + am)));
+                        double ph = (M < 0 && (am & 1)) ? -1.0 : 1.0;
+                        v = cg1 * cg2 * norm * ph / std::sqrt((double)n.dj);
+                    }
+                }
+                out.push_back(v);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// generated-kernel cache (one module per device and source text)
+// ---------------------------------------------------------------------------
+static std::mutex g_mod_mu;
+static std::map<std::pair<int, std::string>, hipModule_t> g_mods;
+
+static hipFunction_t compile_eval(int device, const std::string& src) {
+    std::lock_guard<std::mutex> lk(g_mod_mu);
+    auto key = std::make_pair(device, src);
+    auto it = g_mods.find(key);
+    hipModule_t mod;
+    if (it != g_mods.end()) {
+        mod = it->second;
+    } else {
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+            throw std::runtime_error("hiprtcCreateProgram failed");
+        const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast"};
+        hiprtcResult r = hiprtcCompileProgram(prog, 4, opts);
+        if (r != HIPRTC_SUCCESS) {
+            size_t ls = 0;
+            hiprtcGetProgramLogSize(prog, &ls);
+            std::string log(ls + 1, '\0');
+            hiprtcGetProgramLog(prog, &log[0]);
+            hiprtcDestroyProgram(&prog);
+            throw std::runtime_error("hipRTC compile of the fused kernel failed: " + log);
+        }
+        size_t cs = 0;
+        hiprtcGetCodeSize(prog, &cs);
+        std::vector<char> code(cs);
+        hiprtcGetCode(prog, code.data());
+        hiprtcDestroyProgram(&prog);
+        HIPCHECK(hipModuleLoadData(&mod, code.data()));
+        g_mods[key] = mod;
+    }
+    hipFunction_t fn;
+    HIPCHECK(hipModuleGetFunction(&fn, mod, "ctpwa_eval"));
+    return fn;
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct ctpwa_handle {
+    int device = 0;
+    Model_ M;
+    long long n[3] = {0, 0, 0}, off[3] = {0, 0, 0}, ntot = 0;
+    double Wd = 0, Wb = 0;
+    int pw[3] = {0, 0, 0};
+    double wc[3] = {1.0, 1.0, 1.0};
+    std::vector<void*> allocs;
+    double2* sl = nullptr;
+    double* inv = nullptr;
+    double* wts = nullptr;
+    double* table = nullptr;
+    double* part = nullptr;
+    int nwg = 0, g0[3] = {0, 0, 0}, g1[3] = {0, 0, 0};
+    int* wg_cls = nullptr;
+    long long* wg_rng = nullptr;
+    int nacc = 0;
+    PrepArgs prep{};
+    FinArgs fin{};
+    AuxArgs aux{};
+    hipFunction_t fn_eval = nullptr;
+    std::string src;
+    std::vector<int> used_inv;
+
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        if (count == 0) count = 1;
+        HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+        allocs.push_back(p);
+        return (T*)p;
+    }
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* p = alloc<T>(v.size());
+        if (!v.empty()) HIPCHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~ctpwa_handle() {
+        int prev = 0;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(device);
+        for (void* p : allocs) (void)hipFree(p);
+        (void)hipSetDevice(prev);
+    }
+};
+
+static void build_tables(ctpwa_handle* h, std::vector<DTopo>& topos, std::vector<DNode>& nodes,
+                         std::vector<DKin>& kins, std::vector<DU>& us, std::vector<double>& cgtab,
+                         std::vector<DInv>& invs) {
+    const Model_& M = h->M;
+    const Plan& P = M.P;
+    if ((int)M.kin.size() > MAXKIN) throw std::runtime_error("too many distinct kinematic nodes");
+    if (P.npol > MAXPOL) throw std::runtime_error("too many polarisation states");
+    if (P.nfinal + 1 + MAXNODE > MAXPART) throw std::runtime_error("too many final-state particles");
+    kins.resize(M.kin.size());
+    for (size_t i = 0; i < M.kin.size(); ++i)
+        kins[i] = DKin{M.kin[i].mset, M.kin[i].d1set, M.kin[i].d2set, -1, -1, 0};
+    for (size_t t = 0; t < P.topos.size(); ++t) {
+        const PTopo& T = P.topos[t];
+        DTopo dt{};
+        dt.nn = (int)T.nodes.size();
+        dt.node0 = (int)nodes.size();
+        // daughters-first order of the intermediates
+        std::vector<int> order;
+        std::function<void(int)> visit = [&](int ref) {
+            if (ref >= 0) return;
+            int k = -ref - 1;
+            visit(T.nodes[k].d1ref);
+            visit(T.nodes[k].d2ref);
+            if (std::find(order.begin(), order.end(), k) == order.end()) order.push_back(k);
+        };
+        visit(T.nodes[0].d1ref);
+        visit(T.nodes[0].d2ref);
+        if ((int)order.size() != dt.nn - 1) throw std::runtime_error("topology has unreachable nodes");
+        for (size_t i = 0; i < order.size(); ++i) dt.order[i] = order[i];
+        topos.push_back(dt);
+        for (size_t k = 0; k < T.nodes.size(); ++k) {
+            const PNode& n = T.nodes[k];
+            int kin = M.topo_kin[t][k];
+            nodes.push_back(DNode{n.dj, n.dj1, n.dj2, n.d1ref, n.d2ref, n.has_bf, n.lmin, kin, n.bf_d, n.mm, n.md1, n.md2});
+            if (kins[kin].owner_topo < 0) { kins[kin].owner_topo = (int)t; kins[kin].owner_node = (int)k; }
+            if (n.dj1 > 5 || n.dj2 > 5) throw std::runtime_error("daughter spin above 2 is not built");
+            // an intermediate daughter's q0 mass must come from a configured mass or a
+            // resonance mass parameter (otherwise q0 would depend on the event)
+        }
+    }
+    for (const USpec& U : M.us) {
+        DU du{};
+        du.topo = U.topo;
+        du.mask = (int)U.mask;
+        const PTopo& T = P.topos[U.topo];
+        for (size_t k = 0; k < T.nodes.size(); ++k) {
+            du.S[k] = T.slc[U.slc][k].first;
+            du.L[k] = T.slc[U.slc][k].second;
+            if (du.L[k] > 7) throw std::runtime_error("orbital angular momentum above 7 is not built");
+            du.cg[k] = (int)cgtab.size();
+            node_cg_table(T.nodes[k], du.S[k], du.L[k], cgtab);
+        }
+        us.push_back(du);
+    }
+    for (const InvSpec& s : M.inv) invs.push_back(DInv{s.kin, s.kind, s.L, 0, s.d, s.ma, s.mb});
+}
+
+static void check_q0_masses(const Model_& M) {
+    const Plan& P = M.P;
+    for (const PBlock& B : P.blocks) {
+        if (B.nsl == 0) continue;
+        const PTopo& T = P.topos[B.topo];
+        for (const PNode& n : T.nodes) {
+            for (int side = 0; side < 2; ++side) {
+                double cfg = side ? n.md2 : n.md1;
+                int ref = side ? n.d2ref : n.d1ref;
+                if (cfg > 0) continue;
+                bool ok = false;
+                if (ref < 0)
+                    for (const PRes& r : B.res) if (r.node == -ref - 1 && !r.params.empty()) ok = true;
+                if (!ok) throw std::runtime_error("a daughter without a configured mass and without a resonance mass "
+                                                  "parameter makes q0 event-dependent; not built");
+            }
+        }
+    }
+}
+
+static void setup_partition(ctpwa_handle* h) {
+    long long target = std::min<long long>(1024, std::max<long long>(1, (h->ntot + 1023) / 1024));
+    std::vector<int> cls;
+    std::vector<long long> rng;
+    for (int c = 0; c < 3; ++c) {
+        h->g0[c] = (int)cls.size();
+        long long n = h->n[c];
+        if (n > 0) {
+            long long want = std::max<long long>(1, (long long)std::llround((double)target * n / h->ntot));
+            long long chunk = (n + want - 1) / want;
+            chunk = ((chunk + 255) / 256) * 256;
+            long long nw = (n + chunk - 1) / chunk;
+            for (long long g = 0; g < nw; ++g) {
+                cls.push_back(c);
+                rng.push_back(h->off[c] + g * chunk);
+                rng.push_back(h->off[c] + std::min(n, (g + 1) * chunk));
+            }
+        }
+        h->g1[c] = (int)cls.size();
+    }
+    h->nwg = (int)cls.size();
+    h->wg_cls = h->upload(cls);
+    h->wg_rng = h->upload(rng);
+}
+
+static bool constant_weights(const double* w, long long n, double* c) {
+    if (!w || n == 0) { *c = 1.0; return true; }
+    for (long long i = 1; i < n; ++i) if (w[i] != w[0]) return false;
+    *c = w[0];
+    return true;
+}
+
+extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long n_phsp, const double* w_phsp,
+                            const double* data, long long n_data, const double* w_data,
+                            const double* bkg, long long n_bkg, const double* w_bkg, int device, ctpwa_handle** out) {
+    if (!out) return fail("ctpwa_create: out is NULL");
+    *out = nullptr;
+    ctpwa_handle* h = nullptr;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    try {
+        if (!plan_text) throw std::runtime_error("plan_text is NULL");
+        if (n_phsp < 0 || n_data < 0 || n_bkg < 0) throw std::runtime_error("negative event count");
+        if (n_phsp == 0) throw std::runtime_error("phase-space sample is empty");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device available");
+        if (device < 0 || device >= ndev) throw std::runtime_error("device index out of range");
+        HIPCHECK(hipSetDevice(device));
+        h = new ctpwa_handle();
+        h->device = device;
+        h->M = derive(parse_plan(plan_text));
+        check_q0_masses(h->M);
+        const Model_& M = h->M;
+        const Plan& P = M.P;
+        const double* moms[3] = {phsp, data, bkg};
+        const double* ws[3] = {w_phsp, w_data, w_bkg};
+        long long ns[3] = {n_phsp, n_data, n_bkg};
+        for (int c = 0; c < 3; ++c) {
+            h->n[c] = ns[c];
+            h->off[c] = h->ntot;
+            h->ntot += ns[c];
+            if (ns[c] > 0 && !moms[c]) throw std::runtime_error("event array is NULL");
+        }
+        // weights: constant per class unless they vary
+        std::vector<double> wall;
+        bool any_var = false;
+        for (int c = 1; c < 3; ++c) {
+            double cst;
+            if (!constant_weights(ws[c], ns[c], &cst)) { any_var = true; h->pw[c] = 1; }
+            h->wc[c] = cst;
+        }
+        if (ws[0] && n_phsp > 0) {
+            double cst;
+            if (!constant_weights(ws[0], n_phsp, &cst) || cst != 1.0)
+                throw std::runtime_error("phase-space weights are not built (the reference's phase-space mean is unweighted)");
+        }
+        h->Wd = 0;
+        for (long long i = 0; i < n_data; ++i) h->Wd += w_data ? w_data[i] : 1.0;
+        h->Wb = 0;
+        for (long long i = 0; i < n_bkg; ++i) h->Wb += w_bkg ? w_bkg[i] : 1.0;
+        if (any_var) {
+            wall.assign(h->ntot, 1.0);
+            for (int c = 1; c < 3; ++c)
+                if (ws[c]) std::copy(ws[c], ws[c] + ns[c], wall.begin() + h->off[c]);
+            h->wts = h->upload(wall);
+        }
+
+        std::vector<DTopo> topos;
+        std::vector<DNode> nodes;
+        std::vector<DKin> kins;
+        std::vector<DU> us;
+        std::vector<double> cgtab;
+        std::vector<DInv> invs;
+        build_tables(h, topos, nodes, kins, us, cgtab, invs);
+        int nu = (int)us.size(), ninv = (int)invs.size();
+        h->sl = h->alloc<double2>((size_t)std::max(1, nu) * P.npol * h->ntot);
+        h->inv = h->alloc<double>((size_t)std::max(1, ninv) * h->ntot);
+        {
+            DTopo* d_topos = h->upload(topos);
+            DNode* d_nodes = h->upload(nodes);
+            DKin* d_kins = h->upload(kins);
+            DU* d_us = h->upload(us);
+            double* d_cg = h->upload(cgtab);
+            int* d_pol = h->upload(P.polmap);
+            DInv* d_invs = h->upload(invs);
+            // stage events class by class through a bounded buffer
+            long long maxn = std::max(n_phsp, std::max(n_data, n_bkg));
+            long long stage = std::min<long long>(maxn, 1 << 20);
+            double* d_mom = nullptr;
+            HIPCHECK(hipMalloc(&d_mom, (size_t)stage * P.nfinal * 4 * sizeof(double)));
+            try {
+                for (int c = 0; c < 3; ++c) {
+                    for (long long s0 = 0; s0 < ns[c]; s0 += stage) {
+                        long long cnt = std::min(stage, ns[c] - s0);
+                        HIPCHECK(hipMemcpy(d_mom, moms[c] + (size_t)s0 * P.nfinal * 4,
+                                           (size_t)cnt * P.nfinal * 4 * sizeof(double), hipMemcpyHostToDevice));
+                        HIPCHECK(ctpwa_launch_init(d_mom, cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
+                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                   h->sl, h->inv, h->ntot, h->off[c] + s0, nullptr));
+                        HIPCHECK(hipDeviceSynchronize());
+                    }
+                }
+            } catch (...) {
+                (void)hipFree(d_mom);
+                throw;
+            }
+            (void)hipFree(d_mom);
+        }
+
+        // model tables for prep / finalize / aux
+        std::vector<DBlock> blocks;
+        std::vector<DRes> res;
+        std::vector<int> slcL, slc0;
+        std::vector<DDen> dens;
+        std::vector<DAmp> amps;
+        std::vector<DDenTerms> dterms;
+        std::vector<DTerm> terms;
+        for (size_t t = 0; t < P.topos.size(); ++t) {
+            slc0.push_back((int)(slcL.size() / MAXNODE));
+            for (const auto& row : P.topos[t].slc) {
+                for (int k = 0; k < MAXNODE; ++k) slcL.push_back(k < (int)row.size() ? row[k].second : 0);
+            }
+        }
+        for (size_t b = 0; b < P.blocks.size(); ++b) {
+            const PBlock& B = P.blocks[b];
+            DBlock db{};
+            db.topo = B.topo;
+            db.nloc = (int)M.bvars[b].size();
+            db.nres = (int)B.res.size();
+            db.res0 = (int)res.size();
+            db.nsl = B.nsl;
+            db.site = B.site;
+            db.nden = (int)M.block_dens[b].size();
+            db.den0 = db.nden ? M.block_dens[b][0] : 0;
+            for (int i = 0; i < NLOC; ++i) db.vars[i] = i < db.nloc ? M.bvars[b][i] : -1;
+            for (size_t k = 1; k < M.block_dens[b].size(); ++k)
+                if (M.block_dens[b][k] != db.den0 + (int)k) throw std::runtime_error("denominators of a block are not contiguous");
+            blocks.push_back(db);
+            for (const PRes& r : B.res) {
+                DRes dr{};
+                dr.node = r.node;
+                dr.model = r.model;
+                dr.np = (int)r.params.size();
+                dr.nch = (int)r.ch.size() / 2;
+                if (dr.np > 6 || dr.nch > 4) throw std::runtime_error("too many lineshape parameters or channels");
+                for (int i = 0; i < dr.np; ++i) { dr.p[i] = r.params[i]; dr.slot[i] = r.slots[i]; }
+                for (int i = 0; i < 2 * dr.nch; ++i) dr.ch[i] = r.ch[i];
+                res.push_back(dr);
+            }
+        }
+        for (const DenSpec& D : M.dens) {
+            dens.push_back(DDen{D.block, D.res, D.coef_base, D.ncoef});
+            DDenTerms dt{(int)D.re.size(), (int)D.im.size(), (int)terms.size(), 0};
+            for (const Term& t : D.re) terms.push_back(DTerm{t.inv, t.coef, t.cval});
+            for (const Term& t : D.im) terms.push_back(DTerm{t.inv, t.coef, t.cval});
+            dterms.push_back(dt);
+        }
+        for (int a = 0; a < P.namp; ++a) {
+            const AmpSpec& A = M.amps[a];
+            amps.push_back(DAmp{A.block, A.slc, P.amap[a], A.coef_base, A.u, 0, P.aratio[a]});
+        }
+        DBlock* d_blocks = h->upload(blocks);
+        DRes* d_res = h->upload(res);
+        DNode* d_nodes2 = h->upload(nodes);
+        DTopo* d_topos2 = h->upload(topos);
+        int* d_slcL = h->upload(slcL);
+        int* d_slc0 = h->upload(slc0);
+        DDen* d_dens = h->upload(dens);
+        DAmp* d_amps = h->upload(amps);
+        DDenTerms* d_dterms = h->upload(dterms);
+        DTerm* d_terms = h->upload(terms);
+        int* d_gslot = h->upload(P.g_slot);
+        double* d_gmu = h->upload(P.g_mu);
+        double* d_gsig = h->upload(P.g_sig);
+        h->table = h->alloc<double>(M.table_size());
+        HIPCHECK(hipMemset(h->table, 0, M.table_size() * sizeof(double)));
+
+        setup_partition(h);
+        h->nacc = 1 + 2 * P.namp + P.ntheta;
+        h->part = h->alloc<double>((size_t)h->nacc * std::max(1, h->nwg));
+
+        h->prep = PrepArgs{d_blocks, d_res, d_nodes2, d_topos2, d_slcL, d_slc0, d_dens, d_amps,
+                           (int)dens.size(), P.namp, P.nfree, P.ntheta,
+                           (long long)M.off_v(), (long long)M.off_c(), (long long)M.off_rec()};
+        h->fin = FinArgs{};
+        h->fin.part = h->part;
+        h->fin.nwg = h->nwg;
+        h->fin.nacc = h->nacc;
+        for (int c = 0; c < 3; ++c) { h->fin.g0[c] = h->g0[c]; h->fin.g1[c] = h->g1[c]; }
+        h->fin.table = h->table;
+        h->fin.off_v = (long long)M.off_v();
+        h->fin.off_rec = (long long)M.off_rec();
+        h->fin.amps = d_amps;
+        h->fin.blocks = d_blocks;
+        h->fin.namp = P.namp;
+        h->fin.nfree = P.nfree;
+        h->fin.ntheta = P.ntheta;
+        h->fin.ngauss = (int)P.g_slot.size();
+        h->fin.g_slot = d_gslot;
+        h->fin.g_mu = d_gmu;
+        h->fin.g_sig = d_gsig;
+        h->fin.Np = (double)n_phsp;
+        h->fin.Wd = h->Wd;
+        h->fin.Wb = h->Wb;
+        h->fin.totals = nullptr;
+        h->aux = AuxArgs{h->sl, h->inv, h->ntot, h->table, (long long)M.off_c(), (long long)M.off_rec(),
+                         d_amps, d_blocks, d_dens, d_dterms, d_terms, P.namp, P.npol};
+
+        h->src = generate_eval_kernel(M);
+        h->fn_eval = compile_eval(device, h->src);
+        h->used_inv.assign(M.inv.size(), 0);
+        for (const DenSpec& D : M.dens) {
+            for (const Term& t : D.re) if (t.inv >= 0) h->used_inv[t.inv] = 1;
+            for (const Term& t : D.im) if (t.inv >= 0) h->used_inv[t.inv] = 1;
+        }
+        HIPCHECK(hipDeviceSynchronize());
+        (void)hipSetDevice(prev);
+        *out = h;
+        return 0;
+    } catch (const std::exception& e) {
+        delete h;
+        (void)hipSetDevice(prev);
+        return fail(std::string("ctpwa_create: ") + e.what());
+    }
+}
+
+extern "C" void ctpwa_destroy(ctpwa_handle* h) { delete h; }
+
+extern "C" int ctpwa_dims(const ctpwa_handle* h, int* n_amp, int* n_free, int* n_theta, int* n_pol) {
+    if (!h) return fail("ctpwa_dims: NULL handle");
+    if (n_amp) *n_amp = h->M.P.namp;
+    if (n_free) *n_free = h->M.P.nfree;
+    if (n_theta) *n_theta = h->M.P.ntheta;
+    if (n_pol) *n_pol = h->M.P.npol;
+    return 0;
+}
+
+extern "C" int ctpwa_param_info(const ctpwa_handle* h, double* init, double* lower, double* upper) {
+    if (!h) return fail("ctpwa_param_info: NULL handle");
+    const Plan& P = h->M.P;
+    for (int j = 0; j < P.ntheta; ++j) {
+        if (init) init[j] = P.th_init[j];
+        if (lower) lower[j] = P.th_lo[j];
+        if (upper) upper[j] = P.th_hi[j];
+    }
+    return 0;
+}
+
+static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s) {
+    void* args[] = {&h->table, &h->sl, &h->inv, &h->wts, &h->part, &h->wg_cls, &h->wg_rng, &h->ntot, &h->nwg,
+                    &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2]};
+    return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+}
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) { (void)hipGetDevice(&prev); (void)hipSetDevice(dev); }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double* out_dev, void* stream) {
+    if (!h || !params_dev || !out_dev) return fail("ctpwa_nll_grad: NULL argument");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, s);
+    if (e == hipSuccess) e = launch_eval(h, s);
+    if (e == hipSuccess) {
+        h->fin.params = params_dev;
+        h->fin.out = out_dev;
+        e = ctpwa_launch_finalize(&h->fin, s);
+    }
+    if (e != hipSuccess) return fail(std::string("ctpwa_nll_grad: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream) {
+    (void)params_dev; (void)hess_dev; (void)stream;
+    if (!h) return fail("ctpwa_hessian: NULL handle");
+    return fail("ctpwa_hessian: not available in this build");
+}
+
+static int aux_call(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream, int intensity) {
+    if (!h || !params_dev || !out_dev) return fail("NULL argument");
+    if (cls < 0 || cls > 2) return fail("event class must be 0, 1 or 2");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, s);
+    if (e == hipSuccess) e = ctpwa_launch_aux(&h->aux, h->off[cls], h->n[cls], out_dev, intensity, s);
+    if (e != hipSuccess) return fail(hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ctpwa_intensity(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
+    return aux_call(h, params_dev, cls, out_dev, stream, 1);
+}
+
+extern "C" int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
+    return aux_call(h, params_dev, cls, out_dev, stream, 0);
+}
+
+extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev, int iters, void* stream, double* avg_ms) {
+    if (!h || !params_dev || !avg_ms || iters <= 0) return fail("ctpwa_time_eval_kernel: bad argument");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, s);
+    if (e == hipSuccess) e = hipEventRecord(a, s);
+    for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_eval(h, s);
+    if (e == hipSuccess) e = hipEventRecord(b, s);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return fail(std::string("ctpwa_time_eval_kernel: ") + hipGetErrorString(e));
+    *avg_ms = ms / iters;
+    return 0;
+}
+
+extern "C" long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg) {
+    try {
+        Model_ M = derive(parse_plan(plan_text));
+        long long nt = n_phsp + n_data + n_bkg;
+        long long b = nt * (long long)M.us.size() * M.P.npol * 16 + nt * (long long)M.inv.size() * 8;
+        b += (long long)M.table_size() * 8 + (long long)(1 + 2 * M.P.namp + M.P.ntheta) * 1024 * 8;
+        b += std::min<long long>(std::max(n_phsp, std::max(n_data, n_bkg)), 1 << 20) * M.P.nfinal * 32;
+        return b;
+    } catch (const std::exception& e) {
+        g_err = std::string("ctpwa_footprint_bytes: ") + e.what();
+        return -1;
+    }
+}
+
+extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long long* bytes_per_eval, int* n_sl, int* n_inv,
+                            int* n_wg) {
+    if (!h) return fail("ctpwa_layout: NULL handle");
+    if (n_events3) for (int c = 0; c < 3; ++c) n_events3[c] = h->n[c];
+    int used = 0;
+    for (int u : h->used_inv) used += u;
+    if (bytes_per_eval) {
+        long long b = h->ntot * ((long long)h->M.us.size() * h->M.P.npol * 16 + (long long)used * 8);
+        for (int c = 1; c < 3; ++c) if (h->pw[c]) b += h->n[c] * 8;
+        *bytes_per_eval = b;
+    }
+    if (n_sl) *n_sl = (int)h->M.us.size();
+    if (n_inv) *n_inv = used;
+    if (n_wg) *n_wg = h->nwg;
+    return 0;
+}
+
+extern "C" const char* ctpwa_kernel_source(const ctpwa_handle* h) { return h ? h->src.c_str() : ""; }
+
+// source of the fused kernel for a plan, without a device (buf may be NULL to query the size)
+extern "C" long long ctpwa_generate_source(const char* plan_text, char* buf, long long cap) {
+    try {
+        std::string src = generate_eval_kernel(derive(parse_plan(plan_text)));
+        if (buf && cap > 0) {
+            long long n = std::min<long long>(cap - 1, (long long)src.size());
+            std::memcpy(buf, src.data(), (size_t)n);
+            buf[n] = '\0';
+        }
+        return (long long)src.size() + 1;
+    } catch (const std::exception& e) {
+        g_err = std::string("ctpwa_generate_source: ") + e.what();
+        return -1;
+    }
+}
+
+// compile the fused kernel for a plan with hipRTC (no device needed); returns the
+// code-object size in bytes, or -1 with the compiler log in ctpwa_last_error()
+extern "C" long long ctpwa_compile_check(const char* plan_text) {
+    try {
+        std::string src = generate_eval_kernel(derive(parse_plan(plan_text)));
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+            throw std::runtime_error("hiprtcCreateProgram failed");
+        const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast"};
+        hiprtcResult r = hiprtcCompileProgram(prog, 4, opts);
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0');
+        hiprtcGetProgramLog(prog, &log[0]);
+        size_t cs = 0;
+        if (r == HIPRTC_SUCCESS) hiprtcGetCodeSize(prog, &cs);
+        hiprtcDestroyProgram(&prog);
+        if (r != HIPRTC_SUCCESS) throw std::runtime_error(log);
+        return (long long)cs;
+    } catch (const std::exception& e) {
+        g_err = std::string("ctpwa_compile_check: ") + e.what();
+        return -1;
+    }
+}
+extern "C" const char* ctpwa_last_error(void) { return g_err.c_str(); }
+extern "C" const char* ctpwa_version(void) { return "ctpwa_amd 0.1.0 (gfx950)"; }

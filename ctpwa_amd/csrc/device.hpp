@@ -1,0 +1,106 @@
+// Plain-old-data tables shared by the host driver and the fixed (non-generated)
+// kernels, plus small device math helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ctpwa {
+
+constexpr int MAXNODE = 8;      // decay steps per chain
+constexpr int MAXPART = 16 + 1 + MAXNODE;
+constexpr int MAXKIN = 32;
+constexpr int MAXT = 64;        // entries of a contracted spin tensor
+constexpr int MAXPOL = 64;
+constexpr int NLOC = 8;         // free parameters per block
+constexpr int REC = 1 + NLOC + NLOC * NLOC;
+
+struct DNode {            // one decay step of a topology
+    int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin, kin;
+    double bf_d, mm, md1, md2;
+};
+struct DTopo {
+    int nn, node0;        // nodes [node0, node0 + nn)
+    int order[MAXNODE];   // nodes >= 1 in an order where daughters come first
+};
+struct DKin { uint32_t mset, d1set, d2set; int owner_topo, owner_node, pad; };
+struct DInv { int kin, kind, L, pad; double d, ma, mb; };
+struct DU {               // one SL tensor
+    int topo, mask, S[MAXNODE], L[MAXNODE], cg[MAXNODE];
+};
+struct DBlock { int topo, nloc, nres, res0, nsl, site, nden, den0; int vars[NLOC]; };
+struct DRes { int node, model, np, nch; double p[6]; int slot[6]; double ch[8]; };
+struct DDen { int block, res, rec, ncoef; };
+struct DAmp { int block, slc, freeidx, rec, u, pad; double ratio; };
+struct DTerm { int inv, coef; double cval; };   // inv < 0: 1; coef < 0: constant cval
+struct DDenTerms { int nre, nim, t0, pad; };      // terms [t0, t0+nre) real, then nim imaginary
+
+// Blatt-Weisskopf polynomial N_L(z), Bf(L,q,q0,d) = sqrt(N_L(q0 d)/N_L(q d))
+// (include/ResModel.cuh:184-211); N is 1 for L = 0 and for L > 6.
+__host__ __device__ inline double bf_poly(int L, double z) {
+    double z2 = z * z;
+    switch (L) {
+    case 1: return 1.0 + z2;
+    case 2: return 9.0 + z2 * (3.0 + z2);
+    case 3: return 225.0 + z2 * (45.0 + z2 * (6.0 + z2));
+    case 4: return 11025.0 + z2 * (1575.0 + z2 * (135.0 + z2 * (10.0 + z2)));
+    case 5: return 893025.0 + z2 * (99225.0 + z2 * (6300.0 + z2 * (315.0 + z2 * (15.0 + z2))));
+    case 6: return 540326025.0 + z2 * (6185025.0 + z2 * (363825.0 + z2 * (17325.0 + z2 * (630.0 + z2 * (21.0 + z2)))));
+    default: return 1.0;
+    }
+}
+
+// breakup momentum from invariant masses, 0 below threshold (src/AmpGen.cu:2733-2737)
+__host__ __device__ inline double breakup(double m, double m1, double m2) {
+    double q2 = (m * m - (m1 + m2) * (m1 + m2)) * (m * m - (m1 - m2) * (m1 - m2));
+    if (q2 <= 0.0) return 0.0;
+    return sqrt(q2) / (2.0 * m);
+}
+
+// argument blocks of the fixed kernels (prep.hip)
+struct PrepArgs {
+    const DBlock* blocks;
+    const DRes* res;
+    const DNode* nodes;      // flattened per topology
+    const DTopo* topos;
+    const int* slcL;         // [topo slc row][MAXNODE], row offset per topology in slc0
+    const int* slc0;         // per topology first row
+    const DDen* dens;
+    const DAmp* amps;
+    int nden, namp, nfree, ntheta;
+    long long off_v, off_c, off_rec;
+};
+
+struct FinArgs {
+    const double* part;      // [nacc][nwg]
+    int nwg, nacc;
+    int g0[3], g1[3];        // workgroup ranges of phsp, data, bkg
+    const double* table;
+    long long off_v, off_rec;
+    const DAmp* amps;
+    const DBlock* blocks;
+    int namp, nfree, ntheta, ngauss;
+    const int* g_slot;
+    const double* g_mu;
+    const double* g_sig;
+    double Np, Wd, Wb;
+    const double* params;
+    double* out;             // [1 + 2 nfree + ntheta]
+    double* totals;          // [3][nacc] scratch (kept for inspection)
+};
+
+struct AuxArgs {
+    const double2* sl;
+    const double* inv;
+    long long ld;
+    const double* table;
+    long long off_c, off_rec;
+    const DAmp* amps;
+    const DBlock* blocks;
+    const DDen* dens;
+    const DDenTerms* dterms;
+    const DTerm* terms;
+    int namp, npol;
+};
+
+}  // namespace ctpwa

@@ -1,0 +1,338 @@
+// Host-side model for the native likelihood path.
+//
+// The plan text (written by ctpwa_amd/model.py) describes the decay topologies, their
+// (S, L) waves, the resonance blocks, the amplitude order and the free-parameter
+// slots exactly as the reference enumerates them (src/Info.cu:43-357,
+// src/Analysis.cu:4481-4830, src/AmpGen.cu:2009-2242).  This header parses it and
+// derives the device-side layout:
+//
+//  * kinematic nodes   - distinct (mother, daughter1, daughter2) final-state subsets;
+//  * invariants        - event-only scalars the lineshape denominators need
+//                        (s = m^2, the BWR width factor q^(2L+1)/(m N_L(qd)), Flatte
+//                        phase-space factors, Gounaris-Sakurai terms);
+//  * SL tensors ("u")  - per (topology, SL combination, barrier mask): the spin
+//                        amplitude with the event-only part of every Blatt-Weisskopf
+//                        factor folded in;
+//  * denominators      - one per resonance propagator: D = sum_t c_t x_t - i sum_t d_t y_t,
+//                        with theta-only coefficients c_t, d_t and event invariants x, y;
+//  * amplitude numerators K_a(theta) (real) - the theta-only barrier parts.
+//
+// Every lineshape of the reference's built-in set factorises this way:
+// Bf(L,q,q0) = sqrt(N_L(q0 d)) / sqrt(N_L(q d)) (include/ResModel.cuh:184-211) and
+// m0 Gamma(m) of the BWR (include/ResModel.cuh:213-223) is a theta-only factor times
+// q^(2L+1) / (m N_L(q d)).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace ctpwa {
+
+enum Model { M_BWR = 0, M_BW = 1, M_ONE = 2, M_FLATTE = 3, M_GS = 4 };
+enum InvKind { INV_ONE = 0, INV_S = 1, INV_E = 2, INV_RHO_RE = 3, INV_RHO_IM = 4,
+               INV_GS_Q2H = 5, INV_GS_Q2 = 6, INV_GS_Q3M = 7 };
+
+struct PNode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md1, md2; };
+struct PTopo { std::vector<PNode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
+struct PRes { int node, model; std::vector<double> params; std::vector<int> slots; std::vector<double> ch; };
+struct PBlock { int topo, site, nsl; std::vector<PRes> res; };
+
+struct Plan {
+    int nfinal = 0, npol = 0;
+    std::vector<int> polmap;
+    std::vector<PTopo> topos;
+    std::vector<PBlock> blocks;
+    int ntheta = 0, namp = 0, nfree = 0;
+    std::vector<double> th_init, th_lo, th_hi;
+    std::vector<int> amap;
+    std::vector<double> aratio;
+    std::vector<int> g_slot;
+    std::vector<double> g_mu, g_sig;
+};
+
+inline Plan parse_plan(const std::string& text) {
+    Plan P;
+    std::istringstream in(text);
+    std::string line;
+    if (!std::getline(in, line) || line.rfind("CTPWA_PLAN", 0) != 0)
+        throw std::runtime_error("plan text does not start with CTPWA_PLAN");
+    bool ended = false;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        if (!(ls >> key)) continue;
+        if (key == "nfinal") ls >> P.nfinal;
+        else if (key == "npol") { ls >> P.npol; P.polmap.resize(P.npol); for (auto& x : P.polmap) ls >> x; }
+        else if (key == "ntopo" || key == "nblock") {}
+        else if (key == "topo") { P.topos.emplace_back(); }
+        else if (key == "node") {
+            if (P.topos.empty()) throw std::runtime_error("node before topo");
+            PNode n; int k;
+            ls >> k >> n.dj >> n.dj1 >> n.dj2 >> n.d1ref >> n.d2ref >> n.has_bf >> n.bf_d >> n.lmin >> n.mm >> n.md1 >> n.md2;
+            P.topos.back().nodes.push_back(n);
+        } else if (key == "slc") {
+            std::vector<std::pair<int, int>> c; int S, L;
+            while (ls >> S >> L) c.push_back({S, L});
+            P.topos.back().slc.push_back(c);
+        } else if (key == "block") {
+            PBlock b; int idx, nres;
+            ls >> idx >> b.topo >> b.site >> b.nsl >> nres;
+            P.blocks.push_back(b);
+        } else if (key == "res") {
+            if (P.blocks.empty()) throw std::runtime_error("res before block");
+            PRes r; int np, nch;
+            ls >> r.node >> r.model >> np;
+            r.params.resize(np); for (auto& x : r.params) ls >> x;
+            r.slots.resize(np); for (auto& x : r.slots) ls >> x;
+            ls >> nch; r.ch.resize(2 * nch); for (auto& x : r.ch) ls >> x;
+            P.blocks.back().res.push_back(r);
+        } else if (key == "ntheta") ls >> P.ntheta;
+        else if (key == "theta") { int j; double a, lo, hi; ls >> j >> a >> lo >> hi; P.th_init.push_back(a); P.th_lo.push_back(lo); P.th_hi.push_back(hi); }
+        else if (key == "namp") ls >> P.namp >> P.nfree;
+        else if (key == "amap") { P.amap.resize(P.namp); for (auto& x : P.amap) ls >> x; }
+        else if (key == "aratio") { P.aratio.resize(P.namp); for (auto& x : P.aratio) ls >> x; }
+        else if (key == "ngauss") {}
+        else if (key == "gauss") { int s; double mu, sg; ls >> s >> mu >> sg; P.g_slot.push_back(s); P.g_mu.push_back(mu); P.g_sig.push_back(sg); }
+        else if (key == "end") { ended = true; break; }
+        else throw std::runtime_error("unknown plan record '" + key + "'");
+    }
+    if (!ended) throw std::runtime_error("plan text is truncated (no 'end')");
+    if (P.nfinal <= 0 || P.nfinal > 16) throw std::runtime_error("plan: nfinal must be in [1, 16]");
+    if (P.npol <= 0) throw std::runtime_error("plan: npol must be positive");
+    if ((int)P.th_init.size() != P.ntheta) throw std::runtime_error("plan: theta count mismatch");
+    return P;
+}
+
+// ---------------------------------------------------------------------------
+// Derived layout
+// ---------------------------------------------------------------------------
+struct KinNode { uint32_t mset, d1set, d2set; };
+
+struct InvSpec {
+    int kin, kind, L;
+    double d, ma, mb;
+    bool operator<(const InvSpec& o) const {
+        return std::tie(kin, kind, L, d, ma, mb) < std::tie(o.kin, o.kind, o.L, o.d, o.ma, o.mb);
+    }
+};
+
+struct USpec { int topo, slc; uint32_t mask; };   // mask bit k: fold node k's barrier
+
+struct Term { int inv; int coef; double cval; };   // inv < 0: constant 1; coef < 0: constant cval
+struct DenSpec {
+    int block, res, node;
+    std::vector<Term> re, im;   // D = sum re - i sum im
+    int ncoef = 0;              // theta-dependent coefficients (slots in the table)
+    int coef_base = 0;          // index of the first coefficient record
+};
+
+struct AmpSpec { int block, slc, u; int coef_base; };
+
+struct Model_ {
+    Plan P;
+    std::vector<KinNode> kin;
+    std::vector<std::vector<int>> topo_kin;      // [topo][node] -> kin index
+    std::vector<InvSpec> inv;
+    std::vector<USpec> us;
+    std::vector<DenSpec> dens;
+    std::vector<std::vector<int>> block_dens;     // [block] -> den indices
+    std::vector<AmpSpec> amps;                    // [namp]
+    std::vector<std::vector<int>> bvars;          // [block] -> local var -> global slot
+    int maxloc = 0;
+    // coefficient records: each holds value, NLOC first and NLOC*NLOC second derivatives
+    static const int NLOC = 8;
+    static const int REC = 1 + NLOC + NLOC * NLOC;
+    int n_coef_rec = 0;                           // amplitude K records + denominator coefficient records
+    int amp_rec0 = 0;                             // first amplitude K record
+    // table layout (doubles): [v: 2 namp][c: 2 namp][records]
+    size_t off_v() const { return 0; }
+    size_t off_c() const { return 2 * (size_t)P.namp; }
+    size_t off_rec() const { return 4 * (size_t)P.namp; }
+    size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC; }
+};
+
+// final-state subset of a momentum reference
+inline uint32_t ref_set(const PTopo& T, int ref, int nfinal, int depth = 0) {
+    if (depth > 64) throw std::runtime_error("cyclic decay topology");
+    if (ref >= 0) return 1u << ref;
+    int k = -ref - 1;
+    if (k <= 0 || k >= (int)T.nodes.size()) throw std::runtime_error("bad node reference");
+    return ref_set(T, T.nodes[k].d1ref, nfinal, depth + 1) | ref_set(T, T.nodes[k].d2ref, nfinal, depth + 1);
+}
+
+inline Model_ derive(const Plan& P) {
+    Model_ M;
+    M.P = P;
+    uint32_t all = (P.nfinal >= 32) ? 0xffffffffu : ((1u << P.nfinal) - 1u);
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, int> kin_idx;
+    for (const PTopo& T : P.topos) {
+        std::vector<int> tk;
+        for (size_t k = 0; k < T.nodes.size(); ++k) {
+            const PNode& n = T.nodes[k];
+            uint32_t d1 = ref_set(T, n.d1ref, P.nfinal), d2 = ref_set(T, n.d2ref, P.nfinal);
+            uint32_t m = (k == 0) ? all : (d1 | d2);
+            auto key = std::make_tuple(m, d1, d2);
+            auto it = kin_idx.find(key);
+            if (it == kin_idx.end()) {
+                kin_idx[key] = (int)M.kin.size();
+                tk.push_back((int)M.kin.size());
+                M.kin.push_back(KinNode{m, d1, d2});
+            } else tk.push_back(it->second);
+            if (n.dj * n.dj1 * n.dj2 > 64) throw std::runtime_error("node tensor larger than 64 entries");
+        }
+        if (T.nodes.size() > 8) throw std::runtime_error("more than 8 decay steps in a chain");
+        // contracted-tensor sizes (labels: mother replaced by its two daughters, in order)
+        {
+            std::vector<int> shape = {T.nodes[0].dj, T.nodes[0].dj1, T.nodes[0].dj2};
+            std::vector<int> lab = {-1, T.nodes[0].d1ref, T.nodes[0].d2ref};
+            for (size_t k = 1; k < T.nodes.size(); ++k) {
+                int ci = -1;
+                for (size_t j = 0; j < lab.size(); ++j) if (lab[j] == -(int)k - 1) { ci = (int)j; break; }
+                if (ci < 0) throw std::runtime_error("decay step whose mother is not produced earlier in the chain");
+                shape.erase(shape.begin() + ci);
+                lab.erase(lab.begin() + ci);
+                shape.push_back(T.nodes[k].dj1); lab.push_back(T.nodes[k].d1ref);
+                shape.push_back(T.nodes[k].dj2); lab.push_back(T.nodes[k].d2ref);
+                int sz = 1;
+                for (int d : shape) sz *= d;
+                if (sz > 64) throw std::runtime_error("contracted spin tensor larger than 64 entries");
+            }
+            int sz = 1;
+            for (int d : shape) sz *= d;
+            for (int pm : P.polmap)
+                if (pm < 0 || pm >= sz) throw std::runtime_error("polarisation map index outside the spin tensor");
+        }
+        M.topo_kin.push_back(tk);
+    }
+    std::map<InvSpec, int> inv_idx;
+    auto add_inv = [&](InvSpec s) -> int {
+        if (s.kind == INV_ONE) return -1;          // the constant 1 needs no storage
+        auto it = inv_idx.find(s);
+        if (it != inv_idx.end()) return it->second;
+        int i = (int)M.inv.size();
+        inv_idx[s] = i;
+        M.inv.push_back(s);
+        return i;
+    };
+    std::map<std::tuple<int, int, uint32_t>, int> u_idx;
+    int rec = 0;
+    M.block_dens.resize(P.blocks.size());
+    M.amps.resize(P.namp);
+    for (size_t b = 0; b < P.blocks.size(); ++b) {
+        const PBlock& B = P.blocks[b];
+        const PTopo& T = P.topos[B.topo];
+        int nn = (int)T.nodes.size();
+        // local variables of the block
+        std::vector<int> vars;
+        for (const PRes& r : B.res)
+            for (int s : r.slots)
+                if (s >= 0 && std::find(vars.begin(), vars.end(), s) == vars.end()) vars.push_back(s);
+        if ((int)vars.size() > Model_::NLOC) throw std::runtime_error("more than 8 free parameters in one block");
+        M.maxloc = std::max(M.maxloc, (int)vars.size());
+        M.bvars.push_back(vars);
+        std::vector<int> res_at(nn, -1);
+        for (size_t r = 0; r < B.res.size(); ++r) {
+            int k = B.res[r].node;
+            if (k <= 0 || k >= nn) throw std::runtime_error("resonance is not attached to a decaying intermediate");
+            res_at[k] = (int)r;
+        }
+        // barrier mask of this block
+        uint32_t mask = 0;
+        for (int k = 0; k < nn; ++k) {
+            bool fold;
+            if (k == 0 || res_at[k] < 0) fold = T.nodes[k].has_bf;
+            else {
+                int mdl = B.res[res_at[k]].model;
+                fold = T.nodes[k].has_bf && mdl != M_BW;
+            }
+            if (fold) mask |= 1u << k;
+        }
+        // denominators (a block without waves contributes nothing but keeps its slots)
+        for (size_t r = 0; r < B.res.size() && B.nsl > 0; ++r) {
+            const PRes& R = B.res[r];
+            int k = R.node;
+            int kin = M.topo_kin[B.topo][k];
+            const PNode& n = T.nodes[k];
+            DenSpec D{(int)b, (int)r, k, {}, {}, 0, rec};
+            auto S = add_inv({kin, INV_S, 0, 0.0, 0.0, 0.0});
+            auto ONE = add_inv({kin, INV_ONE, 0, 0.0, 0.0, 0.0});
+            switch (R.model) {
+            case M_ONE: continue;
+            case M_BWR: {
+                if (R.params.size() < 2) throw std::runtime_error("BWR needs mass and width");
+                int E = add_inv({kin, INV_E, n.lmin, n.has_bf ? n.bf_d : 0.0, 0.0, 0.0});
+                D.re = {{ONE, 0, 0.0}, {S, -1, -1.0}};
+                D.im = {{E, 1, 0.0}};
+                D.ncoef = 2;
+                break;
+            }
+            case M_BW: {
+                if (R.params.size() < 2) throw std::runtime_error("BW needs mass and width");
+                D.re = {{ONE, 0, 0.0}, {S, -1, -1.0}};
+                D.im = {{ONE, 1, 0.0}};
+                D.ncoef = 2;
+                break;
+            }
+            case M_FLATTE: {
+                int nch = (int)R.ch.size() / 2;
+                if ((int)R.params.size() < 1 + nch) throw std::runtime_error("Flatte needs one coupling per channel");
+                D.re = {{ONE, 0, 0.0}, {S, -1, -1.0}};
+                for (int i = 0; i < nch; ++i) {
+                    int ri = add_inv({kin, INV_RHO_IM, 0, 0.0, R.ch[2 * i], R.ch[2 * i + 1]});
+                    int rr = add_inv({kin, INV_RHO_RE, 0, 0.0, R.ch[2 * i], R.ch[2 * i + 1]});
+                    D.re.push_back({ri, 1 + i, 0.0});
+                    D.im.push_back({rr, 1 + i, 0.0});
+                }
+                D.ncoef = 1 + nch;
+                break;
+            }
+            case M_GS: {
+                if (R.params.size() < 2) throw std::runtime_error("GS needs mass and width");
+                double mpi = R.ch.empty() ? 0.1396 : R.ch[0];
+                int Q2H = add_inv({kin, INV_GS_Q2H, 0, 0.0, mpi, 0.0});
+                int Q2 = add_inv({kin, INV_GS_Q2, 0, 0.0, 0.0, 0.0});
+                int Q3M = add_inv({kin, INV_GS_Q3M, 0, 0.0, 0.0, 0.0});
+                // D = A0 - C s + c1 q^2 h - c2 q^2 - i c1 q^3/m
+                D.re = {{ONE, 0, 0.0}, {S, 1, 0.0}, {Q2H, 2, 0.0}, {Q2, 3, 0.0}};
+                D.im = {{Q3M, 2, 0.0}};
+                D.ncoef = 4;
+                break;
+            }
+            default: throw std::runtime_error("lineshape model not supported");
+            }
+            D.coef_base = rec;
+            rec += D.ncoef;
+            M.block_dens[b].push_back((int)M.dens.size());
+            M.dens.push_back(D);
+        }
+        // amplitudes
+        for (int c = 0; c < B.nsl; ++c) {
+            auto key = std::make_tuple(B.topo, c, mask);
+            int u;
+            auto it = u_idx.find(key);
+            if (it == u_idx.end()) {
+                u = (int)M.us.size();
+                u_idx[key] = u;
+                M.us.push_back(USpec{B.topo, c, mask});
+            } else u = it->second;
+            int a = B.site + c;
+            if (a < 0 || a >= P.namp) throw std::runtime_error("amplitude index out of range");
+            M.amps[a] = AmpSpec{(int)b, c, u, 0};
+        }
+    }
+    // amplitude records follow the denominator records
+    M.amp_rec0 = rec;
+    for (int a = 0; a < P.namp; ++a) M.amps[a].coef_base = rec + a;
+    M.n_coef_rec = rec + P.namp;
+    if (P.topos.empty() || P.namp <= 0) throw std::runtime_error("plan has no amplitudes");
+    return M;
+}
+
+}  // namespace ctpwa

@@ -1,0 +1,408 @@
+// Per-evaluation theta-only work and the final assembly.
+//
+//  k_prep      params -> coefficient table: couplings v_a, c_a = v_a K_a, amplitude
+//              numerators K_a and every propagator-denominator coefficient, each with
+//              its first and second derivatives over the block's free parameters
+//              (forward-mode jets).  One thread per item; runs in microseconds.
+//  k_finalize  per-workgroup partial sums -> NLL and gradient in the fit-parameter
+//              layout [Re p, Im p, theta], following the assembly of the reference's
+//              NLLFunction::forward/backward (src/Analysis.cu:1134-1261): phase-space
+//              mean, N_eff log term, NaN/Inf -> 1e30 with zero gradient, Gaussian
+//              constraints (src/Parameters.cu:769-806).
+//  k_amplitudes / k_intensity   per-event amplitudes and intensities (diagnostics,
+//              event generation).
+#include "device.hpp"
+
+namespace ctpwa {
+
+// ---------------------------------------------------------------------------
+// forward-mode jet over NLOC block-local variables (value, gradient, Hessian)
+// ---------------------------------------------------------------------------
+struct DJ {
+    double v, g[NLOC], h[NLOC][NLOC];
+};
+__device__ inline DJ jc(double c) {
+    DJ r;
+    r.v = c;
+    for (int i = 0; i < NLOC; ++i) { r.g[i] = 0.0; for (int k = 0; k < NLOC; ++k) r.h[i][k] = 0.0; }
+    return r;
+}
+__device__ inline DJ jv(double c, int i) { DJ r = jc(c); if (i >= 0 && i < NLOC) r.g[i] = 1.0; return r; }
+__device__ inline DJ jadd(const DJ& a, const DJ& b) {
+    DJ r;
+    r.v = a.v + b.v;
+    for (int i = 0; i < NLOC; ++i) { r.g[i] = a.g[i] + b.g[i]; for (int k = 0; k < NLOC; ++k) r.h[i][k] = a.h[i][k] + b.h[i][k]; }
+    return r;
+}
+__device__ inline DJ jscale(const DJ& a, double c) {
+    DJ r;
+    r.v = a.v * c;
+    for (int i = 0; i < NLOC; ++i) { r.g[i] = a.g[i] * c; for (int k = 0; k < NLOC; ++k) r.h[i][k] = a.h[i][k] * c; }
+    return r;
+}
+__device__ inline DJ jsub(const DJ& a, const DJ& b) { return jadd(a, jscale(b, -1.0)); }
+__device__ inline DJ jmul(const DJ& a, const DJ& b) {
+    DJ r;
+    r.v = a.v * b.v;
+    for (int i = 0; i < NLOC; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+    for (int i = 0; i < NLOC; ++i)
+        for (int k = 0; k < NLOC; ++k)
+            r.h[i][k] = a.h[i][k] * b.v + a.v * b.h[i][k] + a.g[i] * b.g[k] + a.g[k] * b.g[i];
+    return r;
+}
+// f(a) given f, f', f''
+__device__ inline DJ jfun(const DJ& a, double f, double d1, double d2) {
+    DJ r;
+    r.v = f;
+    for (int i = 0; i < NLOC; ++i) r.g[i] = d1 * a.g[i];
+    for (int i = 0; i < NLOC; ++i)
+        for (int k = 0; k < NLOC; ++k) r.h[i][k] = d1 * a.h[i][k] + d2 * a.g[i] * a.g[k];
+    return r;
+}
+__device__ inline DJ jinv(const DJ& a) { double x = a.v; return jfun(a, 1.0 / x, -1.0 / (x * x), 2.0 / (x * x * x)); }
+__device__ inline DJ jdiv(const DJ& a, const DJ& b) { return jmul(a, jinv(b)); }
+__device__ inline DJ jsqrt(const DJ& a) { double s = sqrt(a.v); return jfun(a, s, 0.5 / s, -0.25 / (s * a.v)); }
+__device__ inline DJ jlog(const DJ& a) { return jfun(a, log(a.v), 1.0 / a.v, -1.0 / (a.v * a.v)); }
+__device__ inline DJ jpowi(const DJ& a, int n) {
+    if (n == 0) return jc(1.0);
+    double x = a.v;
+    double xn2 = 1.0;
+    for (int i = 0; i < n - 2; ++i) xn2 *= x;
+    double xn1 = (n >= 2) ? xn2 * x : 1.0;
+    double xn = (n >= 1) ? xn1 * x : 1.0;
+    if (n == 1) return a;
+    return jfun(a, xn, n * xn1, n * (n - 1.0) * xn2);
+}
+// N_L(z) with z = q0 d as a jet in q0
+__device__ inline DJ jbfpoly(int L, const DJ& q0, double d) {
+    DJ z = jscale(q0, d);
+    DJ z2 = jmul(z, z);
+    double c[7];
+    int deg = 0;
+    switch (L) {
+    case 1: c[0] = 1; c[1] = 1; deg = 1; break;
+    case 2: c[0] = 9; c[1] = 3; c[2] = 1; deg = 2; break;
+    case 3: c[0] = 225; c[1] = 45; c[2] = 6; c[3] = 1; deg = 3; break;
+    case 4: c[0] = 11025; c[1] = 1575; c[2] = 135; c[3] = 10; c[4] = 1; deg = 4; break;
+    case 5: c[0] = 893025; c[1] = 99225; c[2] = 6300; c[3] = 315; c[4] = 15; c[5] = 1; deg = 5; break;
+    case 6: c[0] = 540326025; c[1] = 6185025; c[2] = 363825; c[3] = 17325; c[4] = 630; c[5] = 21; c[6] = 1; deg = 6; break;
+    default: return jc(1.0);
+    }
+    DJ r = jc(c[deg]);                       // Horner in z^2
+    for (int i = deg - 1; i >= 0; --i) r = jadd(jmul(r, z2), jc(c[i]));
+    return r;
+}
+// breakup momentum (src/AmpGen.cu:2733-2737) and computeQ0AD (include/ResModel.cuh:88-97)
+__device__ inline DJ jbreakup(const DJ& m, const DJ& m1, const DJ& m2) {
+    DJ s = jadd(m1, m2), d = jsub(m1, m2);
+    DJ mm = jmul(m, m);
+    DJ q2 = jmul(jsub(mm, jmul(s, s)), jsub(mm, jmul(d, d)));
+    if (q2.v <= 0.0) return jc(0.0);
+    return jdiv(jsqrt(q2), jscale(m, 2.0));
+}
+__device__ inline DJ jq0ad(const DJ& m0, const DJ& m1, const DJ& m2) {
+    DJ s = jadd(m1, m2), d = jsub(m1, m2);
+    DJ m0s = jmul(m0, m0);
+    DJ q2 = jdiv(jmul(jsub(m0s, jmul(s, s)), jsub(m0s, jmul(d, d))), jscale(m0s, 4.0));
+    if (q2.v < 0.0) return jc(0.0);
+    return jsqrt(q2);
+}
+
+
+__device__ inline int local_index(const DBlock& B, int slot) {
+    for (int i = 0; i < B.nloc; ++i) if (B.vars[i] == slot) return i;
+    return -1;
+}
+__device__ inline DJ res_param(const DRes& R, int p, const DBlock& B, const double* theta) {
+    if (p >= R.np) return jc(0.0);
+    int s = R.slot[p];
+    if (s < 0) return jc(R.p[p]);
+    return jv(theta[s], local_index(B, s));
+}
+// mass used in q0 for a daughter: configured mass, else the mass parameter of the
+// block's resonance that occupies it (src/AmpGen.cu:1664-1731)
+__device__ inline DJ q0_dmass(double cfg, int ref, const DBlock& B, const DRes* res, const double* theta) {
+    if (cfg > 0) return jc(cfg);
+    if (ref < 0) {
+        int kk = -ref - 1;
+        for (int r = 0; r < B.nres; ++r) {
+            const DRes& R = res[B.res0 + r];
+            if (R.node == kk && R.np > 0) return res_param(R, 0, B, theta);
+        }
+    }
+    return jc(1.0);   // unreachable: the host rejects event-dependent q0
+}
+__device__ inline void store(double* t, const DJ& x) {
+    t[0] = x.v;
+    for (int i = 0; i < NLOC; ++i) t[1 + i] = x.g[i];
+    for (int i = 0; i < NLOC; ++i)
+        for (int k = 0; k < NLOC; ++k) t[1 + NLOC + i * NLOC + k] = x.h[i][k];
+}
+
+__global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
+    int item = blockIdx.x * blockDim.x + threadIdx.x;
+    const double* theta = params + 2 * a.nfree;
+    if (item < a.nden) {
+        const DDen& D = a.dens[item];
+        const DBlock& B = a.blocks[D.block];
+        const DRes& R = a.res[B.res0 + D.res];
+        const DTopo& T = a.topos[B.topo];
+        const DNode& n = a.nodes[T.node0 + R.node];
+        DJ m0 = (R.np > 0) ? res_param(R, 0, B, theta) : jc(1.0);
+        DJ q0 = jq0ad(m0, q0_dmass(n.md1, n.d1ref, B, a.res, theta), q0_dmass(n.md2, n.d2ref, B, a.res, theta));
+        double* t = table + a.off_rec + (long long)D.rec * REC;
+        DJ m02 = jmul(m0, m0);
+        switch (R.model) {
+        case 0: {   // BWR: D = m0^2 - s - i g0 m0^2 N_L(q0 d')/q0^(2L+1) * E
+            DJ g0 = res_param(R, 1, B, theta);
+            double dp = n.has_bf ? n.bf_d : 0.0;
+            DJ N0 = (dp != 0.0) ? jbfpoly(n.lmin, q0, dp) : jc(1.0);
+            DJ b = jdiv(jmul(jmul(g0, m02), N0), jpowi(q0, 2 * n.lmin + 1));
+            store(t, m02);
+            store(t + REC, b);
+            break;
+        }
+        case 1: {   // BW: D = m0^2 - s - i m0 g0
+            DJ g0 = res_param(R, 1, B, theta);
+            store(t, m02);
+            store(t + REC, jmul(m0, g0));
+            break;
+        }
+        case 3: {   // Flatte: D = m0^2 - s + sum b_i rho_im,i - i sum b_i rho_re,i, b_i = m0 g_i / 2
+            store(t, m02);
+            for (int i = 0; i < R.nch; ++i) store(t + (1 + i) * REC, jscale(jmul(m0, res_param(R, 1 + i, B, theta)), 0.5));
+            break;
+        }
+        case 4: {   // Gounaris-Sakurai (src/ResModel.cu:484-561)
+            const double pi = 3.14159265358979323846;
+            DJ g0 = res_param(R, 1, B, theta);
+            double mpi = R.nch > 0 ? R.ch[0] : 0.1396;
+            DJ q02 = jmul(q0, q0), q03 = jmul(q02, q0);
+            DJ logA = jlog(jscale(jadd(m0, jscale(q0, 2.0)), 1.0 / (2.0 * mpi)));
+            DJ h0 = jmul(jscale(jdiv(q0, m0), 2.0 / pi), logA);
+            DJ dh0 = jadd(jmul(h0, jsub(jinv(jscale(q02, 8.0)), jinv(jscale(m02, 2.0)))), jinv(jscale(m02, 2.0 * pi)));
+            DJ c1 = jdiv(jmul(g0, m02), q03);
+            DJ c3 = jdiv(jmul(jmul(g0, m02), dh0), q0);
+            DJ one_c3 = jadd(jc(1.0), c3);
+            store(t, jmul(one_c3, m02));
+            store(t + REC, jscale(one_c3, -1.0));
+            store(t + 2 * REC, c1);
+            store(t + 3 * REC, jscale(jmul(c1, h0), -1.0));
+            break;
+        }
+        default: break;
+        }
+        return;
+    }
+    item -= a.nden;
+    if (item >= a.namp) return;
+    const DAmp& A = a.amps[item];
+    const DBlock& B = a.blocks[A.block];
+    const DTopo& T = a.topos[B.topo];
+    const int* Ls = a.slcL + (long long)(a.slc0[B.topo] + A.slc) * MAXNODE;
+    DJ K = jc(1.0);
+    for (int k = 0; k < T.nn; ++k) {
+        const DNode& n = a.nodes[T.node0 + k];
+        int L = Ls[k];
+        int r = -1;
+        for (int i = 0; i < B.nres; ++i) if (a.res[B.res0 + i].node == k) { r = i; break; }
+        DJ m1 = q0_dmass(n.md1, n.d1ref, B, a.res, theta), m2 = q0_dmass(n.md2, n.d2ref, B, a.res, theta);
+        if (k == 0 || r < 0) {
+            if (n.has_bf && L >= 1 && L <= 6) {
+                DJ q0 = jbreakup(jc(n.mm > 0 ? n.mm : 1.0), m1, m2);
+                K = jmul(K, jsqrt(jbfpoly(L, q0, n.bf_d)));
+            }
+            continue;
+        }
+        const DRes& R = a.res[B.res0 + r];
+        if (R.model == 1) continue;                        // BW carries no barrier
+        DJ m0 = (R.np > 0) ? res_param(R, 0, B, theta) : jc(1.0);
+        DJ q0 = jq0ad(m0, m1, m2);
+        if (n.has_bf && L >= 1 && L <= 6) K = jmul(K, jsqrt(jbfpoly(L, q0, n.bf_d)));
+        if (R.model == 4) {                                // GS numerator 1 + d(m0) g0 / m0
+            const double pi = 3.14159265358979323846;
+            DJ g0 = res_param(R, 1, B, theta);
+            double mpi = R.nch > 0 ? R.ch[0] : 0.1396;
+            DJ q02 = jmul(q0, q0), q03 = jmul(q02, q0);
+            DJ logA = jlog(jscale(jadd(m0, jscale(q0, 2.0)), 1.0 / (2.0 * mpi)));
+            DJ dv = jadd(jsub(jmul(jscale(jdiv(jc(mpi * mpi), q02), 3.0 / pi), logA), jmul(jc(mpi * mpi / pi), jdiv(m0, q03))),
+                         jdiv(m0, jscale(q0, 2.0 * pi)));
+            K = jmul(K, jadd(jc(1.0), jdiv(jmul(dv, g0), m0)));
+        }
+    }
+    store(table + a.off_rec + (long long)A.rec * REC, K);
+    double pr = params[A.freeidx] * A.ratio, pi_ = params[a.nfree + A.freeidx] * A.ratio;
+    table[a.off_v + 2 * item] = pr;
+    table[a.off_v + 2 * item + 1] = pi_;
+    table[a.off_c + 2 * item] = pr * K.v;
+    table[a.off_c + 2 * item + 1] = pi_ * K.v;
+}
+
+// ---------------------------------------------------------------------------
+// final assembly
+// ---------------------------------------------------------------------------
+
+constexpr int FIN_THREADS = 1024;
+
+__global__ void __launch_bounds__(FIN_THREADS) k_finalize(FinArgs f) {
+    extern __shared__ double sh[];          // [3 * nacc] totals + [namp * 2] X + flags
+    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = FIN_THREADS / 64;
+    // 1. class totals, one wave per (class, column), fixed summation order
+    for (int pair = wave; pair < 3 * f.nacc; pair += nwave) {
+        int c = pair / f.nacc, col = pair - c * f.nacc;
+        double s = 0.0;
+        for (int g = f.g0[c] + lane; g < f.g1[c]; g += 64) s += f.part[(long long)col * f.nwg + g];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) sh[pair] = s;
+    }
+    __syncthreads();
+    double* tot = sh;                        // tot[c * nacc + col]
+    double* X = sh + 3 * f.nacc;             // combined X_a (re, im)
+    double Np = f.Np, Neff = f.Wd - f.Wb;
+    double pf = (Np > 0) ? tot[0] / Np : 0.0;
+    if (!(pf > 1e-30) || isinf(pf)) pf = 1e-30;
+    double cp = (Np > 0) ? Neff / (pf * Np) : 0.0;
+    const double* theta = f.params + 2 * f.nfree;
+    double loss = -tot[f.nacc] + tot[2 * f.nacc] + Neff * log(pf);
+    for (int g = 0; g < f.ngauss; ++g) {
+        double d = theta[f.g_slot[g]] - f.g_mu[g];
+        loss += d * d / (2.0 * f.g_sig[g] * f.g_sig[g]);
+    }
+    bool reset = isnan(loss) || isinf(loss);
+    // 2. X_a = -data + bkg + cp * phsp   (column 1 + 2a is Re, 2 + 2a is Im)
+    for (int a = tid; a < f.namp; a += FIN_THREADS) {
+        for (int q = 0; q < 2; ++q) {
+            int col = 1 + 2 * a + q;
+            X[2 * a + q] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
+        }
+    }
+    __syncthreads();
+    double* out = f.out;
+    if (tid == 0) out[0] = reset ? 1e30 : loss;
+    // 3. coupling gradient: dL/dp_f* = sum_a r_a K_a X_a ; output [2 Re, 2 Im]
+    for (int fi = tid; fi < f.nfree; fi += FIN_THREADS) {
+        double gr = 0.0, gi = 0.0;
+        for (int a = 0; a < f.namp; ++a) {
+            const DAmp& A = f.amps[a];
+            if (A.freeidx != fi) continue;
+            double K = f.table[f.off_rec + (long long)A.rec * REC];
+            gr += A.ratio * K * X[2 * a];
+            gi += A.ratio * K * X[2 * a + 1];
+        }
+        out[1 + fi] = reset ? 0.0 : 2.0 * gr;
+        out[1 + f.nfree + fi] = reset ? 0.0 : 2.0 * gi;
+    }
+    // 4. theta gradient: kernel-accumulated propagator part + numerator part
+    //    2 Re sum_a v_a dK_a/dtheta conj(X_a)
+    for (int j = tid; j < f.ntheta; j += FIN_THREADS) {
+        int col = 1 + 2 * f.namp + j;
+        double g = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
+        for (int a = 0; a < f.namp; ++a) {
+            const DAmp& A = f.amps[a];
+            const DBlock& B = f.blocks[A.block];
+            int li = -1;
+            for (int i = 0; i < B.nloc; ++i) if (B.vars[i] == j) { li = i; break; }
+            if (li < 0) continue;
+            double dK = f.table[f.off_rec + (long long)A.rec * REC + 1 + li];
+            double vr = f.table[f.off_v + 2 * a], vi = f.table[f.off_v + 2 * a + 1];
+            // Re(v conj(X)) = vr Xr + vi Xi
+            g += 2.0 * dK * (vr * X[2 * a] + vi * X[2 * a + 1]);
+        }
+        for (int q = 0; q < f.ngauss; ++q)
+            if (f.g_slot[q] == j) g += (theta[j] - f.g_mu[q]) / (f.g_sig[q] * f.g_sig[q]);
+        out[1 + 2 * f.nfree + j] = reset ? 0.0 : g;
+    }
+    if (f.totals) for (int i = tid; i < 3 * f.nacc; i += FIN_THREADS) f.totals[i] = tot[i];
+}
+
+// ---------------------------------------------------------------------------
+// generic per-event amplitudes and intensities
+// ---------------------------------------------------------------------------
+
+__device__ inline double2 block_psi(const AuxArgs& a, const DBlock& B, long long e) {
+    double pr = 1.0, pi = 0.0;
+    for (int d = 0; d < B.nden; ++d) {
+        const DDen& D = a.dens[B.den0 + d];
+        const DDenTerms& DT = a.dterms[B.den0 + d];
+        double dr = 0.0, di = 0.0;
+        for (int t = 0; t < DT.nre + DT.nim; ++t) {
+            const DTerm& T = a.terms[DT.t0 + t];
+            double c = (T.coef < 0) ? T.cval : a.table[a.off_rec + (long long)(D.rec + T.coef) * REC];
+            double x = (T.inv < 0) ? 1.0 : a.inv[(long long)T.inv * a.ld + e];
+            if (t < DT.nre) dr += c * x; else di += c * x;
+        }
+        double q = 1.0 / (dr * dr + di * di);
+        double xr = dr * q, xi = di * q;                  // 1 / (dr - i di)
+        double nr = pr * xr - pi * xi, ni = pr * xi + pi * xr;
+        pr = nr; pi = ni;
+    }
+    return make_double2(pr, pi);
+}
+
+// A[e][p][a] = K_a Psi_b(e) sl_u,p(e)  (interleaved complex), events [e0, e0 + n)
+__global__ void k_amplitudes(AuxArgs a, long long e0, long long n, double* __restrict__ out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long e = e0 + i;
+    for (int am = 0; am < a.namp; ++am) {
+        const DAmp& A = a.amps[am];
+        const DBlock& B = a.blocks[A.block];
+        double2 psi = block_psi(a, B, e);
+        double K = a.table[a.off_rec + (long long)A.rec * REC];
+        for (int p = 0; p < a.npol; ++p) {
+            double2 s = a.sl[((long long)A.u * a.npol + p) * a.ld + e];
+            double r = K * (psi.x * s.x - psi.y * s.y), im = K * (psi.x * s.y + psi.y * s.x);
+            long long o = ((i * a.npol + p) * a.namp + am) * 2;
+            out[o] = r;
+            out[o + 1] = im;
+        }
+    }
+}
+
+__global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __restrict__ out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long e = e0 + i;
+    double Sr[MAXPOL], Si[MAXPOL];
+    for (int p = 0; p < a.npol; ++p) { Sr[p] = 0.0; Si[p] = 0.0; }
+    for (int am = 0; am < a.namp; ++am) {
+        const DAmp& A = a.amps[am];
+        double2 psi = block_psi(a, a.blocks[A.block], e);
+        double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
+        double yr = cr * psi.x - ci * psi.y, yi = cr * psi.y + ci * psi.x;
+        for (int p = 0; p < a.npol; ++p) {
+            double2 s = a.sl[((long long)A.u * a.npol + p) * a.ld + e];
+            Sr[p] += yr * s.x - yi * s.y;
+            Si[p] += yr * s.y + yi * s.x;
+        }
+    }
+    double I = 0.0;
+    for (int p = 0; p < a.npol; ++p) I += Sr[p] * Sr[p] + Si[p] * Si[p];
+    out[i] = I;
+}
+
+}  // namespace ctpwa
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep(const double* params, double* table, const ctpwa::PrepArgs* a,
+                                         hipStream_t stream) {
+    int items = a->nden + a->namp;
+    int bs = 64;
+    hipLaunchKernelGGL(ctpwa::k_prep, dim3((items + bs - 1) / bs), dim3(bs), 0, stream, params, table, *a);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs* f, hipStream_t stream) {
+    size_t shmem = (size_t)(3 * f->nacc + 2 * f->namp) * sizeof(double);
+    hipLaunchKernelGGL(ctpwa::k_finalize, dim3(1), dim3(ctpwa::FIN_THREADS), shmem, stream, *f);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs* a, long long e0, long long n, double* out, int intensity,
+                                        hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    int bs = 128;
+    long long nb = (n + bs - 1) / bs;
+    if (intensity) hipLaunchKernelGGL(ctpwa::k_intensity, dim3((unsigned)nb), dim3(bs), 0, stream, *a, e0, n, out);
+    else hipLaunchKernelGGL(ctpwa::k_amplitudes, dim3((unsigned)nb), dim3(bs), 0, stream, *a, e0, n, out);
+    return hipGetLastError();
+}

@@ -1,0 +1,449 @@
+"""Amplitude enumeration: config -> model plan.
+
+The plan is the host-side description of the likelihood: decay topologies with their
+(S, L) partial waves, resonance blocks, the amplitude order, the free resonance
+parameter slots (theta) and the coupling layout.  It follows the reference's
+enumeration order exactly, because the amplitude order and the parameter layout are
+part of the drop-in contract:
+
+* (S, L) rules: ``Amp2BD::ComSL``                 src/AmpGen.cu:69-129
+* SL combinations (last step fastest):            src/AmpGen.cu:489-508
+* width convention Lmin per node:                 src/AmpGen.cu:443-454
+* chains x J^P combos x resonance combos x SL:    src/Info.cu:43-357,
+                                                  src/Analysis.cu:4481-4830
+* theta slot registration and sharing by name:    src/AmpGen.cu:2009-2242
+* polarisation map (config-particle C order):     src/Analysis.cu:4249-4322
+* vspace coupling layout (fit mode 1):            src/Analysis.cu:4382-4478,
+                                                  src/Parameters.cu:586-731
+
+The plan is handed to the native library (and to the CPU oracle) as a small
+line-oriented text document, see :meth:`Plan.to_text`.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+from .config import ConfigError, ConfigParser, Particle, ResonanceConfig
+
+# lineshape model codes shared with the native library
+MODEL_CODES = {"BWR": 0, "BW": 1, "ONE": 2, "Flatte": 3, "GS": 4}
+# ordered parameter names per model (src/Resonance.cu:14-75)
+_MODEL_PARAM_SPECS = {
+    "BWR": [("mass", False), ("width", False), ("r", True)],
+    "BW": [("mass", False), ("width", False)],
+    "ONE": [("mass", False)],
+    "Flatte": [("mass", False)],
+    "GS": [("mass", False), ("width", False)],
+}
+_EXTRA_PREFIX = {"Flatte": "g"}
+
+
+def canonical_model(name: str) -> str:
+    for k in MODEL_CODES:
+        if k.lower() == name.lower():
+            return k
+    raise ConfigError("lineshape model '%s' is not supported by this build" % name)
+
+
+def ordered_params(model: str, values: List[float]) -> Tuple[List[float], List[str]]:
+    """Resonance::setParamsFromModel (src/Resonance.cu:355-395)."""
+    specs = _MODEL_PARAM_SPECS[model]
+    required = sum(1 for _, opt in specs if not opt)
+    if len(values) < required:
+        raise ConfigError("model '%s' requires at least %d parameter(s), got %d"
+                          % (model, required, len(values)))
+    names, vals = [], []
+    for i, (nm, _) in enumerate(specs):
+        if i < len(values):
+            names.append(nm)
+            vals.append(float(values[i]))
+    pref = _EXTRA_PREFIX.get(model, "")
+    if pref:
+        for i in range(len(specs), len(values)):
+            names.append("%s%d" % (pref, i))
+            vals.append(float(values[i]))
+    return vals, names
+
+
+def com_sl(spins, parities, p_break=False, identical=False, boson=True, maxL=-1,
+           sl_filter=()) -> List[Tuple[int, int]]:
+    """Allowed (2S+1, L) for mother -> d1 d2, in the reference's order (src/AmpGen.cu:69-129)."""
+    tj1, tj2, tj3 = spins[0] - 1, spins[1] - 1, spins[2] - 1
+    p1, p2, p3 = parities
+    out = []
+    for tS in range(abs(tj2 - tj3), tj2 + tj3 + 1, 2):
+        for tL in range(abs(tj1 - tS), tj1 + tS + 1, 2):
+            L = tL // 2
+            sign = 1 if L % 2 == 0 else -1
+            if not (p_break or p1 == p2 * p3 * sign):
+                continue
+            if identical:
+                S = tS // 2
+                if (1 if (L + S) % 2 == 0 else -1) != (1 if boson else -1):
+                    continue
+            if maxL > 0 and L > maxL:
+                continue
+            if sl_filter and (tS + 1, L) not in set(sl_filter):
+                continue
+            out.append((tS + 1, L))
+    return out
+
+
+@dataclass
+class Node:
+    mother: str
+    d1: str
+    d2: str
+    spins: Tuple[int, int, int]
+    parities: Tuple[int, int, int]
+    sl_list: List[Tuple[int, int]]
+    has_bf: bool
+    bf_d: float
+    masses: Tuple[float, float, float]   # config mass or -1 (intermediates)
+    d1_ref: int = 0                      # >=0 final index; -(k+1): mother of node k
+    d2_ref: int = 0
+    lmin: int = 0
+
+
+@dataclass
+class Topology:
+    chain: str
+    comb: List[Tuple[str, int, int]]     # (intermediate, 2J+1, P)
+    nodes: List[Node]
+    sl_combos: List[Tuple[Tuple[int, int], ...]]
+
+
+@dataclass
+class ResInst:
+    name: str
+    tag: str                 # intermediate name it occupies
+    model: str
+    params: List[float]
+    param_names: List[str]
+    channels: List[Tuple[float, float]]
+    node: int                # node of the topology whose mother is ``tag``
+    slots: List[int] = field(default_factory=list)   # theta slot per param or -1
+
+
+@dataclass
+class Block:
+    topo: int
+    site: int
+    nsl: int
+    res: List[ResInst]
+    chain_key: str
+
+
+@dataclass
+class ThetaSlot:
+    name: str
+    res_name: str
+    param_idx: int
+    init: float
+    lower: float
+    upper: float
+
+
+@dataclass
+class Plan:
+    particles: List[Particle]
+    final_order: List[str]
+    initial: str
+    npol: int
+    polmap: List[int]
+    topologies: List[Topology]
+    blocks: List[Block]
+    n_amp: int
+    amp_names: List[str]
+    theta: List[ThetaSlot]
+    amp_free: List[int]          # amplitude -> free coupling index
+    amp_ratio: List[float]       # amplitude -> real ratio (trans folding)
+    n_free: int
+    param_names: List[str]
+    gauss: List[Tuple[int, float, float]] = field(default_factory=list)  # (slot, mu, sigma)
+
+    @property
+    def n_theta(self) -> int:
+        return len(self.theta)
+
+    def n_params(self) -> int:
+        return 2 * self.n_free + self.n_theta
+
+    def to_text(self) -> str:
+        """Serialise for the native library and the oracle (one record per line)."""
+        out = ["CTPWA_PLAN 1"]
+        out.append("nfinal %d" % len(self.final_order))
+        out.append("npol %d %s" % (self.npol, " ".join(str(x) for x in self.polmap)))
+        out.append("ntopo %d" % len(self.topologies))
+        for t, topo in enumerate(self.topologies):
+            out.append("topo %d %d %d" % (t, len(topo.nodes), len(topo.sl_combos)))
+            for k, nd in enumerate(topo.nodes):
+                out.append("node %d %d %d %d %d %d %d %r %d %r %r %r" % (
+                    k, nd.spins[0], nd.spins[1], nd.spins[2], nd.d1_ref, nd.d2_ref,
+                    1 if nd.has_bf else 0, float(nd.bf_d), nd.lmin,
+                    float(nd.masses[0]), float(nd.masses[1]), float(nd.masses[2])))
+            for c in topo.sl_combos:
+                out.append("slc " + " ".join("%d %d" % sl for sl in c))
+        out.append("nblock %d" % len(self.blocks))
+        for b, blk in enumerate(self.blocks):
+            out.append("block %d %d %d %d %d" % (b, blk.topo, blk.site, blk.nsl, len(blk.res)))
+            for r in blk.res:
+                toks = ["res", str(r.node), str(MODEL_CODES[r.model]), str(len(r.params))]
+                toks += [repr(float(x)) for x in r.params]
+                toks += [str(s) for s in r.slots]
+                toks.append(str(len(r.channels)))
+                for ch in r.channels:
+                    toks += [repr(float(ch[0])), repr(float(ch[1]))]
+                out.append(" ".join(toks))
+        out.append("ntheta %d" % len(self.theta))
+        for j, th in enumerate(self.theta):
+            out.append("theta %d %r %r %r" % (j, th.init, th.lower, th.upper))
+        out.append("namp %d %d" % (self.n_amp, self.n_free))
+        out.append("amap " + " ".join(str(x) for x in self.amp_free))
+        out.append("aratio " + " ".join(repr(float(x)) for x in self.amp_ratio))
+        out.append("ngauss %d" % len(self.gauss))
+        for (s, mu, sig) in self.gauss:
+            out.append("gauss %d %r %r" % (s, mu, sig))
+        out.append("end")
+        return "\n".join(out) + "\n"
+
+
+def _polarization(particles: List[Particle]) -> Tuple[int, List[int]]:
+    """initializePolarization (src/Analysis.cu:4249-4322).  Returns (nPol, map)."""
+    n = len(particles)
+    dims = [p.spin for p in particles]
+    keep = []
+    any_masked = False
+    for p in particles:
+        if p.is_polarized():
+            any_masked = True
+            keep.append([(p.spin - 1 - tm) // 2 for tm in p.polarization_2m])
+        else:
+            keep.append([])
+    orig = [1] * n
+    for i in range(n - 2, -1, -1):
+        orig[i] = orig[i + 1] * dims[i + 1]
+    eff = [len(k) if k else d for k, d in zip(keep, dims)]
+    masked = [1] * n
+    for i in range(n - 2, -1, -1):
+        masked[i] = masked[i + 1] * eff[i + 1]
+    npol = 1
+    for d in eff:
+        npol *= d
+    if not any_masked:
+        return npol, list(range(npol))
+    pm = []
+    for out in range(npol):
+        rem, idx = out, 0
+        for p in range(n):
+            j = rem // masked[p]
+            rem %= masked[p]
+            i = keep[p][j] if keep[p] else j
+            idx += i * orig[p]
+        pm.append(idx)
+    return npol, pm
+
+
+def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
+    if fit_mode != 1:
+        raise ConfigError("only fit mode 1 (vspace) is built; chain x step mode is not")
+    particles = cfg.particles
+    pmap = {p.name: p for p in particles}
+    final_order = list(cfg.data_order)
+    if not final_order:
+        raise ConfigError("Data.order is required")
+    for nm in final_order:
+        if nm not in pmap:
+            raise ConfigError("Data.order particle '%s' is not in Particles" % nm)
+    for p in particles:
+        if p.identical_group:
+            raise ConfigError("identical-particle symmetrisation is not built")
+    if any(c.type == "trans" for c in cfg.constraints):
+        raise ConfigError("trans constraints are not built")
+    if cfg.var_equal:
+        raise ConfigError("var_equal is not built")
+    npol, polmap = _polarization(particles)
+
+    topologies: List[Topology] = []
+    blocks: List[Block] = []
+    theta: List[ThetaSlot] = []
+    owners: Dict[str, Tuple[int, int]] = {}
+    amp_names: List[str] = []
+    chain_keys: List[str] = []
+    amp_chain_key: List[str] = []
+    gls = 0
+    initial: Optional[str] = None
+
+    for chain in cfg.decay_chains:
+        # intermediate_resonance_map (src/Info.cu:60-120)
+        ir_map: Dict[Tuple[str, int, int], List[ResonanceConfig]] = {}
+        inter_particles: List[List[Tuple[str, int, int]]] = []
+        for rc in chain.resonance_chains:
+            jps = []
+            for sc in rc.spin_chains:
+                if len(sc.spin_parity) < 2:
+                    raise ConfigError("intermediate %s needs [J, P]" % rc.intermediate)
+                key = (rc.intermediate, sc.spin_parity[0], sc.spin_parity[1])
+                lst = []
+                for rname in sc.resonances:
+                    r = cfg.resonances.get(rname)
+                    if r is not None and (r.J < 0 or r.J == sc.spin_parity[0]):
+                        lst.append(r)
+                ir_map[key] = lst
+                jps.append(key)
+            inter_particles.append(jps)
+        combs = [list(c) for c in itertools.product(*inter_particles)] if inter_particles else [[]]
+
+        for comb in combs:
+            jp = {name: (s, p) for (name, s, p) in comb}
+            nodes: List[Node] = []
+            for st in chain.decay_steps:
+                if len(st.daughters) != 2:
+                    raise ConfigError("decay steps must be two-body")
+
+                def qn(nm):
+                    if nm in jp:
+                        return jp[nm]
+                    if nm in pmap:
+                        return pmap[nm].spin, pmap[nm].parity
+                    return 0, 0
+                (s0, p0), (s1, p1), (s2, p2) = qn(st.mother), qn(st.daughters[0]), qn(st.daughters[1])
+                if 0 in (s0, s1, s2):
+                    raise ConfigError("unresolved quantum numbers in chain %s" % chain.name)
+                sl = com_sl((s0, s1, s2), (p0, p1, p2), st.p_break, False, True,
+                            cfg.global_maxL, st.sl_filter)
+                masses = tuple(pmap[nm].mass if nm in pmap else -1.0
+                               for nm in (st.mother, st.daughters[0], st.daughters[1]))
+                nodes.append(Node(st.mother, st.daughters[0], st.daughters[1], (s0, s1, s2),
+                                  (p0, p1, p2), sl, st.has_bf, st.bf_d, masses))
+            # momentum references
+            root = nodes[0].mother
+            if initial is None:
+                initial = root
+            elif initial != root:
+                raise ConfigError("all chains must start from the same initial particle")
+            mothers = {nd.mother: k for k, nd in enumerate(nodes)}
+            for nd in nodes:
+                refs = []
+                for d in (nd.d1, nd.d2):
+                    if d in final_order:
+                        refs.append(final_order.index(d))
+                    elif d in mothers and mothers[d] > 0:
+                        refs.append(-(mothers[d] + 1))
+                    else:
+                        raise ConfigError("particle '%s' is neither final nor decaying" % d)
+                nd.d1_ref, nd.d2_ref = refs
+                nd.lmin = min((L for _, L in nd.sl_list), default=0)
+            sl_combos = [tuple(c) for c in itertools.product(*[nd.sl_list for nd in nodes])]
+            topo_idx = len(topologies)
+            topologies.append(Topology(chain.name, comb, nodes, sl_combos))
+
+            # resonance combinations (last intermediate fastest)
+            res_lists = [ir_map[key] for key in comb]
+            for rcombo in (itertools.product(*res_lists) if res_lists else [()]):
+                insts: List[ResInst] = []
+                free_lists: List[List[int]] = []
+                range_lists: List[List[List[float]]] = []
+                for key, rconf in zip(comb, rcombo):
+                    model = canonical_model(rconf.type)
+                    vals, names = ordered_params(model, rconf.parameters)
+                    # Constraints.fix_var overrides values (src/Analysis.cu:4632-4650)
+                    for pi, pn in enumerate(names):
+                        nm = rconf.name + "_" + pn
+                        if nm in cfg.fix_var and nm not in cfg.free_var:
+                            vals[pi] = cfg.fix_var[nm]
+                    node_k = mothers.get(key[0], -1)
+                    if node_k <= 0:
+                        raise ConfigError("intermediate %s does not decay in chain %s"
+                                          % (key[0], chain.name))
+                    if model == "Flatte" and len(rconf.channels) < len(vals) - 1:
+                        raise ConfigError("Flatte %s needs one channel per coupling" % rconf.name)
+                    chans = [(float(c[0]), float(c[1])) for c in rconf.channels]
+                    insts.append(ResInst(rconf.name, key[0], model, vals, names, chans, node_k,
+                                         [-1] * len(vals)))
+                    if rconf.free:
+                        if len(rconf.free) == 1 and rconf.free[0] == -1:
+                            fidx = list(range(len(vals)))
+                        else:
+                            fidx = [i for i in rconf.free if 0 <= i < len(vals)]
+                        fidx = [i for i in fidx
+                                if not (rconf.name + "_" + names[i] in cfg.fix_var
+                                        and rconf.name + "_" + names[i] not in cfg.free_var)]
+                        free_lists.append(fidx)
+                    else:
+                        free_lists.append([])
+                    range_lists.append(rconf.free_range)
+
+                # theta slot registration (src/AmpGen.cu:2165-2242)
+                for inst, fidx, rng in zip(insts, free_lists, range_lists):
+                    own = owners.get(inst.name)
+                    if own is not None and fidx:
+                        ob, oi = own
+                        src = blocks[ob].res[oi] if ob < len(blocks) else insts[oi]
+                        inst.slots = list(src.slots)
+                        continue
+                    owners[inst.name] = (len(blocks), insts.index(inst))
+                    for si, p in enumerate(fidx):
+                        val = inst.params[p]
+                        if si < len(rng) and len(rng[si]) >= 2:
+                            lo, hi = rng[si][0], rng[si][1]
+                        else:
+                            half = abs(val) * 0.5
+                            lo, hi = val - half, val + half
+                        nm = inst.name + "_" + inst.param_names[p]
+                        if nm in cfg.var_range:
+                            lo, hi = cfg.var_range[nm]
+                        inst.slots[p] = len(theta)
+                        theta.append(ThetaSlot(nm, inst.name, p, val, lo, hi))
+
+                tagmap = {inst.tag: inst.name for inst in insts}
+                chain_key = chain.name + "".join("_%s[%s]" % (i.tag, i.name) for i in insts)
+                blocks.append(Block(topo_idx, gls, len(sl_combos), insts, chain_key))
+                for c in sl_combos:
+                    parts = []
+                    for nd, (S, L) in zip(nodes, c):
+                        parts.append("%s→%s+%s_SL(%d,%d)" % (
+                            tagmap.get(nd.mother, nd.mother), tagmap.get(nd.d1, nd.d1),
+                            tagmap.get(nd.d2, nd.d2), S, L))
+                    amp_names.append("_".join(parts))
+                    amp_chain_key.append(chain_key)
+                    if chain_key not in chain_keys:
+                        chain_keys.append(chain_key)
+                gls += len(sl_combos)
+
+    n_amp = gls
+    # vspace layout (src/Analysis.cu:4382-4478) without trans folding
+    owners_per_chain: Dict[str, List[int]] = {}
+    for a, ck in enumerate(amp_chain_key):
+        owners_per_chain.setdefault(ck, []).append(a)
+    needs_split = any(len(v) > 1 for v in owners_per_chain.values())
+    amp_free = [0] * n_amp
+    if not needs_split:
+        for a, ck in enumerate(amp_chain_key):
+            amp_free[a] = chain_keys.index(ck)
+        n_free = len(chain_keys)
+    else:
+        nxt = 0
+        for ck in sorted(owners_per_chain):          # std::map order
+            for a in sorted(owners_per_chain[ck]):
+                amp_free[a] = nxt
+                nxt += 1
+        n_free = nxt
+    amp_ratio = [1.0] * n_amp
+    free_names: Dict[int, str] = {}
+    for a in range(n_amp):
+        free_names[amp_free[a]] = amp_names[a]
+    param_names = [free_names[i] for i in sorted(free_names)] + [t.name for t in theta]
+
+    gauss = []
+    for nm, sig in cfg.gauss_constr.items():
+        for j, t in enumerate(theta):
+            if t.name == nm:
+                gauss.append((j, t.init, float(sig)))
+
+    return Plan(particles=particles, final_order=final_order, initial=initial or "",
+                npol=npol, polmap=polmap, topologies=topologies, blocks=blocks,
+                n_amp=n_amp, amp_names=amp_names, theta=theta, amp_free=amp_free,
+                amp_ratio=amp_ratio, n_free=n_free, param_names=param_names, gauss=gauss)

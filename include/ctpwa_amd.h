@@ -1,0 +1,102 @@
+/*
+ * ctpwa_amd.h - C ABI of the MI355X likelihood library (libctpwa_amd.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses on the
+ * device the handle was created on; `stream` is a hipStream_t (NULL = default
+ * stream).  Every call returns 0 on success and a non-zero status otherwise;
+ * ctpwa_last_error() gives the message of the last failure on the calling thread.
+ *
+ * The entry points replace the reference's pybind11 `analysis` boundary
+ * (src/main.cu:90-131) and the device work behind it:
+ *
+ *   ctpwa_create       <- analysis::analysis(config) / initialize()
+ *                         (src/Analysis.cu:1271-1281, 3849-4208): amplitude plan,
+ *                         events, SL amplitudes (src/AmpGen.cu:655-908)
+ *   ctpwa_dims         <- analysis::getNVector / getNFreeTheta / getNParams /
+ *                         getNPolarizations (src/Analysis.cu:1312-1316)
+ *   ctpwa_param_info   <- analysis::getFreeResParams (src/AmpGen.cu:2193-2197)
+ *   ctpwa_nll_grad     <- NLLFunction::forward + backward (src/Analysis.cu:625-1261)
+ *   ctpwa_hessian      <- analysis::getHessian (src/Analysis.cu:2689-3378)
+ *   ctpwa_intensity    <- per-event |sum_a v_a A_a|^2 summed over polarisations, the
+ *                         quantity behind writeResult's weights
+ *                         (src/ComputeResults.cu:22-182)
+ *   ctpwa_amplitudes   <- analysis::getDataTensor/getPhspTensor-style amplitude
+ *                         access (src/Analysis.cu:3640-3700)
+ *   ctpwa_footprint_bytes <- DeviceManager::estimate (src/DeviceManager.cu:172-221)
+ */
+#ifndef CTPWA_AMD_H
+#define CTPWA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ctpwa_handle ctpwa_handle;
+
+/* event classes */
+#define CTPWA_PHSP 0
+#define CTPWA_DATA 1
+#define CTPWA_BKG 2
+
+/* Build the likelihood for `plan_text` (written by ctpwa_amd.model.Plan.to_text).
+ * Events are host arrays [n][nfinal][4] of float64 (E, px, py, pz) in Data.order;
+ * weights are host arrays [n] or NULL for unit weights.  n may be 0 for bkg.
+ * The library copies what it needs; the host arrays may be freed on return. */
+int ctpwa_create(const char* plan_text,
+                 const double* phsp, long long n_phsp, const double* w_phsp,
+                 const double* data, long long n_data, const double* w_data,
+                 const double* bkg, long long n_bkg, const double* w_bkg,
+                 int device, ctpwa_handle** out);
+
+void ctpwa_destroy(ctpwa_handle* h);
+
+/* n_amp: amplitudes; n_free: free complex couplings (getNVector); n_theta: free
+ * resonance parameters (getNFreeTheta); n_pol: polarisation states. */
+int ctpwa_dims(const ctpwa_handle* h, int* n_amp, int* n_free, int* n_theta, int* n_pol);
+
+/* initial value, lower and upper bound of each free resonance parameter (host [n_theta]) */
+int ctpwa_param_info(const ctpwa_handle* h, double* init, double* lower, double* upper);
+
+/* NLL and gradient.  params_dev: device [2 n_free + n_theta] float64 laid out
+ * [Re p, Im p, theta]; out_dev: device [1 + 2 n_free + n_theta] = [NLL, gradient]. */
+int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double* out_dev, void* stream);
+
+/* Hessian in the same layout: device [(2 n_free + n_theta)^2], row-major. */
+int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream);
+
+/* per-event intensity of class `cls`: device [n_cls] */
+int ctpwa_intensity(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
+
+/* per-event amplitudes of class `cls` (without couplings): device [n_cls][n_pol][n_amp][2] */
+int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
+
+/* average duration in ms of the fused per-event kernel over `iters` launches at
+ * `params_dev` (HIP events on `stream`; the prep kernel runs once before) */
+int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev, int iters, void* stream, double* avg_ms);
+
+/* device bytes a handle for this plan and these event counts holds */
+long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg);
+
+/* number of events of a class, and algorithmic bytes the fused kernel must read per
+ * evaluation (SL tensors + invariants + per-event weights) */
+int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long long* bytes_per_eval,
+                 int* n_sl_tensors, int* n_invariants, int* n_workgroups);
+
+/* generated source of the fused kernel (NUL-terminated, owned by the handle) */
+const char* ctpwa_kernel_source(const ctpwa_handle* h);
+
+/* source of the fused kernel for a plan, without a device: writes at most cap bytes
+ * (NUL-terminated) into buf and returns the full size including the NUL, -1 on error */
+long long ctpwa_generate_source(const char* plan_text, char* buf, long long cap);
+
+/* compile the fused kernel for a plan with hipRTC (no device needed); returns the
+ * code-object size, or -1 with the compiler log in ctpwa_last_error() */
+long long ctpwa_compile_check(const char* plan_text);
+
+const char* ctpwa_last_error(void);
+const char* ctpwa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
