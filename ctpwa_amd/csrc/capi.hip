@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -25,7 +26,7 @@ std::string generate_eval_kernel(const Model_& M);
 extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctpwa::DTopo*, int, const ctpwa::DNode*,
                                          const ctpwa::DKin*, int, const ctpwa::DU*, int, const double*, const int*, int,
                                          const ctpwa::DInv*, int, double2*, double*, long long, long long, hipStream_t);
-extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
@@ -208,6 +209,13 @@ struct ctpwa_handle {
     }
 };
 
+// code-generation options for the fused kernel (tuning; defaults are the measured best)
+static void apply_codegen_env(Model_& M) {
+    if (const char* e = std::getenv("CTPWA_EVAL_WAVES")) M.min_waves = std::atoi(e);
+    if (const char* e = std::getenv("CTPWA_EVAL_RELOAD")) M.reload_coef = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CTPWA_EVAL_PREFETCH")) M.prefetch = std::atoi(e) != 0;
+}
+
 static void build_tables(ctpwa_handle* h, std::vector<DTopo>& topos, std::vector<DNode>& nodes,
                          std::vector<DKin>& kins, std::vector<DU>& us, std::vector<double>& cgtab,
                          std::vector<DInv>& invs) {
@@ -336,6 +344,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h = new ctpwa_handle();
         h->device = device;
         h->M = derive(parse_plan(plan_text));
+        apply_codegen_env(h->M);
         check_q0_masses(h->M);
         const Model_& M = h->M;
         const Plan& P = M.P;
@@ -483,7 +492,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         HIPCHECK(hipMemset(h->table, 0, M.table_size() * sizeof(double)));
 
         setup_partition(h);
-        h->nacc = 1 + 2 * P.namp + P.ntheta;
+        h->nacc = M.nacc();
         h->part = h->alloc<double>((size_t)h->nacc * std::max(1, h->nwg));
 
         h->prep = PrepArgs{d_blocks, d_res, d_nodes2, d_topos2, d_slcL, d_slc0, d_dens, d_amps,
@@ -499,6 +508,12 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->fin.off_rec = (long long)M.off_rec();
         h->fin.amps = d_amps;
         h->fin.blocks = d_blocks;
+        {
+            std::vector<DThTerm> th;
+            for (const ThTerm& t : M.thterms) th.push_back(DThTerm{M.dens[t.den].block, M.dens[t.den].coef_base + t.coef});
+            h->fin.thterms = h->upload(th);
+            h->fin.nth = (int)th.size();
+        }
         h->fin.namp = P.namp;
         h->fin.nfree = P.nfree;
         h->fin.ntheta = P.ntheta;
@@ -509,7 +524,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->fin.Np = (double)n_phsp;
         h->fin.Wd = h->Wd;
         h->fin.Wb = h->Wb;
-        h->fin.totals = nullptr;
+        h->fin.totals = h->alloc<double>((size_t)3 * h->nacc);
         h->aux = AuxArgs{h->sl, h->inv, h->ntot, h->table, (long long)M.off_c(), (long long)M.off_rec(),
                          d_amps, d_blocks, d_dens, d_dterms, d_terms, P.namp, P.npol};
 
@@ -569,7 +584,7 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     if (!h || !params_dev || !out_dev) return fail("ctpwa_nll_grad: NULL argument");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, s);
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
     if (e == hipSuccess) e = launch_eval(h, s);
     if (e == hipSuccess) {
         h->fin.params = params_dev;
@@ -591,7 +606,7 @@ static int aux_call(ctpwa_handle* h, const double* params_dev, int cls, double* 
     if (cls < 0 || cls > 2) return fail("event class must be 0, 1 or 2");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, s);
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
     if (e == hipSuccess) e = ctpwa_launch_aux(&h->aux, h->off[cls], h->n[cls], out_dev, intensity, s);
     if (e != hipSuccess) return fail(hipGetErrorString(e));
     return 0;
@@ -612,7 +627,7 @@ extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev,
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
-    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, s);
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
     if (e == hipSuccess) e = hipEventRecord(a, s);
     for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_eval(h, s);
     if (e == hipSuccess) e = hipEventRecord(b, s);
@@ -662,7 +677,9 @@ extern "C" const char* ctpwa_kernel_source(const ctpwa_handle* h) { return h ? h
 // source of the fused kernel for a plan, without a device (buf may be NULL to query the size)
 extern "C" long long ctpwa_generate_source(const char* plan_text, char* buf, long long cap) {
     try {
-        std::string src = generate_eval_kernel(derive(parse_plan(plan_text)));
+        Model_ M = derive(parse_plan(plan_text));
+        apply_codegen_env(M);
+        std::string src = generate_eval_kernel(M);
         if (buf && cap > 0) {
             long long n = std::min<long long>(cap - 1, (long long)src.size());
             std::memcpy(buf, src.data(), (size_t)n);
@@ -679,7 +696,9 @@ extern "C" long long ctpwa_generate_source(const char* plan_text, char* buf, lon
 // code-object size in bytes, or -1 with the compiler log in ctpwa_last_error()
 extern "C" long long ctpwa_compile_check(const char* plan_text) {
     try {
-        std::string src = generate_eval_kernel(derive(parse_plan(plan_text)));
+        Model_ M = derive(parse_plan(plan_text));
+        apply_codegen_env(M);
+        std::string src = generate_eval_kernel(M);
         hiprtcProgram prog;
         if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
             throw std::runtime_error("hiprtcCreateProgram failed");

@@ -33,6 +33,7 @@ struct DRes { int node, model, np, nch; double p[6]; int slot[6]; double ch[8]; 
 struct DDen { int block, res, rec, ncoef; };
 struct DAmp { int block, slc, freeidx, rec, u, pad; double ratio; };
 struct DTerm { int inv, coef; double cval; };   // inv < 0: 1; coef < 0: constant cval
+struct DThTerm { int block, rec; };               // theta-gradient accumulator -> coefficient record
 struct DDenTerms { int nre, nim, t0, pad; };      // terms [t0, t0+nre) real, then nim imaginary
 
 // Blatt-Weisskopf polynomial N_L(z), Bf(L,q,q0,d) = sqrt(N_L(q0 d)/N_L(q d))
@@ -79,6 +80,8 @@ struct FinArgs {
     long long off_v, off_rec;
     const DAmp* amps;
     const DBlock* blocks;
+    const DThTerm* thterms;
+    int nth;
     int namp, nfree, ntheta, ngauss;
     const int* g_slot;
     const double* g_mu;

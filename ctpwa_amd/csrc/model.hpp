@@ -135,6 +135,10 @@ struct DenSpec {
 
 struct AmpSpec { int block, slc, u; int coef_base; };
 
+// one accumulator of the theta gradient: sum_e x_t(e) Re R_d(e) (real term) or
+// sum_e y_t(e) Im R_d(e) (imaginary term) of a theta-dependent denominator coefficient
+struct ThTerm { int den, coef, im, inv; };
+
 struct Model_ {
     Plan P;
     std::vector<KinNode> kin;
@@ -145,7 +149,12 @@ struct Model_ {
     std::vector<std::vector<int>> block_dens;     // [block] -> den indices
     std::vector<AmpSpec> amps;                    // [namp]
     std::vector<std::vector<int>> bvars;          // [block] -> local var -> global slot
+    std::vector<ThTerm> thterms;                  // theta-gradient accumulators
     int maxloc = 0;
+    // code-generation options of the fused kernel
+    int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
+    bool reload_coef = false; // reload coefficients per iteration (scalar loads) instead of pinning
+    bool prefetch = false;    // load the next event's inputs while computing the current one
     // coefficient records: each holds value, NLOC first and NLOC*NLOC second derivatives
     static const int NLOC = 8;
     static const int REC = 1 + NLOC + NLOC * NLOC;
@@ -156,6 +165,7 @@ struct Model_ {
     size_t off_c() const { return 2 * (size_t)P.namp; }
     size_t off_rec() const { return 4 * (size_t)P.namp; }
     size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC; }
+    int nacc() const { return 1 + 2 * P.namp + (int)thterms.size(); }
 };
 
 // final-state subset of a momentum reference
@@ -326,6 +336,12 @@ inline Model_ derive(const Plan& P) {
             if (a < 0 || a >= P.namp) throw std::runtime_error("amplitude index out of range");
             M.amps[a] = AmpSpec{(int)b, c, u, 0};
         }
+    }
+    for (size_t d = 0; d < M.dens.size(); ++d) {
+        const DenSpec& Dn = M.dens[d];
+        if (M.bvars[Dn.block].empty()) continue;
+        for (const Term& t : Dn.re) if (t.coef >= 0) M.thterms.push_back(ThTerm{(int)d, t.coef, 0, t.inv});
+        for (const Term& t : Dn.im) if (t.coef >= 0) M.thterms.push_back(ThTerm{(int)d, t.coef, 1, t.inv});
     }
     // amplitude records follow the denominator records
     M.amp_rec0 = rec;

@@ -4,7 +4,7 @@
 //              numerators K_a and every propagator-denominator coefficient, each with
 //              its first and second derivatives over the block's free parameters
 //              (forward-mode jets).  One thread per item; runs in microseconds.
-//  k_finalize  per-workgroup partial sums -> NLL and gradient in the fit-parameter
+//  k_reduce, k_assemble  per-workgroup partial sums -> NLL and gradient in the fit-parameter
 //              layout [Re p, Im p, theta], following the assembly of the reference's
 //              NLLFunction::forward/backward (src/Analysis.cu:1134-1261): phase-space
 //              mean, N_eff log term, NaN/Inf -> 1e30 with zero gradient, Gaussian
@@ -16,67 +16,115 @@
 namespace ctpwa {
 
 // ---------------------------------------------------------------------------
-// forward-mode jet over NLOC block-local variables (value, gradient, Hessian)
+// forward-mode jet over NV block-local variables: value, gradient and (when H)
+// Hessian.  NV is the smallest of 1/2/4/8 that holds the model's largest block, so
+// the jets stay in registers.
 // ---------------------------------------------------------------------------
+template <int NV, bool H>
 struct DJ {
-    double v, g[NLOC], h[NLOC][NLOC];
+    double v, g[NV], h[H ? NV : 1][H ? NV : 1];
 };
-__device__ inline DJ jc(double c) {
-    DJ r;
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jc(double c) {
+    DJ<NV, H> r;
     r.v = c;
-    for (int i = 0; i < NLOC; ++i) { r.g[i] = 0.0; for (int k = 0; k < NLOC; ++k) r.h[i][k] = 0.0; }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) r.g[i] = 0.0;
+    if constexpr (H) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) r.h[i][k] = 0.0;
+    }
     return r;
 }
-__device__ inline DJ jv(double c, int i) { DJ r = jc(c); if (i >= 0 && i < NLOC) r.g[i] = 1.0; return r; }
-__device__ inline DJ jadd(const DJ& a, const DJ& b) {
-    DJ r;
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jv(double c, int i) {
+    DJ<NV, H> r = jc<NV, H>(c);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) if (k == i) r.g[k] = 1.0;
+    return r;
+}
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jadd(const DJ<NV, H>& a, const DJ<NV, H>& b) {
+    DJ<NV, H> r;
     r.v = a.v + b.v;
-    for (int i = 0; i < NLOC; ++i) { r.g[i] = a.g[i] + b.g[i]; for (int k = 0; k < NLOC; ++k) r.h[i][k] = a.h[i][k] + b.h[i][k]; }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) r.g[i] = a.g[i] + b.g[i];
+    if constexpr (H) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) r.h[i][k] = a.h[i][k] + b.h[i][k];
+    }
     return r;
 }
-__device__ inline DJ jscale(const DJ& a, double c) {
-    DJ r;
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jscale(const DJ<NV, H>& a, double c) {
+    DJ<NV, H> r;
     r.v = a.v * c;
-    for (int i = 0; i < NLOC; ++i) { r.g[i] = a.g[i] * c; for (int k = 0; k < NLOC; ++k) r.h[i][k] = a.h[i][k] * c; }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) r.g[i] = a.g[i] * c;
+    if constexpr (H) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) r.h[i][k] = a.h[i][k] * c;
+    }
     return r;
 }
-__device__ inline DJ jsub(const DJ& a, const DJ& b) { return jadd(a, jscale(b, -1.0)); }
-__device__ inline DJ jmul(const DJ& a, const DJ& b) {
-    DJ r;
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jsub(const DJ<NV, H>& a, const DJ<NV, H>& b) { return jadd(a, jscale(b, -1.0)); }
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jmul(const DJ<NV, H>& a, const DJ<NV, H>& b) {
+    DJ<NV, H> r;
     r.v = a.v * b.v;
-    for (int i = 0; i < NLOC; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
-    for (int i = 0; i < NLOC; ++i)
-        for (int k = 0; k < NLOC; ++k)
-            r.h[i][k] = a.h[i][k] * b.v + a.v * b.h[i][k] + a.g[i] * b.g[k] + a.g[k] * b.g[i];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+    if constexpr (H) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                r.h[i][k] = a.h[i][k] * b.v + a.v * b.h[i][k] + a.g[i] * b.g[k] + a.g[k] * b.g[i];
+    }
     return r;
 }
 // f(a) given f, f', f''
-__device__ inline DJ jfun(const DJ& a, double f, double d1, double d2) {
-    DJ r;
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jfun(const DJ<NV, H>& a, double f, double d1, double d2) {
+    DJ<NV, H> r;
     r.v = f;
-    for (int i = 0; i < NLOC; ++i) r.g[i] = d1 * a.g[i];
-    for (int i = 0; i < NLOC; ++i)
-        for (int k = 0; k < NLOC; ++k) r.h[i][k] = d1 * a.h[i][k] + d2 * a.g[i] * a.g[k];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) r.g[i] = d1 * a.g[i];
+    if constexpr (H) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) r.h[i][k] = d1 * a.h[i][k] + d2 * a.g[i] * a.g[k];
+    }
     return r;
 }
-__device__ inline DJ jinv(const DJ& a) { double x = a.v; return jfun(a, 1.0 / x, -1.0 / (x * x), 2.0 / (x * x * x)); }
-__device__ inline DJ jdiv(const DJ& a, const DJ& b) { return jmul(a, jinv(b)); }
-__device__ inline DJ jsqrt(const DJ& a) { double s = sqrt(a.v); return jfun(a, s, 0.5 / s, -0.25 / (s * a.v)); }
-__device__ inline DJ jlog(const DJ& a) { return jfun(a, log(a.v), 1.0 / a.v, -1.0 / (a.v * a.v)); }
-__device__ inline DJ jpowi(const DJ& a, int n) {
-    if (n == 0) return jc(1.0);
-    double x = a.v;
-    double xn2 = 1.0;
-    for (int i = 0; i < n - 2; ++i) xn2 *= x;
-    double xn1 = (n >= 2) ? xn2 * x : 1.0;
-    double xn = (n >= 1) ? xn1 * x : 1.0;
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jinv(const DJ<NV, H>& a) { double x = a.v; return jfun(a, 1.0 / x, -1.0 / (x * x), 2.0 / (x * x * x)); }
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jdiv(const DJ<NV, H>& a, const DJ<NV, H>& b) { return jmul(a, jinv(b)); }
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jsqrt(const DJ<NV, H>& a) { double s = sqrt(a.v); return jfun(a, s, 0.5 / s, -0.25 / (s * a.v)); }
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jlog(const DJ<NV, H>& a) { return jfun(a, log(a.v), 1.0 / a.v, -1.0 / (a.v * a.v)); }
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jpowi(const DJ<NV, H>& a, int n) {
+    if (n == 0) return jc<NV, H>(1.0);
     if (n == 1) return a;
+    double x = a.v, xn2 = 1.0;
+    for (int i = 0; i < n - 2; ++i) xn2 *= x;
+    double xn1 = xn2 * x, xn = xn1 * x;
     return jfun(a, xn, n * xn1, n * (n - 1.0) * xn2);
 }
 // N_L(z) with z = q0 d as a jet in q0
-__device__ inline DJ jbfpoly(int L, const DJ& q0, double d) {
-    DJ z = jscale(q0, d);
-    DJ z2 = jmul(z, z);
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jbfpoly(int L, const DJ<NV, H>& q0, double d) {
     double c[7];
     int deg = 0;
     switch (L) {
@@ -86,60 +134,73 @@ __device__ inline DJ jbfpoly(int L, const DJ& q0, double d) {
     case 4: c[0] = 11025; c[1] = 1575; c[2] = 135; c[3] = 10; c[4] = 1; deg = 4; break;
     case 5: c[0] = 893025; c[1] = 99225; c[2] = 6300; c[3] = 315; c[4] = 15; c[5] = 1; deg = 5; break;
     case 6: c[0] = 540326025; c[1] = 6185025; c[2] = 363825; c[3] = 17325; c[4] = 630; c[5] = 21; c[6] = 1; deg = 6; break;
-    default: return jc(1.0);
+    default: return jc<NV, H>(1.0);
     }
-    DJ r = jc(c[deg]);                       // Horner in z^2
-    for (int i = deg - 1; i >= 0; --i) r = jadd(jmul(r, z2), jc(c[i]));
+    DJ<NV, H> z = jscale(q0, d);
+    DJ<NV, H> z2 = jmul(z, z);
+    DJ<NV, H> r = jc<NV, H>(c[deg]);          // Horner in z^2
+    for (int i = deg - 1; i >= 0; --i) r = jadd(jmul(r, z2), jc<NV, H>(c[i]));
     return r;
 }
 // breakup momentum (src/AmpGen.cu:2733-2737) and computeQ0AD (include/ResModel.cuh:88-97)
-__device__ inline DJ jbreakup(const DJ& m, const DJ& m1, const DJ& m2) {
-    DJ s = jadd(m1, m2), d = jsub(m1, m2);
-    DJ mm = jmul(m, m);
-    DJ q2 = jmul(jsub(mm, jmul(s, s)), jsub(mm, jmul(d, d)));
-    if (q2.v <= 0.0) return jc(0.0);
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jbreakup(const DJ<NV, H>& m, const DJ<NV, H>& m1, const DJ<NV, H>& m2) {
+    DJ<NV, H> s = jadd(m1, m2), d = jsub(m1, m2);
+    DJ<NV, H> mm = jmul(m, m);
+    DJ<NV, H> q2 = jmul(jsub(mm, jmul(s, s)), jsub(mm, jmul(d, d)));
+    if (q2.v <= 0.0) return jc<NV, H>(0.0);
     return jdiv(jsqrt(q2), jscale(m, 2.0));
 }
-__device__ inline DJ jq0ad(const DJ& m0, const DJ& m1, const DJ& m2) {
-    DJ s = jadd(m1, m2), d = jsub(m1, m2);
-    DJ m0s = jmul(m0, m0);
-    DJ q2 = jdiv(jmul(jsub(m0s, jmul(s, s)), jsub(m0s, jmul(d, d))), jscale(m0s, 4.0));
-    if (q2.v < 0.0) return jc(0.0);
+template <int NV, bool H>
+__device__ inline DJ<NV, H> jq0ad(const DJ<NV, H>& m0, const DJ<NV, H>& m1, const DJ<NV, H>& m2) {
+    DJ<NV, H> s = jadd(m1, m2), d = jsub(m1, m2);
+    DJ<NV, H> m0s = jmul(m0, m0);
+    DJ<NV, H> q2 = jdiv(jmul(jsub(m0s, jmul(s, s)), jsub(m0s, jmul(d, d))), jscale(m0s, 4.0));
+    if (q2.v < 0.0) return jc<NV, H>(0.0);
     return jsqrt(q2);
 }
-
 
 __device__ inline int local_index(const DBlock& B, int slot) {
     for (int i = 0; i < B.nloc; ++i) if (B.vars[i] == slot) return i;
     return -1;
 }
-__device__ inline DJ res_param(const DRes& R, int p, const DBlock& B, const double* theta) {
-    if (p >= R.np) return jc(0.0);
+template <int NV, bool H>
+__device__ inline DJ<NV, H> res_param(const DRes& R, int p, const DBlock& B, const double* theta) {
+    if (p >= R.np) return jc<NV, H>(0.0);
     int s = R.slot[p];
-    if (s < 0) return jc(R.p[p]);
-    return jv(theta[s], local_index(B, s));
+    if (s < 0) return jc<NV, H>(R.p[p]);
+    return jv<NV, H>(theta[s], local_index(B, s));
 }
 // mass used in q0 for a daughter: configured mass, else the mass parameter of the
 // block's resonance that occupies it (src/AmpGen.cu:1664-1731)
-__device__ inline DJ q0_dmass(double cfg, int ref, const DBlock& B, const DRes* res, const double* theta) {
-    if (cfg > 0) return jc(cfg);
+template <int NV, bool H>
+__device__ inline DJ<NV, H> q0_dmass(double cfg, int ref, const DBlock& B, const DRes* res, const double* theta) {
+    if (cfg > 0) return jc<NV, H>(cfg);
     if (ref < 0) {
         int kk = -ref - 1;
         for (int r = 0; r < B.nres; ++r) {
             const DRes& R = res[B.res0 + r];
-            if (R.node == kk && R.np > 0) return res_param(R, 0, B, theta);
+            if (R.node == kk && R.np > 0) return res_param<NV, H>(R, 0, B, theta);
         }
     }
-    return jc(1.0);   // unreachable: the host rejects event-dependent q0
+    return jc<NV, H>(1.0);   // unreachable: the host rejects event-dependent q0
 }
-__device__ inline void store(double* t, const DJ& x) {
+template <int NV, bool H>
+__device__ inline void store(double* t, const DJ<NV, H>& x) {
     t[0] = x.v;
-    for (int i = 0; i < NLOC; ++i) t[1 + i] = x.g[i];
-    for (int i = 0; i < NLOC; ++i)
-        for (int k = 0; k < NLOC; ++k) t[1 + NLOC + i * NLOC + k] = x.h[i][k];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) t[1 + i] = x.g[i];
+    if constexpr (H) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) t[1 + NLOC + i * NLOC + k] = x.h[i][k];
+    }
 }
 
+template <int NV, bool H>
 __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
+    using J = DJ<NV, H>;
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     const double* theta = params + 2 * a.nfree;
     if (item < a.nden) {
@@ -148,42 +209,42 @@ __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, 
         const DRes& R = a.res[B.res0 + D.res];
         const DTopo& T = a.topos[B.topo];
         const DNode& n = a.nodes[T.node0 + R.node];
-        DJ m0 = (R.np > 0) ? res_param(R, 0, B, theta) : jc(1.0);
-        DJ q0 = jq0ad(m0, q0_dmass(n.md1, n.d1ref, B, a.res, theta), q0_dmass(n.md2, n.d2ref, B, a.res, theta));
+        J m0 = (R.np > 0) ? res_param<NV, H>(R, 0, B, theta) : jc<NV, H>(1.0);
+        J q0 = jq0ad(m0, q0_dmass<NV, H>(n.md1, n.d1ref, B, a.res, theta), q0_dmass<NV, H>(n.md2, n.d2ref, B, a.res, theta));
         double* t = table + a.off_rec + (long long)D.rec * REC;
-        DJ m02 = jmul(m0, m0);
+        J m02 = jmul(m0, m0);
         switch (R.model) {
         case 0: {   // BWR: D = m0^2 - s - i g0 m0^2 N_L(q0 d')/q0^(2L+1) * E
-            DJ g0 = res_param(R, 1, B, theta);
+            J g0 = res_param<NV, H>(R, 1, B, theta);
             double dp = n.has_bf ? n.bf_d : 0.0;
-            DJ N0 = (dp != 0.0) ? jbfpoly(n.lmin, q0, dp) : jc(1.0);
-            DJ b = jdiv(jmul(jmul(g0, m02), N0), jpowi(q0, 2 * n.lmin + 1));
+            J N0 = (dp != 0.0) ? jbfpoly(n.lmin, q0, dp) : jc<NV, H>(1.0);
+            J b = jdiv(jmul(jmul(g0, m02), N0), jpowi(q0, 2 * n.lmin + 1));
             store(t, m02);
             store(t + REC, b);
             break;
         }
         case 1: {   // BW: D = m0^2 - s - i m0 g0
-            DJ g0 = res_param(R, 1, B, theta);
+            J g0 = res_param<NV, H>(R, 1, B, theta);
             store(t, m02);
             store(t + REC, jmul(m0, g0));
             break;
         }
         case 3: {   // Flatte: D = m0^2 - s + sum b_i rho_im,i - i sum b_i rho_re,i, b_i = m0 g_i / 2
             store(t, m02);
-            for (int i = 0; i < R.nch; ++i) store(t + (1 + i) * REC, jscale(jmul(m0, res_param(R, 1 + i, B, theta)), 0.5));
+            for (int i = 0; i < R.nch; ++i) store(t + (1 + i) * REC, jscale(jmul(m0, res_param<NV, H>(R, 1 + i, B, theta)), 0.5));
             break;
         }
         case 4: {   // Gounaris-Sakurai (src/ResModel.cu:484-561)
             const double pi = 3.14159265358979323846;
-            DJ g0 = res_param(R, 1, B, theta);
+            J g0 = res_param<NV, H>(R, 1, B, theta);
             double mpi = R.nch > 0 ? R.ch[0] : 0.1396;
-            DJ q02 = jmul(q0, q0), q03 = jmul(q02, q0);
-            DJ logA = jlog(jscale(jadd(m0, jscale(q0, 2.0)), 1.0 / (2.0 * mpi)));
-            DJ h0 = jmul(jscale(jdiv(q0, m0), 2.0 / pi), logA);
-            DJ dh0 = jadd(jmul(h0, jsub(jinv(jscale(q02, 8.0)), jinv(jscale(m02, 2.0)))), jinv(jscale(m02, 2.0 * pi)));
-            DJ c1 = jdiv(jmul(g0, m02), q03);
-            DJ c3 = jdiv(jmul(jmul(g0, m02), dh0), q0);
-            DJ one_c3 = jadd(jc(1.0), c3);
+            J q02 = jmul(q0, q0), q03 = jmul(q02, q0);
+            J logA = jlog(jscale(jadd(m0, jscale(q0, 2.0)), 1.0 / (2.0 * mpi)));
+            J h0 = jmul(jscale(jdiv(q0, m0), 2.0 / pi), logA);
+            J dh0 = jadd(jmul(h0, jsub(jinv(jscale(q02, 8.0)), jinv(jscale(m02, 2.0)))), jinv(jscale(m02, 2.0 * pi)));
+            J c1 = jdiv(jmul(g0, m02), q03);
+            J c3 = jdiv(jmul(jmul(g0, m02), dh0), q0);
+            J one_c3 = jadd(jc<NV, H>(1.0), c3);
             store(t, jmul(one_c3, m02));
             store(t + REC, jscale(one_c3, -1.0));
             store(t + 2 * REC, c1);
@@ -200,34 +261,34 @@ __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, 
     const DBlock& B = a.blocks[A.block];
     const DTopo& T = a.topos[B.topo];
     const int* Ls = a.slcL + (long long)(a.slc0[B.topo] + A.slc) * MAXNODE;
-    DJ K = jc(1.0);
+    J K = jc<NV, H>(1.0);
     for (int k = 0; k < T.nn; ++k) {
         const DNode& n = a.nodes[T.node0 + k];
         int L = Ls[k];
         int r = -1;
         for (int i = 0; i < B.nres; ++i) if (a.res[B.res0 + i].node == k) { r = i; break; }
-        DJ m1 = q0_dmass(n.md1, n.d1ref, B, a.res, theta), m2 = q0_dmass(n.md2, n.d2ref, B, a.res, theta);
+        J m1 = q0_dmass<NV, H>(n.md1, n.d1ref, B, a.res, theta), m2 = q0_dmass<NV, H>(n.md2, n.d2ref, B, a.res, theta);
         if (k == 0 || r < 0) {
             if (n.has_bf && L >= 1 && L <= 6) {
-                DJ q0 = jbreakup(jc(n.mm > 0 ? n.mm : 1.0), m1, m2);
+                J q0 = jbreakup(jc<NV, H>(n.mm > 0 ? n.mm : 1.0), m1, m2);
                 K = jmul(K, jsqrt(jbfpoly(L, q0, n.bf_d)));
             }
             continue;
         }
         const DRes& R = a.res[B.res0 + r];
         if (R.model == 1) continue;                        // BW carries no barrier
-        DJ m0 = (R.np > 0) ? res_param(R, 0, B, theta) : jc(1.0);
-        DJ q0 = jq0ad(m0, m1, m2);
+        J m0 = (R.np > 0) ? res_param<NV, H>(R, 0, B, theta) : jc<NV, H>(1.0);
+        J q0 = jq0ad(m0, m1, m2);
         if (n.has_bf && L >= 1 && L <= 6) K = jmul(K, jsqrt(jbfpoly(L, q0, n.bf_d)));
         if (R.model == 4) {                                // GS numerator 1 + d(m0) g0 / m0
             const double pi = 3.14159265358979323846;
-            DJ g0 = res_param(R, 1, B, theta);
+            J g0 = res_param<NV, H>(R, 1, B, theta);
             double mpi = R.nch > 0 ? R.ch[0] : 0.1396;
-            DJ q02 = jmul(q0, q0), q03 = jmul(q02, q0);
-            DJ logA = jlog(jscale(jadd(m0, jscale(q0, 2.0)), 1.0 / (2.0 * mpi)));
-            DJ dv = jadd(jsub(jmul(jscale(jdiv(jc(mpi * mpi), q02), 3.0 / pi), logA), jmul(jc(mpi * mpi / pi), jdiv(m0, q03))),
+            J q02 = jmul(q0, q0), q03 = jmul(q02, q0);
+            J logA = jlog(jscale(jadd(m0, jscale(q0, 2.0)), 1.0 / (2.0 * mpi)));
+            J dv = jadd(jsub(jmul(jscale(jdiv(jc<NV, H>(mpi * mpi), q02), 3.0 / pi), logA), jmul(jc<NV, H>(mpi * mpi / pi), jdiv(m0, q03))),
                          jdiv(m0, jscale(q0, 2.0 * pi)));
-            K = jmul(K, jadd(jc(1.0), jdiv(jmul(dv, g0), m0)));
+            K = jmul(K, jadd(jc<NV, H>(1.0), jdiv(jmul(dv, g0), m0)));
         }
     }
     store(table + a.off_rec + (long long)A.rec * REC, K);
@@ -242,22 +303,31 @@ __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, 
 // final assembly
 // ---------------------------------------------------------------------------
 
-constexpr int FIN_THREADS = 1024;
+constexpr int RED_THREADS = 256;
+constexpr int ASM_THREADS = 256;
 
-__global__ void __launch_bounds__(FIN_THREADS) k_finalize(FinArgs f) {
-    extern __shared__ double sh[];          // [3 * nacc] totals + [namp * 2] X + flags
-    int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = FIN_THREADS / 64;
-    // 1. class totals, one wave per (class, column), fixed summation order
-    for (int pair = wave; pair < 3 * f.nacc; pair += nwave) {
-        int c = pair / f.nacc, col = pair - c * f.nacc;
-        double s = 0.0;
-        for (int g = f.g0[c] + lane; g < f.g1[c]; g += 64) s += f.part[(long long)col * f.nwg + g];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) sh[pair] = s;
-    }
+// stage 1: one workgroup per (class, accumulator column); fixed summation order
+__global__ void __launch_bounds__(RED_THREADS) k_reduce(FinArgs f) {
+    __shared__ double sh[RED_THREADS];
+    int pair = blockIdx.x;
+    int c = pair / f.nacc, col = pair - c * f.nacc;
+    const double* src = f.part + (long long)col * f.nwg;
+    double s = 0.0;
+    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
+    sh[threadIdx.x] = s;
     __syncthreads();
-    double* tot = sh;                        // tot[c * nacc + col]
-    double* X = sh + 3 * f.nacc;             // combined X_a (re, im)
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) f.totals[pair] = sh[0];
+}
+
+// stage 2: NLL and gradient from the class totals
+__global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) {
+    extern __shared__ double X[];            // combined X_a (re, im)
+    int tid = threadIdx.x;
+    const double* tot = f.totals;            // tot[c * nacc + col]
     double Np = f.Np, Neff = f.Wd - f.Wb;
     double pf = (Np > 0) ? tot[0] / Np : 0.0;
     if (!(pf > 1e-30) || isinf(pf)) pf = 1e-30;
@@ -269,18 +339,16 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(FinArgs f) {
         loss += d * d / (2.0 * f.g_sig[g] * f.g_sig[g]);
     }
     bool reset = isnan(loss) || isinf(loss);
-    // 2. X_a = -data + bkg + cp * phsp   (column 1 + 2a is Re, 2 + 2a is Im)
-    for (int a = tid; a < f.namp; a += FIN_THREADS) {
-        for (int q = 0; q < 2; ++q) {
-            int col = 1 + 2 * a + q;
-            X[2 * a + q] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
-        }
+    // X_a = -data + bkg + cp * phsp   (column 1 + 2a is Re, 2 + 2a is Im)
+    for (int i = tid; i < 2 * f.namp; i += ASM_THREADS) {
+        int col = 1 + i;
+        X[i] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
     }
     __syncthreads();
     double* out = f.out;
     if (tid == 0) out[0] = reset ? 1e30 : loss;
-    // 3. coupling gradient: dL/dp_f* = sum_a r_a K_a X_a ; output [2 Re, 2 Im]
-    for (int fi = tid; fi < f.nfree; fi += FIN_THREADS) {
+    // coupling gradient: dL/dp_f* = sum_a r_a K_a X_a ; output [2 Re, 2 Im]
+    for (int fi = tid; fi < f.nfree; fi += ASM_THREADS) {
         double gr = 0.0, gi = 0.0;
         for (int a = 0; a < f.namp; ++a) {
             const DAmp& A = f.amps[a];
@@ -292,11 +360,20 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(FinArgs f) {
         out[1 + fi] = reset ? 0.0 : 2.0 * gr;
         out[1 + f.nfree + fi] = reset ? 0.0 : 2.0 * gi;
     }
-    // 4. theta gradient: kernel-accumulated propagator part + numerator part
-    //    2 Re sum_a v_a dK_a/dtheta conj(X_a)
-    for (int j = tid; j < f.ntheta; j += FIN_THREADS) {
-        int col = 1 + 2 * f.namp + j;
-        double g = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
+    // theta gradient: propagator part -2 sum_t dcoef_t/dtheta_j accT_t plus numerator
+    // part 2 Re sum_a v_a dK_a/dtheta_j conj(X_a)
+    for (int j = tid; j < f.ntheta; j += ASM_THREADS) {
+        double g = 0.0;
+        for (int t = 0; t < f.nth; ++t) {
+            const DThTerm& T = f.thterms[t];
+            const DBlock& B = f.blocks[T.block];
+            int li = -1;
+            for (int i = 0; i < B.nloc; ++i) if (B.vars[i] == j) { li = i; break; }
+            if (li < 0) continue;
+            int col = 1 + 2 * f.namp + t;
+            double acc = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
+            g -= 2.0 * f.table[f.off_rec + (long long)T.rec * REC + 1 + li] * acc;
+        }
         for (int a = 0; a < f.namp; ++a) {
             const DAmp& A = f.amps[a];
             const DBlock& B = f.blocks[A.block];
@@ -305,14 +382,12 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(FinArgs f) {
             if (li < 0) continue;
             double dK = f.table[f.off_rec + (long long)A.rec * REC + 1 + li];
             double vr = f.table[f.off_v + 2 * a], vi = f.table[f.off_v + 2 * a + 1];
-            // Re(v conj(X)) = vr Xr + vi Xi
-            g += 2.0 * dK * (vr * X[2 * a] + vi * X[2 * a + 1]);
+            g += 2.0 * dK * (vr * X[2 * a] + vi * X[2 * a + 1]);   // Re(v conj(X))
         }
         for (int q = 0; q < f.ngauss; ++q)
             if (f.g_slot[q] == j) g += (theta[j] - f.g_mu[q]) / (f.g_sig[q] * f.g_sig[q]);
         out[1 + 2 * f.nfree + j] = reset ? 0.0 : g;
     }
-    if (f.totals) for (int i = tid; i < 3 * f.nacc; i += FIN_THREADS) f.totals[i] = tot[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -383,17 +458,30 @@ __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __rest
 
 }  // namespace ctpwa
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep(const double* params, double* table, const ctpwa::PrepArgs* a,
-                                         hipStream_t stream) {
+template <int NV, bool H>
+static hipError_t launch_prep_t(const double* params, double* table, const ctpwa::PrepArgs* a, hipStream_t stream) {
     int items = a->nden + a->namp;
     int bs = 64;
-    hipLaunchKernelGGL(ctpwa::k_prep, dim3((items + bs - 1) / bs), dim3(bs), 0, stream, params, table, *a);
+    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3((items + bs - 1) / bs), dim3(bs), 0, stream, params, table, *a);
     return hipGetLastError();
 }
 
+// nv: largest number of free parameters in one block; hess: also second derivatives
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep(const double* params, double* table,
+                                                                              const ctpwa::PrepArgs* a, int nv,
+                                                                              int hess, hipStream_t stream) {
+    if (nv <= 1) return hess ? launch_prep_t<1, true>(params, table, a, stream) : launch_prep_t<1, false>(params, table, a, stream);
+    if (nv <= 2) return hess ? launch_prep_t<2, true>(params, table, a, stream) : launch_prep_t<2, false>(params, table, a, stream);
+    if (nv <= 4) return hess ? launch_prep_t<4, true>(params, table, a, stream) : launch_prep_t<4, false>(params, table, a, stream);
+    return hess ? launch_prep_t<8, true>(params, table, a, stream) : launch_prep_t<8, false>(params, table, a, stream);
+}
+
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs* f, hipStream_t stream) {
-    size_t shmem = (size_t)(3 * f->nacc + 2 * f->namp) * sizeof(double);
-    hipLaunchKernelGGL(ctpwa::k_finalize, dim3(1), dim3(ctpwa::FIN_THREADS), shmem, stream, *f);
+    hipLaunchKernelGGL(ctpwa::k_reduce, dim3(3 * f->nacc), dim3(ctpwa::RED_THREADS), 0, stream, *f);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t shmem = (size_t)(2 * f->namp) * sizeof(double);
+    hipLaunchKernelGGL(ctpwa::k_assemble, dim3(1), dim3(ctpwa::ASM_THREADS), shmem, stream, *f);
     return hipGetLastError();
 }
 
