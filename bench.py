@@ -31,26 +31,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_intensity(plan_text, events, params):
-    import torch
-    from ctpwa_amd import _native
-    out = np.empty(events.shape[0])
-    step = 4_000_000
-    pt = torch.tensor(params, dtype=torch.float64, device="cuda")
-    for s in range(0, events.shape[0], step):
-        ev = events[s:s + step]
-        h = _native.Handle(plan_text, ev, ev[:1])
-        o = torch.empty(ev.shape[0], dtype=torch.float64, device="cuda")
-        h.intensity(pt.data_ptr(), 0, o.data_ptr(), torch.cuda.current_stream().cuda_stream)
-        out[s:s + ev.shape[0]] = o.cpu().numpy()
-        h.close()
-    return out
-
-
 def make_samples(workdir, name, n_data, n_phsp, n_bkg):
     from ctpwa_amd import workloads as W
     t0 = time.time()
-    cfg = W.write_workload(name, workdir, gpu_intensity, n_data=n_data, n_phsp=n_phsp, n_bkg=n_bkg)
+    cfg = W.write_workload(name, workdir, W.device_intensity, n_data=n_data, n_phsp=n_phsp, n_bkg=n_bkg)
     log("[bench] samples generated in %.1f s" % (time.time() - t0))
     return cfg
 
@@ -101,6 +85,9 @@ def main():
     ap.add_argument("--cpu-frac", type=float, default=0.05)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
+                         "known size that calibrates the HBM byte counters of a --pmc profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (events would be "
@@ -109,7 +96,7 @@ def main():
 
     import torch
     from ctpwa_amd import workloads as W
-    from ctpwa_amd.analysis import analysis
+    from ctpwa_amd import analysis
 
     spec = W.WORKLOADS[args.workload]
     n_data = spec["n_data"] if args.n_data is None else args.n_data
@@ -172,6 +159,28 @@ def main():
                                      % (lay["n_sl_tensors"], plan.npol, lay["n_invariants"]),
             "per_amplitude_bytes_formula": survey_bytes}
     log("[bench] fused kernel %.4f ms, %.1f GB/s of %d B" % (kern_ms, achieved, lay["bytes_per_eval"]))
+
+    if args.copy_probe_mb > 0:
+        nb = args.copy_probe_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: %d bytes read, %d bytes written" % (nb, nb))
+        del src, dst
+
+    # HBM traffic per launch of the fused kernel from the committed FETCH_SIZE/WRITE_SIZE
+    # profile of this workload (rocprofv3 --pmc runs, calibrated on a copy of known size)
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "traffic_ctpwa_eval_%s_fp64.txt" % args.workload)
+    if os.path.exists(tfile) and (n_data, n_phsp, n_bkg) == (spec["n_data"], spec["n_phsp"], spec["n_bkg"]):
+        with open(tfile) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch"):
+                    traffic = int(line.split("=")[1])
+    roof["traffic"] = traffic
 
     cpu = None
     if not args.no_cpu_baseline:

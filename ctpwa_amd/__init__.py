@@ -13,6 +13,6 @@ __version__ = "0.1.0"
 def __getattr__(name):
     # the analysis class pulls in torch and the native library lazily
     if name in ("analysis", "NLLFunction"):
-        from . import analysis as _a
-        return getattr(_a, name)
+        import importlib
+        return getattr(importlib.import_module("._analysis", __name__), name)
     raise AttributeError(name)

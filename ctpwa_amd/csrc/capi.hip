@@ -30,6 +30,30 @@ extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::Pre
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
+namespace ctpwa {
+struct HessArgs {
+    const double2* sl; const double* inv; long long ld; const double* table; long long off_v, off_rec;
+    const DAmp* amps; const DBlock* blocks; const DDen* dens; const DDenTerms* dterms; const DTerm* terms;
+    int namp, nblock, npol, ntheta;
+    const double* wts; double wconst; int cls;
+    double* J; double* U; double* Z; double* W; int C, NU, NZ;
+};
+struct HessAsm {
+    const double* G[3]; const double* Uu[3]; const double* Zs[3]; const double* gs[3];
+    const int2* klG; int nG; const int2* klU; int nUu;
+    int C, NU, NZ; int namp, nfree, ntheta, nblock;
+    const DAmp* amps; const DBlock* blocks; const double* table; long long off_v;
+    double Np, Wd, Wb; int ngauss; const int* g_slot; const double* g_sig;
+    double* Hext; double* out;
+};
+}  // namespace ctpwa
+extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, const double*, int, int, const int2*, int, int,
+                                          double*, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, const double*, int, double*, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_hess_assemble(const ctpwa::HessAsm*, hipStream_t);
+
 using namespace ctpwa;
 
 static thread_local std::string g_err;
@@ -184,6 +208,12 @@ struct ctpwa_handle {
     AuxArgs aux{};
     hipFunction_t fn_eval = nullptr;
     std::string src;
+    // Hessian workspace (allocated on the first ctpwa_hessian call)
+    bool hess_ready = false;
+    long long hchunk = 0;
+    int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0, hnwg = 0;
+    double *hJ = nullptr, *hU = nullptr, *hZ = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
+    int2 *hklG = nullptr, *hklU = nullptr;
     std::vector<int> used_inv;
 
     template <class T>
@@ -497,7 +527,29 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
 
         h->prep = PrepArgs{d_blocks, d_res, d_nodes2, d_topos2, d_slcL, d_slc0, d_dens, d_amps,
                            (int)dens.size(), P.namp, P.nfree, P.ntheta,
-                           (long long)M.off_v(), (long long)M.off_c(), (long long)M.off_rec()};
+                           (long long)M.off_v(), (long long)M.off_c(), (long long)M.off_rec(),
+                           nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        {
+            std::vector<char> blob;
+            auto put = [&](const void* p, size_t n) -> int {
+                size_t o = (blob.size() + 15) / 16 * 16;
+                blob.resize(o + (n + 15) / 16 * 16, 0);
+                if (n) std::memcpy(blob.data() + o, p, n);
+                return (int)o;
+            };
+            h->prep.o_blocks = put(blocks.data(), blocks.size() * sizeof(DBlock));
+            h->prep.o_res = put(res.data(), res.size() * sizeof(DRes));
+            h->prep.o_nodes = put(nodes.data(), nodes.size() * sizeof(DNode));
+            h->prep.o_topos = put(topos.data(), topos.size() * sizeof(DTopo));
+            h->prep.o_slcL = put(slcL.data(), slcL.size() * sizeof(int));
+            h->prep.o_slc0 = put(slc0.data(), slc0.size() * sizeof(int));
+            h->prep.o_dens = put(dens.data(), dens.size() * sizeof(DDen));
+            h->prep.o_amps = put(amps.data(), amps.size() * sizeof(DAmp));
+            if (blob.size() <= 48 * 1024) {
+                h->prep.meta = h->upload(blob);
+                h->prep.meta_bytes = (int)blob.size();
+            }
+        }
         h->fin = FinArgs{};
         h->fin.part = h->part;
         h->fin.nwg = h->nwg;
@@ -513,6 +565,37 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             for (const ThTerm& t : M.thterms) th.push_back(DThTerm{M.dens[t.den].block, M.dens[t.den].coef_base + t.coef});
             h->fin.thterms = h->upload(th);
             h->fin.nth = (int)th.size();
+            // CSR lists of the gradient contractions (see DGEnt)
+            auto local = [&](int b, int j) {
+                for (int i = 0; i < blocks[b].nloc; ++i) if (blocks[b].vars[i] == j) return i;
+                return -1;
+            };
+            std::vector<int> thp{0}, cpp{0};
+            std::vector<DGEnt> the, cpe;
+            for (int j = 0; j < P.ntheta; ++j) {
+                for (size_t t = 0; t < th.size(); ++t) {
+                    int li = local(th[t].block, j);
+                    if (li >= 0) the.push_back(DGEnt{1 + 2 * P.namp + (int)t, th[t].rec * REC + 1 + li, -2.0});
+                }
+                for (int a = 0; a < P.namp; ++a) {
+                    int li = local(amps[a].block, j);
+                    if (li >= 0) the.push_back(DGEnt{-a - 1, amps[a].rec * REC + 1 + li, 2.0});
+                }
+                thp.push_back((int)the.size());
+            }
+            for (int fi = 0; fi < P.nfree; ++fi) {
+                for (int a = 0; a < P.namp; ++a)
+                    if (amps[a].freeidx == fi) cpe.push_back(DGEnt{a, amps[a].rec * REC, amps[a].ratio});
+                cpp.push_back((int)cpe.size());
+            }
+            if (the.empty()) the.push_back(DGEnt{0, 0, 0.0});
+            if (cpe.empty()) cpe.push_back(DGEnt{0, 0, 0.0});
+            h->fin.thp = h->upload(thp);
+            h->fin.the = h->upload(the);
+            h->fin.cpp = h->upload(cpp);
+            h->fin.cpe = h->upload(cpe);
+            if ((size_t)h->nacc * sizeof(double) > 60 * 1024)
+                throw std::runtime_error("too many gradient accumulators for the assembly kernel");
         }
         h->fin.namp = P.namp;
         h->fin.nfree = P.nfree;
@@ -595,10 +678,114 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     return 0;
 }
 
+static std::vector<int2> upper_triangle(int n) {
+    std::vector<int2> v;
+    for (int a = 0; a < n; ++a)
+        for (int b = a; b < n; ++b) v.push_back(make_int2(a, b));
+    return v;
+}
+
+static void hess_setup(ctpwa_handle* h) {
+    const Model_& M = h->M;
+    const Plan& P = M.P;
+    h->hC = P.namp + P.ntheta;
+    h->hNU = 2 * P.namp + P.ntheta;
+    int nz = 1;
+    for (size_t b = 0; b < P.blocks.size(); ++b) {
+        if (P.blocks[b].nsl == 0) continue;
+        int nl = (int)M.bvars[b].size();
+        nz += P.blocks[b].nsl * 2 * nl + nl * (nl + 1) / 2;
+    }
+    h->hNZ = nz;
+    if (2 * h->hC > 500) throw std::runtime_error("Hessian: more than 250 Jacobian columns is not built");
+    {
+        std::vector<int> mult(P.nfree, 0);
+        for (int a = 0; a < P.namp; ++a)
+            if (++mult[P.amap[a]] > 64)
+                throw std::runtime_error("Hessian: more than 64 amplitudes share one free coupling");
+    }
+    auto klG = upper_triangle(2 * h->hC), klU = upper_triangle(h->hNU);
+    h->hnG = (int)klG.size();
+    h->hnUu = (int)klU.size();
+    h->hklG = h->upload(klG);
+    h->hklU = h->upload(klU);
+    long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
+    h->hchunk = std::min<long long>(maxn, 1 << 19);
+    h->hnwg = 512;
+    h->hJ = h->alloc<double>((size_t)h->hchunk * P.npol * 2 * h->hC);
+    h->hU = h->alloc<double>((size_t)h->hchunk * h->hNU);
+    h->hZ = h->alloc<double>((size_t)h->hchunk * h->hNZ);
+    h->hW = h->alloc<double>((size_t)h->hchunk * 2);
+    size_t maxcol = std::max<size_t>(2048, std::max<size_t>(h->hNU, h->hNZ));
+    h->hpart = h->alloc<double>((size_t)h->hnwg * maxcol);
+    // totals: per class G, Uu, Z, g
+    h->htot = h->alloc<double>((size_t)3 * (h->hnG + h->hnUu + h->hNZ + h->hNU));
+    h->hHext = h->alloc<double>((size_t)h->hNU * h->hNU);
+    h->hess_ready = true;
+}
+
 extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream) {
-    (void)params_dev; (void)hess_dev; (void)stream;
-    if (!h) return fail("ctpwa_hessian: NULL handle");
-    return fail("ctpwa_hessian: not available in this build");
+    if (!h || !params_dev || !hess_dev) return fail("ctpwa_hessian: NULL argument");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    try {
+        if (!h->hess_ready) hess_setup(h);
+        const Model_& M = h->M;
+        const Plan& P = M.P;
+        HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, M.maxloc, 1, s));
+        size_t per_class = (size_t)h->hnG + h->hnUu + h->hNZ + h->hNU;
+        HIPCHECK(hipMemsetAsync(h->htot, 0, 3 * per_class * sizeof(double), s));
+        HessArgs a{};
+        a.sl = h->sl; a.inv = h->inv; a.ld = h->ntot; a.table = h->table;
+        a.off_v = (long long)M.off_v(); a.off_rec = (long long)M.off_rec();
+        a.amps = h->aux.amps; a.blocks = h->aux.blocks; a.dens = h->aux.dens; a.dterms = h->aux.dterms; a.terms = h->aux.terms;
+        a.namp = P.namp; a.nblock = (int)P.blocks.size(); a.npol = P.npol; a.ntheta = P.ntheta;
+        a.J = h->hJ; a.U = h->hU; a.Z = h->hZ; a.W = h->hW; a.C = h->hC; a.NU = h->hNU; a.NZ = h->hNZ;
+        HessAsm as{};
+        for (int c = 0; c < 3; ++c) {
+            double* base = h->htot + c * per_class;
+            as.G[c] = base;
+            as.Uu[c] = base + h->hnG;
+            as.Zs[c] = base + h->hnG + h->hnUu;
+            as.gs[c] = base + h->hnG + h->hnUu + h->hNZ;
+            a.cls = c;
+            a.wts = (c > 0 && h->pw[c]) ? h->wts : nullptr;
+            a.wconst = h->wc[c];
+            for (long long s0 = 0; s0 < h->n[c]; s0 += h->hchunk) {
+                long long cnt = std::min(h->hchunk, h->n[c] - s0);
+                HIPCHECK(ctpwa_launch_hess_rows(&a, h->off[c] + s0, cnt, s));
+                for (int e0 = 0; e0 < h->hnG; e0 += 2048) {
+                    int ne = std::min(2048, h->hnG - e0);
+                    HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, cnt * P.npol, h->hW, 2, P.npol, h->hklG, e0, ne,
+                                                h->hpart, h->hnwg, s));
+                    HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.G[c] + e0, s));
+                }
+                if (c > 0) {
+                    for (int e0 = 0; e0 < h->hnUu; e0 += 2048) {
+                        int ne = std::min(2048, h->hnUu - e0);
+                        HIPCHECK(ctpwa_launch_wgram(h->hU, h->hNU, cnt, h->hW + 1, 2, 1, h->hklU, e0, ne, h->hpart,
+                                                    h->hnwg, s));
+                        HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.Uu[c] + e0, s));
+                    }
+                }
+                HIPCHECK(ctpwa_launch_wcolsum(h->hZ, h->hNZ, cnt, h->hW, 2, h->hpart, h->hnwg, s));
+                HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, h->hNZ, (double*)as.Zs[c], s));
+                HIPCHECK(ctpwa_launch_wcolsum(h->hU, h->hNU, cnt, nullptr, 0, h->hpart, h->hnwg, s));
+                HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, h->hNU, (double*)as.gs[c], s));
+            }
+        }
+        as.klG = h->hklG; as.nG = h->hnG; as.klU = h->hklU; as.nUu = h->hnUu;
+        as.C = h->hC; as.NU = h->hNU; as.NZ = h->hNZ;
+        as.namp = P.namp; as.nfree = P.nfree; as.ntheta = P.ntheta; as.nblock = (int)P.blocks.size();
+        as.amps = h->aux.amps; as.blocks = h->aux.blocks; as.table = h->table; as.off_v = (long long)M.off_v();
+        as.Np = (double)h->n[0]; as.Wd = h->Wd; as.Wb = h->Wb;
+        as.ngauss = (int)P.g_slot.size(); as.g_slot = h->fin.g_slot; as.g_sig = h->fin.g_sig;
+        as.Hext = h->hHext; as.out = hess_dev;
+        HIPCHECK(ctpwa_launch_hess_assemble(&as, s));
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(std::string("ctpwa_hessian: ") + e.what());
+    }
 }
 
 static int aux_call(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream, int intensity) {

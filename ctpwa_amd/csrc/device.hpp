@@ -70,7 +70,18 @@ struct PrepArgs {
     const DAmp* amps;
     int nden, namp, nfree, ntheta;
     long long off_v, off_c, off_rec;
+    // the same tables packed in one blob (16-byte aligned sections) that the prep kernel
+    // copies to LDS before use; meta_bytes == 0: read the tables from global memory
+    const void* meta;
+    int meta_bytes;
+    int o_blocks, o_res, o_nodes, o_topos, o_slcL, o_slc0, o_dens, o_amps;
 };
+
+// one term of a gradient contraction in the final assembly
+//   theta entries: idx >= 0  -> scale * dcoef * C[idx]          (propagator coefficient)
+//                  idx <  0  -> scale * dK * Re(v_a conj(X_a)), a = -idx - 1
+//   coupling entries: idx = amplitude a, scale = r_a, rec -> K_a
+struct DGEnt { int idx, rec; double scale; };
 
 struct FinArgs {
     const double* part;      // [nacc][nwg]
@@ -82,6 +93,10 @@ struct FinArgs {
     const DBlock* blocks;
     const DThTerm* thterms;
     int nth;
+    const int* thp;          // [ntheta + 1] CSR row pointers into the
+    const DGEnt* the;        //   theta-gradient entries
+    const int* cpp;          // [nfree + 1] CSR row pointers into the
+    const DGEnt* cpe;        //   coupling-gradient entries
     int namp, nfree, ntheta, ngauss;
     const int* g_slot;
     const double* g_mu;

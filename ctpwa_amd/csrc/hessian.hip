@@ -1,0 +1,465 @@
+// Hessian of the likelihood (replaces analysis::getHessian, src/Analysis.cu:2689-3378).
+//
+// With S_p = sum_a v_a A_a(p), I = sum_p |S_p|^2 and real parameters x = (Re v, Im v,
+// theta), every event needs
+//   dI/dx           = 2 Re <S, dS/dx>
+//   d2I/dx dy       = 2 Re sum_p [conj(dS/dy) dS/dx + conj(S) d2S/dx dy]
+// The first part of d2I is the real embedding of the complex Gram of the Jacobian
+// columns J = [A_1..A_n, dS/dtheta_1..dS/dtheta_P] (rows: event x polarisation), so
+// the sum over events is a weighted Gram  sum_e alpha_e J^H J  (a K = N*nPol tall-skinny
+// contraction).  The second part only couples (v, theta) and (theta, theta) and reduces
+// to per-event scalars: <S, dA_a/dtheta> = dF_a/dtheta conj(T_u) and
+// <S, d2S> = sum_a v_a d2F_a conj(T_u), T_u = sum_p conj(sl'_u,p) S_p.
+//
+// Per class the device forms
+//   G   = sum_e alpha_e Jhat^T Jhat      (Jhat = [Re J | Im J])
+//   U   = sum_e beta_e u u^T,  u = dI/dx
+//   Z   = sum_e alpha_e z,     z = second-part scalars and I
+//   g   = sum_e u
+// with alpha = w/I, beta = -w/I^2 (data, background: second derivative of w ln I) and
+// alpha = 1, beta = 0 (phase space), then assembles
+//   H = -H_data + H_bkg + N_eff [ A2 / (N_p pf) - g g^T / (N_p pf)^2 ] (+ Gaussian terms)
+// and maps it to the fit parameters (v_a = r_a p_f(a)).
+//
+// Events are processed in chunks; every sum is a fixed-order reduction.
+#include "device.hpp"
+
+namespace ctpwa {
+
+struct HessArgs {
+    // model
+    const double2* sl;
+    const double* inv;
+    long long ld;
+    const double* table;
+    long long off_v, off_rec;
+    const DAmp* amps;
+    const DBlock* blocks;
+    const DDen* dens;
+    const DDenTerms* dterms;
+    const DTerm* terms;
+    int namp, nblock, npol, ntheta;
+    // per-event weights of the class
+    const double* wts;       // per-event (global index) or null
+    double wconst;
+    int cls;
+    // outputs (chunk-local rows)
+    double* J;               // [rows = n_ev * npol][2C]
+    double* U;               // [n_ev][NU = 2 namp + ntheta]
+    double* Z;               // [n_ev][NZ]
+    double* W;               // [n_ev][2] alpha, beta
+    int C, NU, NZ;
+};
+
+struct Cd { double r, i; };
+__device__ inline Cd cm(Cd a, Cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ inline Cd ca(Cd a, Cd b) { return {a.r + b.r, a.i + b.i}; }
+__device__ inline Cd cs(Cd a, Cd b) { return {a.r - b.r, a.i - b.i}; }
+__device__ inline Cd csc(Cd a, double s) { return {a.r * s, a.i * s}; }
+__device__ inline Cd cinv(Cd a) { double q = 1.0 / (a.r * a.r + a.i * a.i); return {a.r * q, -a.i * q}; }
+__device__ inline Cd ccj(Cd a) { return {a.r, -a.i}; }
+
+__device__ inline double rec_v(const HessArgs& a, int rec) { return a.table[a.off_rec + (long long)rec * REC]; }
+__device__ inline double rec_g(const HessArgs& a, int rec, int i) { return a.table[a.off_rec + (long long)rec * REC + 1 + i]; }
+__device__ inline double rec_h(const HessArgs& a, int rec, int i, int k) {
+    return a.table[a.off_rec + (long long)rec * REC + 1 + NLOC + i * NLOC + k];
+}
+
+// propagator product of a block and its log-derivatives:
+//   d Psi / d theta_i = -Psi L_i,   d2 Psi = Psi (L_i L_k - M_ik)
+__device__ void block_terms(const HessArgs& a, const DBlock& B, long long e, Cd& psi, Cd* L, Cd* Mm) {
+    int nl = B.nloc;
+    psi = {1.0, 0.0};
+    for (int i = 0; i < nl; ++i) {
+        L[i] = {0.0, 0.0};
+        for (int k = 0; k < nl; ++k) Mm[i * NLOC + k] = {0.0, 0.0};
+    }
+    for (int dd = 0; dd < B.nden; ++dd) {
+        const DDen& D = a.dens[B.den0 + dd];
+        const DDenTerms& T = a.dterms[B.den0 + dd];
+        Cd Dv{0.0, 0.0}, dD[NLOC], d2D[NLOC * NLOC];
+        for (int i = 0; i < nl; ++i) {
+            dD[i] = {0.0, 0.0};
+            for (int k = 0; k < nl; ++k) d2D[i * NLOC + k] = {0.0, 0.0};
+        }
+        for (int t = 0; t < T.nre + T.nim; ++t) {
+            const DTerm& tm = a.terms[T.t0 + t];
+            double x = (tm.inv < 0) ? 1.0 : a.inv[(long long)tm.inv * a.ld + e];
+            bool im = t >= T.nre;           // D = sum re - i sum im
+            double c = (tm.coef < 0) ? tm.cval : rec_v(a, D.rec + tm.coef);
+            if (im) Dv.i -= c * x; else Dv.r += c * x;
+            if (tm.coef < 0) continue;
+            for (int i = 0; i < nl; ++i) {
+                double gi = rec_g(a, D.rec + tm.coef, i) * x;
+                if (im) dD[i].i -= gi; else dD[i].r += gi;
+                for (int k = 0; k < nl; ++k) {
+                    double hik = rec_h(a, D.rec + tm.coef, i, k) * x;
+                    if (im) d2D[i * NLOC + k].i -= hik; else d2D[i * NLOC + k].r += hik;
+                }
+            }
+        }
+        Cd pd = cinv(Dv);
+        psi = cm(psi, pd);
+        for (int i = 0; i < nl; ++i) {
+            Cd li = cm(pd, dD[i]);
+            L[i] = ca(L[i], li);
+            for (int k = 0; k < nl; ++k)
+                Mm[i * NLOC + k] = ca(Mm[i * NLOC + k], cs(cm(pd, d2D[i * NLOC + k]), cm(li, cm(pd, dD[k]))));
+        }
+    }
+}
+
+// F = K Psi, dF_i = Psi (dK_i - K L_i), d2F_ik = Psi [d2K_ik - dK_i L_k - dK_k L_i + K (L_i L_k - M_ik)]
+__device__ inline Cd amp_F(double K, Cd psi) { return csc(psi, K); }
+__device__ inline Cd amp_dF(const HessArgs& a, int rec, double K, Cd psi, const Cd* L, int i) {
+    Cd t = cs(Cd{rec_g(a, rec, i), 0.0}, csc(L[i], K));
+    return cm(psi, t);
+}
+__device__ inline Cd amp_d2F(const HessArgs& a, int rec, double K, Cd psi, const Cd* L, const Cd* Mm, int i, int k) {
+    Cd t{rec_h(a, rec, i, k), 0.0};
+    t = cs(t, csc(L[k], rec_g(a, rec, i)));
+    t = cs(t, csc(L[i], rec_g(a, rec, k)));
+    t = ca(t, csc(cs(cm(L[i], L[k]), Mm[i * NLOC + k]), K));
+    return cm(psi, t);
+}
+
+__global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, long long n) {
+    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    long long e = e0 + r;
+    const int C = a.C, NP = a.npol, na = a.namp;
+    double* Jr = a.J + (size_t)r * NP * 2 * C;
+    Cd S[MAXPOL];
+    for (int p = 0; p < NP; ++p) S[p] = {0.0, 0.0};
+    for (int p = 0; p < NP; ++p)
+        for (int c = 0; c < 2 * C; ++c) Jr[(size_t)p * 2 * C + c] = 0.0;
+    Cd psi, L[NLOC], Mm[NLOC * NLOC];
+    // pass 1: A columns, S, dS/dtheta columns
+    for (int b = 0; b < a.nblock; ++b) {
+        const DBlock& B = a.blocks[b];
+        if (B.nsl == 0) continue;
+        block_terms(a, B, e, psi, L, Mm);
+        for (int c = 0; c < B.nsl; ++c) {
+            int am = B.site + c;
+            const DAmp& A = a.amps[am];
+            double K = rec_v(a, A.rec);
+            Cd F = amp_F(K, psi);
+            Cd v{a.table[a.off_v + 2 * am], a.table[a.off_v + 2 * am + 1]};
+            Cd dF[NLOC];
+            for (int i = 0; i < B.nloc; ++i) dF[i] = cm(v, amp_dF(a, A.rec, K, psi, L, i));
+            for (int p = 0; p < NP; ++p) {
+                double2 s = a.sl[((long long)A.u * NP + p) * a.ld + e];
+                Cd sp{s.x, s.y};
+                Cd Aap = cm(F, sp);
+                Jr[(size_t)p * 2 * C + am] = Aap.r;
+                Jr[(size_t)p * 2 * C + C + am] = Aap.i;
+                S[p] = ca(S[p], cm(v, Aap));
+                for (int i = 0; i < B.nloc; ++i) {
+                    int j = B.vars[i];
+                    Cd d = cm(dF[i], sp);
+                    Jr[(size_t)p * 2 * C + na + j] += d.r;
+                    Jr[(size_t)p * 2 * C + C + na + j] += d.i;
+                }
+            }
+        }
+    }
+    double I = 0.0;
+    for (int p = 0; p < NP; ++p) I += S[p].r * S[p].r + S[p].i * S[p].i;
+    double w = (a.cls == 0) ? 1.0 : (a.wts ? a.wts[e] : a.wconst);
+    double alpha, beta;
+    if (a.cls == 0) { alpha = 1.0; beta = 0.0; }
+    else if (I == 0.0) { alpha = 0.0; beta = 0.0; }
+    else { alpha = w / I; beta = -w / (I * I); }
+    a.W[2 * r] = alpha;
+    a.W[2 * r + 1] = beta;
+    // pass 2: u = dI/dx and the second-part scalars
+    double* Ur = a.U + (size_t)r * a.NU;
+    double* Zr = a.Z + (size_t)r * a.NZ;
+    for (int j = 0; j < a.ntheta; ++j) Ur[2 * na + j] = 0.0;
+    int zq = 0;
+    for (int b = 0; b < a.nblock; ++b) {
+        const DBlock& B = a.blocks[b];
+        if (B.nsl == 0) continue;
+        block_terms(a, B, e, psi, L, Mm);
+        double Rb[NLOC * NLOC];
+        for (int q = 0; q < NLOC * NLOC; ++q) Rb[q] = 0.0;
+        for (int c = 0; c < B.nsl; ++c) {
+            int am = B.site + c;
+            const DAmp& A = a.amps[am];
+            double K = rec_v(a, A.rec);
+            Cd F = amp_F(K, psi);
+            Cd v{a.table[a.off_v + 2 * am], a.table[a.off_v + 2 * am + 1]};
+            // T_u = sum_p conj(sl) S ; conj(T_u) = <S, sl>
+            Cd T{0.0, 0.0};
+            for (int p = 0; p < NP; ++p) {
+                double2 s = a.sl[((long long)A.u * NP + p) * a.ld + e];
+                T = ca(T, cm(Cd{s.x, -s.y}, S[p]));
+            }
+            Cd cT = ccj(T);
+            Cd g = cm(ccj(F), T);                     // dI/dv_a* = <A_a, S>
+            Ur[am] = 2.0 * g.r;
+            Ur[na + am] = 2.0 * g.i;
+            for (int i = 0; i < B.nloc; ++i) {
+                Cd dF = amp_dF(a, A.rec, K, psi, L, i);
+                Cd q = cm(dF, cT);                    // <S, dA_a/dtheta_i>
+                Zr[zq++] = q.r;
+                Zr[zq++] = q.i;
+                Cd vq = cm(v, q);                     // contributes to dI/dtheta
+                Ur[2 * na + B.vars[i]] += 2.0 * vq.r;
+                for (int k = i; k < B.nloc; ++k) {
+                    Cd d2 = cm(v, cm(amp_d2F(a, A.rec, K, psi, L, Mm, i, k), cT));
+                    Rb[i * NLOC + k] += d2.r;
+                }
+            }
+        }
+        for (int i = 0; i < B.nloc; ++i)
+            for (int k = i; k < B.nloc; ++k) Zr[zq++] = Rb[i * NLOC + k];
+    }
+    Zr[zq++] = I;
+}
+
+// weighted Gram of a row-major matrix: part[wg][entry] = sum_rows w(row/rep) X[row][k] X[row][l]
+// for the entries [ent0, ent0 + nent) of the upper-triangle list kl[].
+constexpr int GR_THREADS = 256;
+constexpr int GR_EPT = 8;
+constexpr int GR_TILE = 16;
+
+__global__ void __launch_bounds__(GR_THREADS)
+k_wgram(const double* __restrict__ X, int ncol, long long nrows, const double* __restrict__ Wt, int wstride,
+        int rep, const int2* __restrict__ kl, int ent0, int nent, double* __restrict__ part) {
+    extern __shared__ double tile[];        // [GR_TILE][ncol] + [GR_TILE] weights
+    double* wt = tile + GR_TILE * ncol;
+    int nwg = gridDim.x, wg = blockIdx.x;
+    long long per = (nrows + nwg - 1) / nwg;
+    long long r0 = (long long)wg * per, r1 = r0 + per < nrows ? r0 + per : nrows;
+    double acc[GR_EPT];
+    int2 ids[GR_EPT];
+    for (int q = 0; q < GR_EPT; ++q) {
+        acc[q] = 0.0;
+        int ent = threadIdx.x + q * GR_THREADS;
+        ids[q] = (ent < nent) ? kl[ent0 + ent] : make_int2(-1, -1);
+    }
+    for (long long t0 = r0; t0 < r1; t0 += GR_TILE) {
+        int nt = (int)((r1 - t0) < GR_TILE ? (r1 - t0) : GR_TILE);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nt * ncol; q += GR_THREADS)
+            tile[q] = X[(t0 + q / ncol) * (long long)ncol + q % ncol];
+        for (int q = threadIdx.x; q < nt; q += GR_THREADS) wt[q] = Wt[((t0 + q) / rep) * wstride];
+        __syncthreads();
+        for (int q = 0; q < GR_EPT; ++q) {
+            if (ids[q].x < 0) continue;
+            double s = 0.0;
+            for (int rr = 0; rr < nt; ++rr) s += wt[rr] * tile[rr * ncol + ids[q].x] * tile[rr * ncol + ids[q].y];
+            acc[q] += s;
+        }
+    }
+    for (int q = 0; q < GR_EPT; ++q) {
+        int ent = threadIdx.x + q * GR_THREADS;
+        if (ent < nent) part[(long long)wg * nent + ent] = acc[q];
+    }
+}
+
+// weighted column sums: part[wg][c] = sum_rows w(row) X[row][c]  (w == null: weight 1)
+__global__ void __launch_bounds__(GR_THREADS)
+k_wcolsum(const double* __restrict__ X, int ncol, long long nrows, const double* __restrict__ Wt, int wstride,
+          double* __restrict__ part) {
+    int nwg = gridDim.x, wg = blockIdx.x;
+    long long per = (nrows + nwg - 1) / nwg;
+    long long r0 = (long long)wg * per, r1 = r0 + per < nrows ? r0 + per : nrows;
+    for (int c = threadIdx.x; c < ncol; c += GR_THREADS) {
+        double s = 0.0;
+        for (long long r = r0; r < r1; ++r) s += (Wt ? Wt[r * wstride] : 1.0) * X[r * ncol + c];
+        part[(long long)wg * ncol + c] = s;
+    }
+}
+
+// tot[c] += sum_wg part[wg][c]  (fixed order)
+__global__ void k_accum_parts(const double* __restrict__ part, int nwg, int ncol, double* __restrict__ tot) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    double s = 0.0;
+    for (int g = 0; g < nwg; ++g) s += part[(long long)g * ncol + c];
+    tot[c] += s;
+}
+
+struct HessAsm {
+    // per class c: G[c] (upper-triangle list over 2C), Uu[c] (upper triangle over NU),
+    // Z[c] (NZ), g[c] (NU)
+    const double* G[3];
+    const double* Uu[3];
+    const double* Zs[3];
+    const double* gs[3];
+    const int2* klG;
+    int nG;
+    const int2* klU;
+    int nUu;
+    int C, NU, NZ;
+    int namp, nfree, ntheta, nblock;
+    const DAmp* amps;
+    const DBlock* blocks;
+    const double* table;
+    long long off_v;
+    double Np, Wd, Wb;
+    int ngauss;
+    const int* g_slot;
+    const double* g_sig;
+    double* Hext;            // [NU][NU] scratch
+    double* out;             // [np][np]
+};
+
+__device__ inline double gram_at(const double* G, int n2, int k, int l) {
+    // index of (min, max) in the row-major upper-triangle list of an n2 x n2 matrix
+    int a = k < l ? k : l, b = k < l ? l : k;
+    long long idx = (long long)a * n2 - (long long)a * (a - 1) / 2 + (b - a);
+    return G[idx];
+}
+
+// sum_e alpha d2I[x][y] for one class from its Gram and second-part sums
+__device__ double d2I_sum(const HessAsm& h, int c, int x, int y) {
+    const int n = h.namp, C = h.C, C2 = 2 * C;
+    const double* G = h.G[c];
+    // complex Gram entry G[l][k] = sum conj(J_l) J_k
+    auto Gre = [&](int l, int k) { return gram_at(G, C2, l, k) + gram_at(G, C2, C + l, C + k); };
+    auto Gim = [&](int l, int k) { return gram_at(G, C2, l, C + k) - gram_at(G, C2, C + l, k); };
+    // column index and factor (1 or i) of a real parameter
+    auto col = [&](int p, int& k, bool& im) {
+        if (p < n) { k = p; im = false; }
+        else if (p < 2 * n) { k = p - n; im = true; }
+        else { k = n + (p - 2 * n); im = false; }
+    };
+    int kx, ly;
+    bool ix, iy;
+    col(x, kx, ix);
+    col(y, ly, iy);
+    // 2 Re(conj(c_y) c_x G[ly][kx])
+    double gr = Gre(ly, kx), gi = Gim(ly, kx);
+    double v;
+    if (!ix && !iy) v = 2.0 * gr;
+    else if (!ix && iy) v = 2.0 * gi;
+    else if (ix && !iy) v = -2.0 * gi;
+    else v = 2.0 * gr;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_hess_assemble(HessAsm h) {
+    const int n = h.namp, NU = h.NU, P = h.ntheta;
+    // phase-space mean
+    double Np = h.Np, Neff = h.Wd - h.Wb;
+    double pf = (Np > 0) ? h.Zs[0][h.NZ - 1] / Np : 0.0;
+    if (!(pf > 1e-30) || isinf(pf)) pf = 1e-30;
+    double cp = (Np > 0) ? Neff / (Np * pf) : 0.0;
+    double cq = (Np > 0) ? -Neff / (Np * pf * Np * pf) : 0.0;
+    for (int q = threadIdx.x; q < NU * NU; q += blockDim.x) {
+        int x = q / NU, y = q - x * NU;
+        double H = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            double s = (c == 1) ? -1.0 : 1.0;        // -data + bkg; phase space scaled below
+            double v = d2I_sum(h, c, x, y);
+            if (c != 0) v += gram_at(h.Uu[c], NU, x, y);
+            if (c == 0) H += cp * v + cq * h.gs[0][x] * h.gs[0][y];
+            else H += s * v;
+        }
+        h.Hext[q] = H;
+    }
+    __syncthreads();
+    // second-part terms: (v_a, theta_j) and (theta_j, theta_k)
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 3; ++c) {
+            double s = (c == 0) ? cp : ((c == 1) ? -1.0 : 1.0);
+            const double* Z = h.Zs[c];
+            int zq = 0;
+            for (int b = 0; b < h.nblock; ++b) {
+                const DBlock& B = h.blocks[b];
+                if (B.nsl == 0) continue;
+                for (int cc = 0; cc < B.nsl; ++cc) {
+                    int am = B.site + cc;
+                    double vr = h.table[h.off_v + 2 * am], vi = h.table[h.off_v + 2 * am + 1];
+                    for (int i = 0; i < B.nloc; ++i) {
+                        double qr = Z[zq++], qi = Z[zq++];
+                        int j = 2 * n + B.vars[i];
+                        // d2S/dRe v_a dtheta = dA_a/dtheta: 2 Re Q ; d2S/dIm v_a dtheta = i dA: -2 Im Q
+                        double re = 2.0 * s * qr, im = -2.0 * s * qi;
+                        h.Hext[(long long)am * NU + j] += re;
+                        h.Hext[(long long)j * NU + am] += re;
+                        h.Hext[(long long)(n + am) * NU + j] += im;
+                        h.Hext[(long long)j * NU + n + am] += im;
+                        (void)vr; (void)vi;
+                    }
+                }
+                for (int i = 0; i < B.nloc; ++i)
+                    for (int k = i; k < B.nloc; ++k) {
+                        double rv = 2.0 * s * Z[zq++];
+                        int j1 = 2 * n + B.vars[i], j2 = 2 * n + B.vars[k];
+                        h.Hext[(long long)j1 * NU + j2] += rv;
+                        if (j1 != j2) h.Hext[(long long)j2 * NU + j1] += rv;
+                    }
+            }
+        }
+        for (int g = 0; g < h.ngauss; ++g) {
+            int j = 2 * n + h.g_slot[g];
+            h.Hext[(long long)j * NU + j] += 1.0 / (h.g_sig[g] * h.g_sig[g]);
+        }
+    }
+    __syncthreads();
+    // fit-parameter layout [Re p, Im p, theta]; v_a = r_a p_f(a)
+    const int nf = h.nfree, np = 2 * nf + P;
+    for (int q = threadIdx.x; q < np * np; q += blockDim.x) {
+        int r = q / np, s = q - r * np;
+        double H = 0.0;
+        auto sel = [&](int pidx, int* xs, double* rs) -> int {
+            int cnt = 0;
+            if (pidx >= 2 * nf) { xs[0] = 2 * n + (pidx - 2 * nf); rs[0] = 1.0; return 1; }
+            int f = pidx < nf ? pidx : pidx - nf;
+            int off = pidx < nf ? 0 : n;
+            for (int a = 0; a < n && cnt < 64; ++a)
+                if (h.amps[a].freeidx == f) { xs[cnt] = off + a; rs[cnt] = h.amps[a].ratio; ++cnt; }
+            return cnt;
+        };
+        int xr[64], xs[64];
+        double rr[64], rs[64];
+        int nr = sel(r, xr, rr), ns = sel(s, xs, rs);
+        for (int u = 0; u < nr; ++u)
+            for (int w = 0; w < ns; ++w) H += rr[u] * rs[w] * h.Hext[(long long)xr[u] * NU + xs[w]];
+        h.out[q] = H;
+    }
+}
+
+}  // namespace ctpwa
+
+using namespace ctpwa;
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_hess_rows(const HessArgs* a, long long e0,
+                                                                                   long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int bs = 128;
+    long long nb = (n + bs - 1) / bs;
+    hipLaunchKernelGGL(k_hess_rows, dim3((unsigned)nb), dim3(bs), 0, s, *a, e0, n);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wgram(const double* X, int ncol, long long nrows,
+                                                                               const double* Wt, int wstride, int rep,
+                                                                               const int2* kl, int ent0, int nent,
+                                                                               double* part, int nwg, hipStream_t s) {
+    size_t sh = (size_t)(GR_TILE * ncol + GR_TILE) * sizeof(double);
+    hipLaunchKernelGGL(k_wgram, dim3(nwg), dim3(GR_THREADS), sh, s, X, ncol, nrows, Wt, wstride, rep, kl, ent0, nent, part);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wcolsum(const double* X, int ncol, long long nrows,
+                                                                                 const double* Wt, int wstride,
+                                                                                 double* part, int nwg, hipStream_t s) {
+    hipLaunchKernelGGL(k_wcolsum, dim3(nwg), dim3(GR_THREADS), 0, s, X, ncol, nrows, Wt, wstride, part);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_accum(const double* part, int nwg, int ncol,
+                                                                               double* tot, hipStream_t s) {
+    hipLaunchKernelGGL(k_accum_parts, dim3((ncol + 255) / 256), dim3(256), 0, s, part, nwg, ncol, tot);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_hess_assemble(const HessAsm* h, hipStream_t s) {
+    hipLaunchKernelGGL(k_hess_assemble, dim3(1), dim3(256), 0, s, *h);
+    return hipGetLastError();
+}

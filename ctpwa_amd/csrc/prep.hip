@@ -201,6 +201,22 @@ __device__ inline void store(double* t, const DJ<NV, H>& x) {
 template <int NV, bool H>
 __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
     using J = DJ<NV, H>;
+    extern __shared__ uint4 smeta[];
+    if (a.meta_bytes > 0) {
+        // the model tables are read many times with dependent indices: stage them in LDS
+        const uint4* src = (const uint4*)a.meta;
+        for (int i = threadIdx.x; i < a.meta_bytes / 16; i += blockDim.x) smeta[i] = src[i];
+        __syncthreads();
+        const char* base = (const char*)smeta;
+        a.blocks = (const DBlock*)(base + a.o_blocks);
+        a.res = (const DRes*)(base + a.o_res);
+        a.nodes = (const DNode*)(base + a.o_nodes);
+        a.topos = (const DTopo*)(base + a.o_topos);
+        a.slcL = (const int*)(base + a.o_slcL);
+        a.slc0 = (const int*)(base + a.o_slc0);
+        a.dens = (const DDen*)(base + a.o_dens);
+        a.amps = (const DAmp*)(base + a.o_amps);
+    }
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     const double* theta = params + 2 * a.nfree;
     if (item < a.nden) {
@@ -323,9 +339,11 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(FinArgs f) {
     if (threadIdx.x == 0) f.totals[pair] = sh[0];
 }
 
-// stage 2: NLL and gradient from the class totals
+// stage 2: NLL and gradient from the class totals.  The class-combined accumulators
+// go to LDS first; the gradients are then CSR contractions over host-built entry lists
+// (independent loads, one wave per theta with a fixed-order butterfly sum).
 __global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) {
-    extern __shared__ double X[];            // combined X_a (re, im)
+    extern __shared__ double C[];            // [nacc] = -data + bkg + cp * phsp
     int tid = threadIdx.x;
     const double* tot = f.totals;            // tot[c * nacc + col]
     double Np = f.Np, Neff = f.Wd - f.Wb;
@@ -333,60 +351,52 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) {
     if (!(pf > 1e-30) || isinf(pf)) pf = 1e-30;
     double cp = (Np > 0) ? Neff / (pf * Np) : 0.0;
     const double* theta = f.params + 2 * f.nfree;
+    for (int col = tid; col < f.nacc; col += ASM_THREADS)
+        C[col] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
     double loss = -tot[f.nacc] + tot[2 * f.nacc] + Neff * log(pf);
     for (int g = 0; g < f.ngauss; ++g) {
         double d = theta[f.g_slot[g]] - f.g_mu[g];
         loss += d * d / (2.0 * f.g_sig[g] * f.g_sig[g]);
     }
     bool reset = isnan(loss) || isinf(loss);
-    // X_a = -data + bkg + cp * phsp   (column 1 + 2a is Re, 2 + 2a is Im)
-    for (int i = tid; i < 2 * f.namp; i += ASM_THREADS) {
-        int col = 1 + i;
-        X[i] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
-    }
     __syncthreads();
     double* out = f.out;
     if (tid == 0) out[0] = reset ? 1e30 : loss;
-    // coupling gradient: dL/dp_f* = sum_a r_a K_a X_a ; output [2 Re, 2 Im]
+    // coupling gradient: dL/dp_f* = sum_a r_a K_a X_a, X_a = (C[1 + 2a], C[2 + 2a]);
+    // output [2 Re, 2 Im]
     for (int fi = tid; fi < f.nfree; fi += ASM_THREADS) {
         double gr = 0.0, gi = 0.0;
-        for (int a = 0; a < f.namp; ++a) {
-            const DAmp& A = f.amps[a];
-            if (A.freeidx != fi) continue;
-            double K = f.table[f.off_rec + (long long)A.rec * REC];
-            gr += A.ratio * K * X[2 * a];
-            gi += A.ratio * K * X[2 * a + 1];
+        for (int k = f.cpp[fi]; k < f.cpp[fi + 1]; ++k) {
+            DGEnt e = f.cpe[k];
+            double w = e.scale * f.table[f.off_rec + e.rec];
+            gr += w * C[1 + 2 * e.idx];
+            gi += w * C[2 + 2 * e.idx];
         }
         out[1 + fi] = reset ? 0.0 : 2.0 * gr;
         out[1 + f.nfree + fi] = reset ? 0.0 : 2.0 * gi;
     }
-    // theta gradient: propagator part -2 sum_t dcoef_t/dtheta_j accT_t plus numerator
-    // part 2 Re sum_a v_a dK_a/dtheta_j conj(X_a)
-    for (int j = tid; j < f.ntheta; j += ASM_THREADS) {
+    // theta gradient: propagator part -2 sum_t dcoef_t/dtheta_j C_t plus numerator part
+    // 2 Re sum_a v_a dK_a/dtheta_j conj(X_a)
+    const int lane = tid & 63, nw = ASM_THREADS / 64;
+    for (int j = tid >> 6; j < f.ntheta; j += nw) {
         double g = 0.0;
-        for (int t = 0; t < f.nth; ++t) {
-            const DThTerm& T = f.thterms[t];
-            const DBlock& B = f.blocks[T.block];
-            int li = -1;
-            for (int i = 0; i < B.nloc; ++i) if (B.vars[i] == j) { li = i; break; }
-            if (li < 0) continue;
-            int col = 1 + 2 * f.namp + t;
-            double acc = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
-            g -= 2.0 * f.table[f.off_rec + (long long)T.rec * REC + 1 + li] * acc;
+        for (int k = f.thp[j] + lane; k < f.thp[j + 1]; k += 64) {
+            DGEnt e = f.the[k];
+            double d = f.table[f.off_rec + e.rec];
+            if (e.idx >= 0) {
+                g += e.scale * d * C[e.idx];
+            } else {
+                int a = -e.idx - 1;
+                double vr = f.table[f.off_v + 2 * a], vi = f.table[f.off_v + 2 * a + 1];
+                g += e.scale * d * (vr * C[1 + 2 * a] + vi * C[2 + 2 * a]);
+            }
         }
-        for (int a = 0; a < f.namp; ++a) {
-            const DAmp& A = f.amps[a];
-            const DBlock& B = f.blocks[A.block];
-            int li = -1;
-            for (int i = 0; i < B.nloc; ++i) if (B.vars[i] == j) { li = i; break; }
-            if (li < 0) continue;
-            double dK = f.table[f.off_rec + (long long)A.rec * REC + 1 + li];
-            double vr = f.table[f.off_v + 2 * a], vi = f.table[f.off_v + 2 * a + 1];
-            g += 2.0 * dK * (vr * X[2 * a] + vi * X[2 * a + 1]);   // Re(v conj(X))
+        for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o);
+        if (lane == 0) {
+            for (int q = 0; q < f.ngauss; ++q)
+                if (f.g_slot[q] == j) g += (theta[j] - f.g_mu[q]) / (f.g_sig[q] * f.g_sig[q]);
+            out[1 + 2 * f.nfree + j] = reset ? 0.0 : g;
         }
-        for (int q = 0; q < f.ngauss; ++q)
-            if (f.g_slot[q] == j) g += (theta[j] - f.g_mu[q]) / (f.g_sig[q] * f.g_sig[q]);
-        out[1 + 2 * f.nfree + j] = reset ? 0.0 : g;
     }
 }
 
@@ -462,7 +472,8 @@ template <int NV, bool H>
 static hipError_t launch_prep_t(const double* params, double* table, const ctpwa::PrepArgs* a, hipStream_t stream) {
     int items = a->nden + a->namp;
     int bs = 64;
-    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3((items + bs - 1) / bs), dim3(bs), 0, stream, params, table, *a);
+    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3((items + bs - 1) / bs), dim3(bs), (size_t)a->meta_bytes, stream,
+                       params, table, *a);
     return hipGetLastError();
 }
 
@@ -480,7 +491,7 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finaliz
     hipLaunchKernelGGL(ctpwa::k_reduce, dim3(3 * f->nacc), dim3(ctpwa::RED_THREADS), 0, stream, *f);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    size_t shmem = (size_t)(2 * f->namp) * sizeof(double);
+    size_t shmem = (size_t)f->nacc * sizeof(double);
     hipLaunchKernelGGL(ctpwa::k_assemble, dim3(1), dim3(ctpwa::ASM_THREADS), shmem, stream, *f);
     return hipGetLastError();
 }

@@ -11,10 +11,10 @@ def plan_of(name_or_text):
     return build_plan(ConfigParser("", text=text))
 
 
-def samples(n_phsp, n_data, n_bkg, seed):
-    ph = W.phase_space(n_phsp, seed=seed)
-    da = W.phase_space(n_data, seed=seed + 1)
-    bk = W.phase_space(n_bkg, seed=seed + 2) if n_bkg else None
+def samples(n_phsp, n_data, n_bkg, seed, masses=None):
+    ph = W.phase_space(n_phsp, seed=seed, masses=masses)
+    da = W.phase_space(n_data, seed=seed + 1, masses=masses)
+    bk = W.phase_space(n_bkg, seed=seed + 2, masses=masses) if n_bkg else None
     return ph, da, bk
 
 

@@ -70,3 +70,12 @@ def test_create_without_device_fails_cleanly():
     ev = np.zeros((4, 3, 4))
     with pytest.raises(_native.NativeError):
         _native.Handle(plan.to_text(), ev, ev)
+
+
+def test_package_exposes_analysis_class():
+    """``import ctpwa_amd as ctpwa; ctpwa.analysis(...)`` names the class (src/main.cu:90-131)."""
+    import ctpwa_amd as ctpwa
+    assert isinstance(ctpwa.analysis, type)
+    assert ctpwa.analysis.__name__ == "analysis"
+    for m in ("getNLL", "getHessian", "getNVector", "getNFreeTheta"):
+        assert callable(getattr(ctpwa.analysis, m))
