@@ -8,9 +8,11 @@ rho(770), rho(1450), rho(1700), plus f0(980) and f2(1270) in pi+pi-; 10 amplitud
 
 A step is one NLL+gradient evaluation at a new parameter point (couplings and
 resonance masses/widths changed) through ``analysis.getNLL`` + ``torch.autograd.grad``,
-events resident in HBM.  Prints one JSON line.
+events resident in HBM.  Both forms run on the same events: the mixed form (FP32
+amplitude tensors, FP64 arithmetic; ``value``) and the FP64 form (``fp64``), with the
+NLL difference at the same point.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--precision both|mixed|fp64]
 """
 from __future__ import annotations
 
@@ -73,48 +75,8 @@ def cpu_baseline(cfg_path, frac, threads):
                       "in events to the full workload" % (100 * frac, n, full, t_sample)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="jpsi_3pi")
-    ap.add_argument("--n-data", type=int, default=None)
-    ap.add_argument("--n-phsp", type=int, default=None)
-    ap.add_argument("--n-bkg", type=int, default=None)
-    ap.add_argument("--cpu-frac", type=float, default=0.05)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workdir", default=None)
-    ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
-                         "known size that calibrates the HBM byte counters of a --pmc profile)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (events would be "
-              "sharded per GPU with one all-reduce per evaluation); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    from ctpwa_amd import workloads as W
-    from ctpwa_amd import analysis
-
-    spec = W.WORKLOADS[args.workload]
-    n_data = spec["n_data"] if args.n_data is None else args.n_data
-    n_phsp = spec["n_phsp"] if args.n_phsp is None else args.n_phsp
-    n_bkg = spec["n_bkg"] if args.n_bkg is None else args.n_bkg
-    tmp = None
-    workdir = args.workdir
-    if workdir is None:
-        tmp = tempfile.TemporaryDirectory(prefix="ctpwa_bench_")
-        workdir = tmp.name
-    cfg = make_samples(workdir, args.workload, n_data, n_phsp, n_bkg)
-    samples = {k: np.load(os.path.join(workdir, k + ".npy")) for k in ("phsp", "data", "bkg")
-               if os.path.exists(os.path.join(workdir, k + ".npy"))}
-    t0 = time.time()
-    ana = analysis(cfg, samples=samples)
-    torch.cuda.synchronize()
-    log("[bench] analysis initialised in %.1f s" % (time.time() - t0))
-
+def run_form(ana, args, spec, torch, W, prec, sizes):
+    """Time K NLL+gradient evaluations at perturbed points in one precision form."""
     plan = ana.plan
     p0 = W.nominal_params(plan, None if args.workload == "jpsi_rho" else spec["seed"])
     rng = np.random.default_rng(123)
@@ -143,22 +105,104 @@ def main():
     ms_per_step = dt / args.steps * 1e3
     value = args.steps / dt
     nll_last = float(last[0].item())
-    log("[bench] %d steps in %.3f s: %.3f ms/step, NLL %.6f" % (args.steps, dt, ms_per_step, nll_last))
+    log("[bench] %s: %d steps in %.4f s: %.4f ms/step, NLL %.6f" % (prec, args.steps, dt, ms_per_step, nll_last))
 
     # dominant kernel: the fused per-event kernel, timed with HIP events on the stream it runs on
     lay = ana.layout()
     kern_ms = ana.kernelTimeMs(pts[0].detach(), iters=max(10, args.steps))
     achieved = lay["bytes_per_eval"] / (kern_ms * 1e-3) / 1e9
     nev = sum(lay["n_events"])
-    survey_bytes = nev * (16 * plan.npol * plan.n_amp + 8 * 3)
+    csl = 8 if prec == "mixed" else 16
+    survey_bytes = nev * (csl * plan.npol * plan.n_amp + 8 * 3)
+    # HBM traffic per launch of the fused kernel from the committed FETCH_SIZE/WRITE_SIZE
+    # profile of this workload and form (rocprofv3 --pmc runs, calibrated on a copy of
+    # known size)
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "traffic_ctpwa_eval_%s_%s.txt" % (args.workload, prec))
+    if os.path.exists(tfile) and sizes == (spec["n_data"], spec["n_phsp"], spec["n_bkg"]):
+        with open(tfile) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch"):
+                    traffic = int(line.split("=")[1])
+    if prec == "mixed":
+        bdef = ("per event: %d SL tensors x %d polarisations x 8 B (FP32 complex) + squared masses x 8 B "
+                "+ other invariants x 4 B" % (lay["n_sl_tensors"], plan.npol))
+    else:
+        bdef = ("per event: %d SL tensors x %d polarisations x 16 B + %d invariants x 8 B"
+                % (lay["n_sl_tensors"], plan.npol, lay["n_invariants"]))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "ctpwa_eval", "kernel_ms": round(kern_ms, 5),
-            "algorithmic_bytes": lay["bytes_per_eval"],
-            "algorithmic_bytes_def": "per event: %d SL tensors x %d polarisations x 16 B + %d invariants x 8 B"
-                                     % (lay["n_sl_tensors"], plan.npol, lay["n_invariants"]),
+            "algorithmic_bytes": lay["bytes_per_eval"], "algorithmic_bytes_def": bdef,
             "per_amplitude_bytes_formula": survey_bytes}
-    log("[bench] fused kernel %.4f ms, %.1f GB/s of %d B" % (kern_ms, achieved, lay["bytes_per_eval"]))
+    log("[bench] %s fused kernel %.4f ms, %.1f GB/s of %d B" % (prec, kern_ms, achieved, lay["bytes_per_eval"]))
+    return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="jpsi_3pi")
+    ap.add_argument("--n-data", type=int, default=None)
+    ap.add_argument("--n-phsp", type=int, default=None)
+    ap.add_argument("--n-bkg", type=int, default=None)
+    ap.add_argument("--cpu-frac", type=float, default=0.05)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workdir", default=None)
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
+                    help="both: time the mixed form (the reported value) and the FP64 form beside it")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
+                         "known size that calibrates the HBM byte counters of a --pmc profile)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (events would be "
+              "sharded per GPU with one all-reduce per evaluation); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ctpwa_amd import workloads as W
+    from ctpwa_amd import analysis
+
+    spec = W.WORKLOADS[args.workload]
+    n_data = spec["n_data"] if args.n_data is None else args.n_data
+    n_phsp = spec["n_phsp"] if args.n_phsp is None else args.n_phsp
+    n_bkg = spec["n_bkg"] if args.n_bkg is None else args.n_bkg
+    tmp = None
+    workdir = args.workdir
+    if workdir is None:
+        tmp = tempfile.TemporaryDirectory(prefix="ctpwa_bench_")
+        workdir = tmp.name
+    cfg = make_samples(workdir, args.workload, n_data, n_phsp, n_bkg)
+    precisions = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    samples = {k: np.load(os.path.join(workdir, k + ".npy")) for k in ("phsp", "data", "bkg")
+               if os.path.exists(os.path.join(workdir, k + ".npy"))}
+    plan = None
+    results = {}
+    for prec in precisions:
+        t0 = time.time()
+        ana = analysis(cfg, samples=samples, precision=prec)
+        torch.cuda.synchronize()
+        log("[bench] %s analysis initialised in %.1f s" % (prec, time.time() - t0))
+        plan = ana.plan
+        results[prec] = run_form(ana, args, spec, torch, W, prec, (n_data, n_phsp, n_bkg))
+        del ana
+        torch.cuda.empty_cache()
+    primary = precisions[0]
+    r = results[primary]
+    value, ms_per_step, nll_last, roof = r["value"], r["ms_per_step"], r["nll"], r["roofline"]
+    compare = None
+    if len(precisions) == 2:
+        other = results[precisions[1]]
+        compare = {"precision": precisions[1], "value": round(other["value"], 3),
+                   "ms_per_step": round(other["ms_per_step"], 4), "roofline": other["roofline"],
+                   "nll": other["nll"],
+                   "speedup_mixed_vs_fp64": round(r["value"] / other["value"], 3),
+                   "delta_nll_mixed_minus_fp64": r["nll"] - other["nll"],
+                   "delta_nll_points": "NLL at the same last timed parameter point in both forms"}
 
     if args.copy_probe_mb > 0:
         nb = args.copy_probe_mb << 20
@@ -170,18 +214,6 @@ def main():
         log("[bench] copy probe: %d bytes read, %d bytes written" % (nb, nb))
         del src, dst
 
-    # HBM traffic per launch of the fused kernel from the committed FETCH_SIZE/WRITE_SIZE
-    # profile of this workload (rocprofv3 --pmc runs, calibrated on a copy of known size)
-    traffic = None
-    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                         "traffic_ctpwa_eval_%s_fp64.txt" % args.workload)
-    if os.path.exists(tfile) and (n_data, n_phsp, n_bkg) == (spec["n_data"], spec["n_phsp"], spec["n_bkg"]):
-        with open(tfile) as f:
-            for line in f:
-                if line.startswith("bytes_per_launch"):
-                    traffic = int(line.split("=")[1])
-    roof["traffic"] = traffic
-
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
@@ -189,6 +221,9 @@ def main():
         cpu = cpu_baseline(cfg, args.cpu_frac, threads)
         log("[bench] cpu baseline %.4f evals/s (%.1f s)" % (cpu["value"], time.time() - t0))
 
+    desc = {"mixed": "mixed: FP32 amplitude (SL) tensors and invariants, FP64 squared masses, "
+                     "FP64 arithmetic and sums",
+            "fp64": "fp64: FP64 storage and arithmetic"}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -197,11 +232,13 @@ def main():
         "config": {"workload": args.workload + ": J/psi -> pi+ pi- pi0, 3 chains x (rho770, rho1450, "
                    "rho1700) + f0(980) + f2(1270), %d amplitudes, %d free resonance parameters, "
                    "%d data / %d phase space / %d background (w=0.8)" % (plan.n_amp, plan.n_theta, n_data, n_phsp, n_bkg),
-                   "n_data": n_data, "n_phsp": n_phsp, "n_bkg": n_bkg, "precision": "fp64",
+                   "n_data": n_data, "n_phsp": n_phsp, "n_bkg": n_bkg, "precision": desc[primary],
                    "nll": nll_last},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if compare is not None:
+        out["fp64"] = compare
     print(json.dumps(out), flush=True)
     if tmp is not None:
         tmp.cleanup()

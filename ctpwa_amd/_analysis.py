@@ -8,6 +8,8 @@ pybind11 module (src/main.cu:90-131) and ``class analysis`` (src/Analysis.cu:126
   whose backward gives the analytic gradient (NLLFunction, src/Analysis.cu:625-1261).
   A wrong dtype or device raises RuntimeError (TORCH_CHECK, src/Analysis.cu:1296-1301).
 * ``getHessian(params)``: [(2n+P), (2n+P)] float64 on the device, grouped layout.
+* ``analysis(config, precision="mixed")`` (or ``precision: float`` in the config) stores
+  the per-event amplitude tensors in FP32 and keeps every arithmetic step and sum in FP64.
 * ``getNVector() / getNFreeTheta() / getNParams() / getParamNames() /
   getFreeResParams() / getFitMode() / setFitMode() / getNPolarizations() /
   getAmplitudeNames() / isValid() / reCalcAmp(theta)``.
@@ -73,7 +75,7 @@ class analysis:
     """
 
     def __init__(self, config_file: str = "config.yml", samples: Optional[Dict[str, np.ndarray]] = None,
-                 device: Optional[int] = None):
+                 device: Optional[int] = None, precision: Optional[str] = None):
         self._valid = False
         self._fit_mode = 1
         self.config_file = config_file
@@ -100,14 +102,20 @@ class analysis:
                 w[key] = _read_weights(df[wk], base, n[key])
             else:
                 w[key] = None
+        # config `precision` (src/Analysis.cu:3925-3937): "double" and "auto" give the FP64
+        # form, "float" the mixed form (FP32 amplitude storage, FP64 arithmetic and sums);
+        # the `precision` argument ("fp64" | "mixed") overrides the config
         if self.cfg.precision not in ("auto", "double", "float"):
-            raise RuntimeError("precision must be auto, float or double")
+            raise RuntimeError("invalid precision in config (only float | double | auto)")
+        if precision is None:
+            precision = "mixed" if self.cfg.precision == "float" else "fp64"
+        self.precision = precision
         if device is None:
             device = torch.cuda.current_device() if torch.cuda.is_available() else 0
         self.device = torch.device("cuda", device)
         self.n_events = n
         self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
-                                 w["phsp"], w["data"], w["bkg"], device=device)
+                                 w["phsp"], w["data"], w["bkg"], device=device, precision=precision)
         self._valid = True
 
     # ---- reference API -------------------------------------------------------

@@ -39,7 +39,7 @@ def lib():
     L = ctypes.CDLL(LIB_PATH)
     L.ctpwa_create.restype = ctypes.c_int
     L.ctpwa_create.argtypes = [ctypes.c_char_p, _vp, _ll, _vp, _vp, _ll, _vp, _vp, _ll, _vp,
-                               ctypes.c_int, ctypes.POINTER(_vp)]
+                               ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
     L.ctpwa_destroy.argtypes = [_vp]
     L.ctpwa_destroy.restype = None
     L.ctpwa_dims.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4
@@ -50,15 +50,15 @@ def lib():
         getattr(L, f).argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     L.ctpwa_time_eval_kernel.argtypes = [_vp, _vp, ctypes.c_int, _vp, _dp]
     L.ctpwa_footprint_bytes.restype = _ll
-    L.ctpwa_footprint_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll]
+    L.ctpwa_footprint_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll, ctypes.c_int]
     L.ctpwa_layout.argtypes = [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll), ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.ctpwa_kernel_source.restype = ctypes.c_char_p
     L.ctpwa_kernel_source.argtypes = [_vp]
     L.ctpwa_generate_source.restype = _ll
-    L.ctpwa_generate_source.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _ll]
+    L.ctpwa_generate_source.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, _ll]
     L.ctpwa_compile_check.restype = _ll
-    L.ctpwa_compile_check.argtypes = [ctypes.c_char_p]
+    L.ctpwa_compile_check.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.ctpwa_last_error.restype = ctypes.c_char_p
     L.ctpwa_version.restype = ctypes.c_char_p
     _lib = L
@@ -74,24 +74,39 @@ def check(rc: int, what: str):
         raise NativeError("%s failed: %s" % (what, last_error()))
 
 
-def generate_source(plan_text: str) -> str:
-    n = lib().ctpwa_generate_source(plan_text.encode(), None, 0)
+# storage precision of the per-event inputs (arithmetic is FP64 in both)
+FP64 = 0
+MIXED = 1
+PRECISIONS = {"fp64": FP64, "double": FP64, "mixed": MIXED, "float": MIXED}
+
+
+def precision_code(precision) -> int:
+    if isinstance(precision, int) and precision in (FP64, MIXED):
+        return precision
+    if isinstance(precision, str) and precision.lower() in PRECISIONS:
+        return PRECISIONS[precision.lower()]
+    raise ValueError("precision must be 'fp64' or 'mixed', got %r" % (precision,))
+
+
+def generate_source(plan_text: str, precision=FP64) -> str:
+    pc = precision_code(precision)
+    n = lib().ctpwa_generate_source(plan_text.encode(), pc, None, 0)
     if n < 0:
         raise NativeError(last_error())
     buf = ctypes.create_string_buffer(int(n))
-    lib().ctpwa_generate_source(plan_text.encode(), buf, n)
+    lib().ctpwa_generate_source(plan_text.encode(), pc, buf, n)
     return buf.value.decode()
 
 
-def compile_check(plan_text: str) -> int:
-    n = lib().ctpwa_compile_check(plan_text.encode())
+def compile_check(plan_text: str, precision=FP64) -> int:
+    n = lib().ctpwa_compile_check(plan_text.encode(), precision_code(precision))
     if n < 0:
         raise NativeError(last_error())
     return int(n)
 
 
-def footprint_bytes(plan_text: str, n_phsp: int, n_data: int, n_bkg: int) -> int:
-    n = lib().ctpwa_footprint_bytes(plan_text.encode(), n_phsp, n_data, n_bkg)
+def footprint_bytes(plan_text: str, n_phsp: int, n_data: int, n_bkg: int, precision=FP64) -> int:
+    n = lib().ctpwa_footprint_bytes(plan_text.encode(), n_phsp, n_data, n_bkg, precision_code(precision))
     if n < 0:
         raise NativeError(last_error())
     return int(n)
@@ -107,7 +122,7 @@ class Handle:
     """Owns one native likelihood (device buffers, generated kernel)."""
 
     def __init__(self, plan_text: str, phsp, data, bkg=None, w_phsp=None, w_data=None, w_bkg=None,
-                 device: int = 0):
+                 device: int = 0, precision=FP64):
         import numpy as np
         L = lib()
         arrs = []
@@ -119,11 +134,13 @@ class Handle:
             if w is not None and w.shape[0] != n[k]:
                 raise ValueError("weight array length does not match its sample")
         h = ctypes.c_void_p()
+        pc = precision_code(precision)
         rc = L.ctpwa_create(plan_text.encode(), _addr(phsp), n[0], _addr(w_phsp), _addr(data), n[1],
-                            _addr(w_data), _addr(bkg), n[2], _addr(w_bkg), device, ctypes.byref(h))
+                            _addr(w_data), _addr(bkg), n[2], _addr(w_bkg), pc, device, ctypes.byref(h))
         check(rc, "ctpwa_create")
         self.h = h
         self.device = device
+        self.precision = pc
         a, f, t, p = (ctypes.c_int() for _ in range(4))
         check(L.ctpwa_dims(h, ctypes.byref(a), ctypes.byref(f), ctypes.byref(t), ctypes.byref(p)), "ctpwa_dims")
         self.n_amp, self.n_free, self.n_theta, self.n_pol = a.value, f.value, t.value, p.value

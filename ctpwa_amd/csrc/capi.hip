@@ -26,27 +26,13 @@ std::string generate_eval_kernel(const Model_& M);
 extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctpwa::DTopo*, int, const ctpwa::DNode*,
                                          const ctpwa::DKin*, int, const ctpwa::DU*, int, const double*, const int*, int,
                                          const ctpwa::DInv*, int, double2*, double*, long long, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_to_mixed(const double2*, const double*, long long, int, int, const int*, float2*,
+                                             double*, float*, long long, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
-namespace ctpwa {
-struct HessArgs {
-    const double2* sl; const double* inv; long long ld; const double* table; long long off_v, off_rec;
-    const DAmp* amps; const DBlock* blocks; const DDen* dens; const DDenTerms* dterms; const DTerm* terms;
-    int namp, nblock, npol, ntheta;
-    const double* wts; double wconst; int cls;
-    double* J; double* U; double* Z; double* W; int C, NU, NZ;
-};
-struct HessAsm {
-    const double* G[3]; const double* Uu[3]; const double* Zs[3]; const double* gs[3];
-    const int2* klG; int nG; const int2* klU; int nUu;
-    int C, NU, NZ; int namp, nfree, ntheta, nblock;
-    const DAmp* amps; const DBlock* blocks; const double* table; long long off_v;
-    double Np, Wd, Wb; int ngauss; const int* g_slot; const double* g_sig;
-    double* Hext; double* out;
-};
-}  // namespace ctpwa
+
 extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, const double*, int, int, const int2*, int, int,
                                           double*, int, hipStream_t);
@@ -215,6 +201,13 @@ struct ctpwa_handle {
     double *hJ = nullptr, *hU = nullptr, *hZ = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
     int2 *hklG = nullptr, *hklU = nullptr;
     std::vector<int> used_inv;
+    // storage precision (0 FP64, 1 mixed) and the FP32 stores of the mixed form
+    int mixed = 0;
+    float2* slf = nullptr;
+    float* invf = nullptr;
+    int* d_imap = nullptr;
+    const void* slp = nullptr;      // the SL store the fused kernel reads
+    EvStore store() const { return EvStore{sl, slf, inv, invf, d_imap, ntot, mixed}; }
 
     template <class T>
     T* alloc(size_t count) {
@@ -357,7 +350,8 @@ static bool constant_weights(const double* w, long long n, double* c) {
 
 extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long n_phsp, const double* w_phsp,
                             const double* data, long long n_data, const double* w_data,
-                            const double* bkg, long long n_bkg, const double* w_bkg, int device, ctpwa_handle** out) {
+                            const double* bkg, long long n_bkg, const double* w_bkg, int precision, int device,
+                            ctpwa_handle** out) {
     if (!out) return fail("ctpwa_create: out is NULL");
     *out = nullptr;
     ctpwa_handle* h = nullptr;
@@ -373,7 +367,10 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         HIPCHECK(hipSetDevice(device));
         h = new ctpwa_handle();
         h->device = device;
+        if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         h->M = derive(parse_plan(plan_text));
+        h->M.mixed = precision;
+        h->mixed = precision;
         apply_codegen_env(h->M);
         check_q0_masses(h->M);
         const Model_& M = h->M;
@@ -419,8 +416,20 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         std::vector<DInv> invs;
         build_tables(h, topos, nodes, kins, us, cgtab, invs);
         int nu = (int)us.size(), ninv = (int)invs.size();
-        h->sl = h->alloc<double2>((size_t)std::max(1, nu) * P.npol * h->ntot);
-        h->inv = h->alloc<double>((size_t)std::max(1, ninv) * h->ntot);
+        std::vector<int> imap = M.imap();
+        int nind = 0, ninf = 0;
+        for (int m : imap) (m >= 0 ? nind : ninf)++;
+        if (h->mixed) {
+            h->slf = h->alloc<float2>((size_t)std::max(1, nu) * P.npol * h->ntot);
+            h->inv = h->alloc<double>((size_t)std::max(1, nind) * h->ntot);
+            h->invf = h->alloc<float>((size_t)std::max(1, ninf) * h->ntot);
+            h->d_imap = h->upload(imap);
+            h->slp = h->slf;
+        } else {
+            h->sl = h->alloc<double2>((size_t)std::max(1, nu) * P.npol * h->ntot);
+            h->inv = h->alloc<double>((size_t)std::max(1, ninv) * h->ntot);
+            h->slp = h->sl;
+        }
         {
             DTopo* d_topos = h->upload(topos);
             DNode* d_nodes = h->upload(nodes);
@@ -433,24 +442,43 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             long long maxn = std::max(n_phsp, std::max(n_data, n_bkg));
             long long stage = std::min<long long>(maxn, 1 << 20);
             double* d_mom = nullptr;
+            double2* d_sl64 = nullptr;     // mixed: FP64 batch before rounding
+            double* d_inv64 = nullptr;
+            auto release = [&]() {
+                (void)hipFree(d_mom);
+                if (d_sl64) (void)hipFree(d_sl64);
+                if (d_inv64) (void)hipFree(d_inv64);
+            };
             HIPCHECK(hipMalloc(&d_mom, (size_t)stage * P.nfinal * 4 * sizeof(double)));
             try {
+                if (h->mixed) {
+                    HIPCHECK(hipMalloc(&d_sl64, (size_t)std::max(1, nu) * P.npol * stage * sizeof(double2)));
+                    HIPCHECK(hipMalloc(&d_inv64, (size_t)std::max(1, ninv) * stage * sizeof(double)));
+                }
                 for (int c = 0; c < 3; ++c) {
                     for (long long s0 = 0; s0 < ns[c]; s0 += stage) {
                         long long cnt = std::min(stage, ns[c] - s0);
                         HIPCHECK(hipMemcpy(d_mom, moms[c] + (size_t)s0 * P.nfinal * 4,
                                            (size_t)cnt * P.nfinal * 4 * sizeof(double), hipMemcpyHostToDevice));
-                        HIPCHECK(ctpwa_launch_init(d_mom, cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
-                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
-                                                   h->sl, h->inv, h->ntot, h->off[c] + s0, nullptr));
+                        if (h->mixed) {
+                            HIPCHECK(ctpwa_launch_init(d_mom, cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
+                                                       (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                       d_sl64, d_inv64, cnt, 0, nullptr));
+                            HIPCHECK(ctpwa_launch_to_mixed(d_sl64, d_inv64, cnt, nu * P.npol, ninv, h->d_imap, h->slf,
+                                                           h->inv, h->invf, h->ntot, h->off[c] + s0, nullptr));
+                        } else {
+                            HIPCHECK(ctpwa_launch_init(d_mom, cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
+                                                       (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                       h->sl, h->inv, h->ntot, h->off[c] + s0, nullptr));
+                        }
                         HIPCHECK(hipDeviceSynchronize());
                     }
                 }
             } catch (...) {
-                (void)hipFree(d_mom);
+                release();
                 throw;
             }
-            (void)hipFree(d_mom);
+            release();
         }
 
         // model tables for prep / finalize / aux
@@ -608,7 +636,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->fin.Wd = h->Wd;
         h->fin.Wb = h->Wb;
         h->fin.totals = h->alloc<double>((size_t)3 * h->nacc);
-        h->aux = AuxArgs{h->sl, h->inv, h->ntot, h->table, (long long)M.off_c(), (long long)M.off_rec(),
+        h->aux = AuxArgs{h->store(), h->table, (long long)M.off_c(), (long long)M.off_rec(),
                          d_amps, d_blocks, d_dens, d_dterms, d_terms, P.namp, P.npol};
 
         h->src = generate_eval_kernel(M);
@@ -652,8 +680,8 @@ extern "C" int ctpwa_param_info(const ctpwa_handle* h, double* init, double* low
 }
 
 static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s) {
-    void* args[] = {&h->table, &h->sl, &h->inv, &h->wts, &h->part, &h->wg_cls, &h->wg_rng, &h->ntot, &h->nwg,
-                    &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2]};
+    void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->wts, &h->part, &h->wg_cls, &h->wg_rng, &h->ntot,
+                    &h->nwg, &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2]};
     return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, 256, 1, 1, 0, s, args, nullptr);
 }
 
@@ -736,7 +764,7 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         size_t per_class = (size_t)h->hnG + h->hnUu + h->hNZ + h->hNU;
         HIPCHECK(hipMemsetAsync(h->htot, 0, 3 * per_class * sizeof(double), s));
         HessArgs a{};
-        a.sl = h->sl; a.inv = h->inv; a.ld = h->ntot; a.table = h->table;
+        a.st = h->store(); a.table = h->table;
         a.off_v = (long long)M.off_v(); a.off_rec = (long long)M.off_rec();
         a.amps = h->aux.amps; a.blocks = h->aux.blocks; a.dens = h->aux.dens; a.dterms = h->aux.dterms; a.terms = h->aux.terms;
         a.namp = P.namp; a.nblock = (int)P.blocks.size(); a.npol = P.npol; a.ntheta = P.ntheta;
@@ -828,13 +856,28 @@ extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev,
     return 0;
 }
 
-extern "C" long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg) {
+// per-event bytes of the SL tensors and the invariants a model stores at a precision
+static long long event_bytes(const Model_& M, const std::vector<int>* used) {
+    std::vector<int> imap = M.imap();
+    long long b = (long long)M.us.size() * M.P.npol * (M.mixed ? 8 : 16);
+    for (size_t i = 0; i < M.inv.size(); ++i)
+        if (!used || (*used)[i]) b += (imap[i] >= 0) ? 8 : 4;
+    return b;
+}
+
+extern "C" long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg,
+                                           int precision) {
     try {
+        if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         Model_ M = derive(parse_plan(plan_text));
+        M.mixed = precision;
         long long nt = n_phsp + n_data + n_bkg;
-        long long b = nt * (long long)M.us.size() * M.P.npol * 16 + nt * (long long)M.inv.size() * 8;
+        long long b = nt * event_bytes(M, nullptr);
         b += (long long)M.table_size() * 8 + (long long)(1 + 2 * M.P.namp + M.P.ntheta) * 1024 * 8;
-        b += std::min<long long>(std::max(n_phsp, std::max(n_data, n_bkg)), 1 << 20) * M.P.nfinal * 32;
+        // construction-time staging: momenta, and the FP64 batch of the mixed form
+        long long stage = std::min<long long>(std::max(n_phsp, std::max(n_data, n_bkg)), 1 << 20);
+        b += stage * M.P.nfinal * 32;
+        if (M.mixed) b += stage * ((long long)M.us.size() * M.P.npol * 16 + (long long)M.inv.size() * 8);
         return b;
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_footprint_bytes: ") + e.what();
@@ -849,7 +892,7 @@ extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long lo
     int used = 0;
     for (int u : h->used_inv) used += u;
     if (bytes_per_eval) {
-        long long b = h->ntot * ((long long)h->M.us.size() * h->M.P.npol * 16 + (long long)used * 8);
+        long long b = h->ntot * event_bytes(h->M, &h->used_inv);
         for (int c = 1; c < 3; ++c) if (h->pw[c]) b += h->n[c] * 8;
         *bytes_per_eval = b;
     }
@@ -862,9 +905,11 @@ extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long lo
 extern "C" const char* ctpwa_kernel_source(const ctpwa_handle* h) { return h ? h->src.c_str() : ""; }
 
 // source of the fused kernel for a plan, without a device (buf may be NULL to query the size)
-extern "C" long long ctpwa_generate_source(const char* plan_text, char* buf, long long cap) {
+extern "C" long long ctpwa_generate_source(const char* plan_text, int precision, char* buf, long long cap) {
     try {
+        if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         Model_ M = derive(parse_plan(plan_text));
+        M.mixed = precision;
         apply_codegen_env(M);
         std::string src = generate_eval_kernel(M);
         if (buf && cap > 0) {
@@ -881,9 +926,11 @@ extern "C" long long ctpwa_generate_source(const char* plan_text, char* buf, lon
 
 // compile the fused kernel for a plan with hipRTC (no device needed); returns the
 // code-object size in bytes, or -1 with the compiler log in ctpwa_last_error()
-extern "C" long long ctpwa_compile_check(const char* plan_text) {
+extern "C" long long ctpwa_compile_check(const char* plan_text, int precision) {
     try {
+        if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         Model_ M = derive(parse_plan(plan_text));
+        M.mixed = precision;
         apply_codegen_env(M);
         std::string src = generate_eval_kernel(M);
         hiprtcProgram prog;

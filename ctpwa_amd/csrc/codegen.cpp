@@ -57,9 +57,23 @@ std::string generate_eval_kernel(const Model_& M) {
     o << "#define OFFC " << M.off_c() << "LL\n#define OFFR " << M.off_rec() << "LL\n#define REC " << Model_::REC << "\n";
     o << "extern \"C\" __global__ void __launch_bounds__(256" << (M.min_waves > 0 ? ", " + std::to_string(M.min_waves) : std::string())
       << ") ctpwa_eval(\n"
-         "    const double* __restrict__ tab, const double2* __restrict__ sl, const double* __restrict__ inv,\n"
-         "    const double* __restrict__ wts, double* __restrict__ part, const int* __restrict__ wg_cls,\n"
-         "    const long long* __restrict__ wg_rng, long long ld, int nwg, double w1, double w2, int pw1, int pw2)\n{\n";
+         "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
+         "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
+         "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
+         "    double w1, double w2, int pw1, int pw2)\n{\n";
+    // storage types: FP64, or FP32 SL tensors / FP32 invariants except s (mixed)
+    const char* SLT = M.mixed ? "float2" : "double2";
+    o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n";
+    std::vector<int> imap = M.imap();
+    auto inv_load = [&](size_t i, const std::string& ev) {
+        std::ostringstream q;
+        if (imap[i] >= 0) q << "inv[" << imap[i] << "LL * ld + " << ev << "]";
+        else q << "(double)invf[" << (-imap[i] - 1) << "LL * ld + " << ev << "]";
+        return q.str();
+    };
+    auto sl_conv = [&](const std::string& x) {
+        return M.mixed ? "make_double2((double)" + x + ".x, (double)" + x + ".y)" : x;
+    };
     o << "  __shared__ double red[4][" << NACC << "];\n";
     o << "  const int wg = blockIdx.x;\n  const int cls = wg_cls[wg];\n"
          "  const long long e0 = wg_rng[2 * wg], e1 = wg_rng[2 * wg + 1];\n"
@@ -84,9 +98,9 @@ std::string generate_eval_kernel(const Model_& M) {
         o << "  long long ep = (e < e1) ? e : e0;\n";
         for (int u = 0; u < NU; ++u)
             for (int p = 0; p < NP; ++p)
-                o << "  double2 n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + ep];\n";
+                o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + ep];\n";
         for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "  double nx" << i << " = inv[" << i << "LL * ld + ep];\n";
+            if (used[i]) o << "  double nx" << i << " = " << inv_load(i, "ep") << ";\n";
     }
     o << "  for (; e < e1; e += 256) {\n";
     if (M.reload_coef) {
@@ -96,7 +110,8 @@ std::string generate_eval_kernel(const Model_& M) {
     }
     if (M.prefetch) {
         for (int u = 0; u < NU; ++u)
-            for (int p = 0; p < NP; ++p) o << "    const double2 s" << u << "_" << p << " = n" << u << "_" << p << ";\n";
+            for (int p = 0; p < NP; ++p)
+                o << "    const double2 s" << u << "_" << p << " = " << sl_conv("n" + std::to_string(u) + "_" + std::to_string(p)) << ";\n";
         for (size_t i = 0; i < M.inv.size(); ++i)
             if (used[i]) o << "    const double x" << i << " = nx" << i << ";\n";
         // issue the next event's loads (index clamped to the chunk: always a valid address)
@@ -105,14 +120,17 @@ std::string generate_eval_kernel(const Model_& M) {
             for (int p = 0; p < NP; ++p)
                 o << "      n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + en];\n";
         for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "      nx" << i << " = inv[" << i << "LL * ld + en];\n";
+            if (used[i]) o << "      nx" << i << " = " << inv_load(i, "en") << ";\n";
         o << "    }\n";
     } else {
         for (int u = 0; u < NU; ++u)
-            for (int p = 0; p < NP; ++p)
-                o << "    const double2 s" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+            for (int p = 0; p < NP; ++p) {
+                std::string nm = std::to_string(u) + "_" + std::to_string(p);
+                o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+                o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
+            }
         for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "    const double x" << i << " = inv[" << i << "LL * ld + e];\n";
+            if (used[i]) o << "    const double x" << i << " = " << inv_load(i, "e") << ";\n";
     }
     auto term_expr = [&](int d, const Term& t) {
         std::string coef = (t.coef < 0) ? D(t.cval) : ("k" + std::to_string(d) + "_" + std::to_string(t.coef));

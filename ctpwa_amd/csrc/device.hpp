@@ -107,10 +107,35 @@ struct FinArgs {
     double* totals;          // [3][nacc] scratch (kept for inspection)
 };
 
-struct AuxArgs {
-    const double2* sl;
-    const double* inv;
+// Per-event inputs of a handle: SL tensors [NU][npol][ld] and invariants [K][ld].
+// precision 0: everything FP64.  precision 1 (mixed): SL tensors FP32, invariants FP32
+// except the squared masses s, which stay FP64 (m0^2 - s cancels near a pole).
+struct EvStore {
+    const double2* sl;       // precision 0
+    const float2* slf;       // precision 1
+    const double* inv;       // precision 0: all invariants; 1: the FP64 ones
+    const float* invf;       // precision 1: the FP32 invariants
+    const int* imap;         // precision 1: invariant i -> row in inv (>= 0) or -(row in invf) - 1
     long long ld;
+    int mixed;
+};
+__device__ inline double2 ev_sl(const EvStore& s, long long row, long long e) {
+    if (s.mixed) {
+        float2 f = s.slf[row * s.ld + e];
+        return make_double2((double)f.x, (double)f.y);
+    }
+    return s.sl[row * s.ld + e];
+}
+__device__ inline double ev_inv(const EvStore& s, int i, long long e) {
+    if (s.mixed) {
+        int m = s.imap[i];
+        return (m >= 0) ? s.inv[(long long)m * s.ld + e] : (double)s.invf[(long long)(-m - 1) * s.ld + e];
+    }
+    return s.inv[(long long)i * s.ld + e];
+}
+
+struct AuxArgs {
+    EvStore st;
     const double* table;
     long long off_c, off_rec;
     const DAmp* amps;
@@ -119,6 +144,56 @@ struct AuxArgs {
     const DDenTerms* dterms;
     const DTerm* terms;
     int namp, npol;
+};
+
+// Hessian: per-event rows of one chunk of one class (csrc/hessian.hip)
+struct HessArgs {
+    // model
+    EvStore st;
+    const double* table;
+    long long off_v, off_rec;
+    const DAmp* amps;
+    const DBlock* blocks;
+    const DDen* dens;
+    const DDenTerms* dterms;
+    const DTerm* terms;
+    int namp, nblock, npol, ntheta;
+    // per-event weights of the class
+    const double* wts;       // per-event (global index) or null
+    double wconst;
+    int cls;
+    // outputs (chunk-local rows)
+    double* J;               // [rows = n_ev * npol][2C]
+    double* U;               // [n_ev][NU = 2 namp + ntheta]
+    double* Z;               // [n_ev][NZ]
+    double* W;               // [n_ev][2] alpha, beta
+    int C, NU, NZ;
+};
+
+// Hessian assembly from the per-class sums
+struct HessAsm {
+    // per class c: G[c] (upper-triangle list over 2C), Uu[c] (upper triangle over NU),
+    // Z[c] (NZ), g[c] (NU)
+    const double* G[3];
+    const double* Uu[3];
+    const double* Zs[3];
+    const double* gs[3];
+    const int2* klG;
+    int nG;
+    const int2* klU;
+    int nUu;
+    int C, NU, NZ;
+    int namp, nfree, ntheta, nblock;
+    const DAmp* amps;
+    const DBlock* blocks;
+    const double* table;
+    long long off_v;
+    double Np, Wd, Wb;
+    int ngauss;
+    const int* g_slot;
+    const double* g_sig;
+    double* Hext;            // [NU][NU] scratch
+    double* out;             // [np][np]
 };
 
 }  // namespace ctpwa

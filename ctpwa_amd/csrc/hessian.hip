@@ -26,30 +26,7 @@
 
 namespace ctpwa {
 
-struct HessArgs {
-    // model
-    const double2* sl;
-    const double* inv;
-    long long ld;
-    const double* table;
-    long long off_v, off_rec;
-    const DAmp* amps;
-    const DBlock* blocks;
-    const DDen* dens;
-    const DDenTerms* dterms;
-    const DTerm* terms;
-    int namp, nblock, npol, ntheta;
-    // per-event weights of the class
-    const double* wts;       // per-event (global index) or null
-    double wconst;
-    int cls;
-    // outputs (chunk-local rows)
-    double* J;               // [rows = n_ev * npol][2C]
-    double* U;               // [n_ev][NU = 2 namp + ntheta]
-    double* Z;               // [n_ev][NZ]
-    double* W;               // [n_ev][2] alpha, beta
-    int C, NU, NZ;
-};
+
 
 struct Cd { double r, i; };
 __device__ inline Cd cm(Cd a, Cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
@@ -84,7 +61,7 @@ __device__ void block_terms(const HessArgs& a, const DBlock& B, long long e, Cd&
         }
         for (int t = 0; t < T.nre + T.nim; ++t) {
             const DTerm& tm = a.terms[T.t0 + t];
-            double x = (tm.inv < 0) ? 1.0 : a.inv[(long long)tm.inv * a.ld + e];
+            double x = (tm.inv < 0) ? 1.0 : ev_inv(a.st, tm.inv, e);
             bool im = t >= T.nre;           // D = sum re - i sum im
             double c = (tm.coef < 0) ? tm.cval : rec_v(a, D.rec + tm.coef);
             if (im) Dv.i -= c * x; else Dv.r += c * x;
@@ -148,7 +125,7 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             Cd dF[NLOC];
             for (int i = 0; i < B.nloc; ++i) dF[i] = cm(v, amp_dF(a, A.rec, K, psi, L, i));
             for (int p = 0; p < NP; ++p) {
-                double2 s = a.sl[((long long)A.u * NP + p) * a.ld + e];
+                double2 s = ev_sl(a.st, (long long)A.u * NP + p, e);
                 Cd sp{s.x, s.y};
                 Cd Aap = cm(F, sp);
                 Jr[(size_t)p * 2 * C + am] = Aap.r;
@@ -192,7 +169,7 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             // T_u = sum_p conj(sl) S ; conj(T_u) = <S, sl>
             Cd T{0.0, 0.0};
             for (int p = 0; p < NP; ++p) {
-                double2 s = a.sl[((long long)A.u * NP + p) * a.ld + e];
+                double2 s = ev_sl(a.st, (long long)A.u * NP + p, e);
                 T = ca(T, cm(Cd{s.x, -s.y}, S[p]));
             }
             Cd cT = ccj(T);
@@ -282,30 +259,7 @@ __global__ void k_accum_parts(const double* __restrict__ part, int nwg, int ncol
     tot[c] += s;
 }
 
-struct HessAsm {
-    // per class c: G[c] (upper-triangle list over 2C), Uu[c] (upper triangle over NU),
-    // Z[c] (NZ), g[c] (NU)
-    const double* G[3];
-    const double* Uu[3];
-    const double* Zs[3];
-    const double* gs[3];
-    const int2* klG;
-    int nG;
-    const int2* klU;
-    int nUu;
-    int C, NU, NZ;
-    int namp, nfree, ntheta, nblock;
-    const DAmp* amps;
-    const DBlock* blocks;
-    const double* table;
-    long long off_v;
-    double Np, Wd, Wb;
-    int ngauss;
-    const int* g_slot;
-    const double* g_sig;
-    double* Hext;            // [NU][NU] scratch
-    double* out;             // [np][np]
-};
+
 
 __device__ inline double gram_at(const double* G, int n2, int k, int l) {
     // index of (min, max) in the row-major upper-triangle list of an n2 x n2 matrix

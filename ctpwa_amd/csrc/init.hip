@@ -386,7 +386,37 @@ k_init_events(const double* __restrict__ mom, long long N, int nfinal,
     }
 }
 
+// mixed precision: a batch of FP64 SL tensors and invariants (leading dimension n) is
+// rounded into the FP32 stores; invariants with imap >= 0 are copied to the FP64 store
+__global__ void __launch_bounds__(256)
+k_to_mixed(const double2* __restrict__ sl64, const double* __restrict__ inv64, long long n, int nrow, int ninv,
+           const int* __restrict__ imap, float2* __restrict__ slf, double* __restrict__ invd, float* __restrict__ invf,
+           long long ld, long long e_off) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    for (int r = 0; r < nrow; ++r) {
+        double2 v = sl64[(long long)r * n + e];
+        slf[(long long)r * ld + e_off + e] = make_float2((float)v.x, (float)v.y);
+    }
+    for (int i = 0; i < ninv; ++i) {
+        double v = inv64[(long long)i * n + e];
+        int m = imap[i];
+        if (m >= 0) invd[(long long)m * ld + e_off + e] = v;
+        else invf[(long long)(-m - 1) * ld + e_off + e] = (float)v;
+    }
+}
+
 }  // namespace ctpwa
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_to_mixed(
+    const double2* sl64, const double* inv64, long long n, int nrow, int ninv, const int* imap, float2* slf,
+    double* invd, float* invf, long long ld, long long e_off, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(ctpwa::k_to_mixed, dim3((unsigned)nb), dim3(256), 0, stream, sl64, inv64, n, nrow, ninv, imap,
+                       slf, invd, invf, ld, e_off);
+    return hipGetLastError();
+}
 
 // host launcher (called from capi.hip)
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_init(const double* mom, long long N, int nfinal,

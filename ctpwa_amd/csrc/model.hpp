@@ -155,6 +155,19 @@ struct Model_ {
     int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
     bool reload_coef = false; // reload coefficients per iteration (scalar loads) instead of pinning
     bool prefetch = false;    // load the next event's inputs while computing the current one
+    // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
+    // invariants, FP64 squared masses)
+    int mixed = 0;
+    // storage row of invariant i: >= 0 row of the FP64 store, < 0: -(row of the FP32 store) - 1
+    std::vector<int> imap() const {
+        std::vector<int> m(inv.size());
+        int nd = 0, nf = 0;
+        for (size_t i = 0; i < inv.size(); ++i) {
+            if (!mixed || inv[i].kind == INV_S) m[i] = nd++;
+            else m[i] = -(nf++) - 1;
+        }
+        return m;
+    }
     // coefficient records: each holds value, NLOC first and NLOC*NLOC second derivatives
     static const int NLOC = 8;
     static const int REC = 1 + NLOC + NLOC * NLOC;

@@ -413,7 +413,7 @@ __device__ inline double2 block_psi(const AuxArgs& a, const DBlock& B, long long
         for (int t = 0; t < DT.nre + DT.nim; ++t) {
             const DTerm& T = a.terms[DT.t0 + t];
             double c = (T.coef < 0) ? T.cval : a.table[a.off_rec + (long long)(D.rec + T.coef) * REC];
-            double x = (T.inv < 0) ? 1.0 : a.inv[(long long)T.inv * a.ld + e];
+            double x = (T.inv < 0) ? 1.0 : ev_inv(a.st, T.inv, e);
             if (t < DT.nre) dr += c * x; else di += c * x;
         }
         double q = 1.0 / (dr * dr + di * di);
@@ -435,7 +435,7 @@ __global__ void k_amplitudes(AuxArgs a, long long e0, long long n, double* __res
         double2 psi = block_psi(a, B, e);
         double K = a.table[a.off_rec + (long long)A.rec * REC];
         for (int p = 0; p < a.npol; ++p) {
-            double2 s = a.sl[((long long)A.u * a.npol + p) * a.ld + e];
+            double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
             double r = K * (psi.x * s.x - psi.y * s.y), im = K * (psi.x * s.y + psi.y * s.x);
             long long o = ((i * a.npol + p) * a.namp + am) * 2;
             out[o] = r;
@@ -456,7 +456,7 @@ __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __rest
         double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
         double yr = cr * psi.x - ci * psi.y, yi = cr * psi.y + ci * psi.x;
         for (int p = 0; p < a.npol; ++p) {
-            double2 s = a.sl[((long long)A.u * a.npol + p) * a.ld + e];
+            double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
             Sr[p] += yr * s.x - yi * s.y;
             Si[p] += yr * s.y + yi * s.x;
         }

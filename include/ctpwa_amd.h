@@ -38,6 +38,14 @@ typedef struct ctpwa_handle ctpwa_handle;
 #define CTPWA_DATA 1
 #define CTPWA_BKG 2
 
+/* storage precision of the per-event inputs (the arithmetic is FP64 in both):
+ * CTPWA_FP64   SL tensors and invariants in FP64 (the reference's double build)
+ * CTPWA_MIXED  SL tensors and invariants in FP32, squared masses in FP64; half the
+ *              HBM bytes per evaluation (the reference's default float build stores its
+ *              amplitudes in FP32 as well, include/ComplexType.h:27-32) */
+#define CTPWA_FP64 0
+#define CTPWA_MIXED 1
+
 /* Build the likelihood for `plan_text` (written by ctpwa_amd.model.Plan.to_text).
  * Events are host arrays [n][nfinal][4] of float64 (E, px, py, pz) in Data.order;
  * weights are host arrays [n] or NULL for unit weights.  n may be 0 for bkg.
@@ -46,7 +54,7 @@ int ctpwa_create(const char* plan_text,
                  const double* phsp, long long n_phsp, const double* w_phsp,
                  const double* data, long long n_data, const double* w_data,
                  const double* bkg, long long n_bkg, const double* w_bkg,
-                 int device, ctpwa_handle** out);
+                 int precision, int device, ctpwa_handle** out);
 
 void ctpwa_destroy(ctpwa_handle* h);
 
@@ -74,8 +82,9 @@ int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double*
  * `params_dev` (HIP events on `stream`; the prep kernel runs once before) */
 int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev, int iters, void* stream, double* avg_ms);
 
-/* device bytes a handle for this plan and these event counts holds */
-long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg);
+/* device bytes a handle for this plan, these event counts and this precision holds */
+long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg,
+                                int precision);
 
 /* number of events of a class, and algorithmic bytes the fused kernel must read per
  * evaluation (SL tensors + invariants + per-event weights) */
@@ -87,11 +96,11 @@ const char* ctpwa_kernel_source(const ctpwa_handle* h);
 
 /* source of the fused kernel for a plan, without a device: writes at most cap bytes
  * (NUL-terminated) into buf and returns the full size including the NUL, -1 on error */
-long long ctpwa_generate_source(const char* plan_text, char* buf, long long cap);
+long long ctpwa_generate_source(const char* plan_text, int precision, char* buf, long long cap);
 
 /* compile the fused kernel for a plan with hipRTC (no device needed); returns the
  * code-object size, or -1 with the compiler log in ctpwa_last_error() */
-long long ctpwa_compile_check(const char* plan_text);
+long long ctpwa_compile_check(const char* plan_text, int precision);
 
 const char* ctpwa_last_error(void);
 const char* ctpwa_version(void);

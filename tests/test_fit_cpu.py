@@ -1,0 +1,86 @@
+"""Fit driver (ctpwa_amd.fit) on a likelihood with a known minimum and Hessian (CPU).
+
+The stand-in analysis has NLL = 1/2 (p - mu)^T A (p - mu) in the grouped layout
+[Re p (n), Im p (n), theta (P)], with mu_0 = 1 and mu_n = 0 so that the fixed first
+coupling sits at the minimum, one resonance parameter the NLL does not depend on, and
+bounds on the others.  Checks the minimum, the errors sqrt(diag(inv(A_reduced))) read in
+the grouped layout, the fixed and unconstrained parameters, and the projected gradient
+at an active bound.
+"""
+import numpy as np
+import torch
+
+from ctpwa_amd.fit import Fitter
+
+
+class QuadraticAnalysis:
+    def __init__(self, n, P, mu, A, lo, hi):
+        self.n, self.P = n, P
+        self.mu = torch.tensor(mu)
+        self.A = torch.tensor(A)
+        self.lo, self.hi = lo, hi
+        self.device = torch.device("cpu")
+
+    def getNVector(self):
+        return self.n
+
+    def getNFreeTheta(self):
+        return self.P
+
+    def getFreeResParams(self):
+        init = np.clip(self.mu.numpy()[2 * self.n:], self.lo, self.hi)
+        return torch.tensor(np.stack([init, self.lo, self.hi]))
+
+    def getNLL(self, p):
+        d = p - self.mu
+        return 0.5 * d @ (self.A @ d)
+
+    def getHessian(self, p):
+        return self.A.clone()
+
+
+def _problem(seed, dead=True):
+    rng = np.random.default_rng(seed)
+    n, P = 3, 3
+    m = 2 * n + P
+    B = rng.standard_normal((m, m))
+    A = B @ B.T + m * np.eye(m)
+    if dead:                                     # the last theta does not enter the NLL
+        A[-1, :] = 0.0
+        A[:, -1] = 0.0
+    mu = rng.uniform(0.5, 1.5, m)
+    mu[0], mu[n] = 1.0, 0.0
+    lo = np.full(P, 0.0)
+    hi = np.full(P, 5.0)
+    return n, P, mu, A, lo, hi
+
+
+def test_minimum_and_grouped_errors():
+    n, P, mu, A, lo, hi = _problem(1)
+    ana = QuadraticAnalysis(n, P, mu, A, lo, hi)
+    start = mu + 0.3
+    start[-1] = mu[-1]
+    r = Fitter(ana).minimize(start, tolerance_grad=1e-12, tolerance_change=1e-14)
+    assert np.allclose(r.params[:-1], mu[:-1], atol=1e-7)
+    assert r.unconstrained == [2 * n + P - 1]
+    assert r.pos_def
+    keep = np.ones(2 * n + P, bool)
+    keep[[0, n, 2 * n + P - 1]] = False
+    cov = np.linalg.inv(A[np.ix_(keep, keep)])
+    exp = np.zeros(2 * n + P)
+    exp[keep] = np.sqrt(np.diag(cov))
+    assert np.allclose(r.errors, exp, rtol=1e-12)
+    assert r.errors[0] == 0.0 and r.errors[n] == 0.0
+
+
+def test_bounds_and_projected_gradient():
+    n, P, mu, A, lo, hi = _problem(2, dead=False)
+    hi = np.array([5.0, 5.0, 0.2])               # the minimum of the last theta lies above its bound
+    ana = QuadraticAnalysis(n, P, mu, A, lo, hi)
+    f = Fitter(ana)
+    r = f.minimize(mu.copy(), tolerance_grad=1e-12, tolerance_change=1e-14, hessian=False)
+    assert r.params[-1] == 0.2
+    # at the solution the projected gradient vanishes on the free coordinates
+    p = torch.tensor(r.params, requires_grad=True)
+    _, g = f.loss_and_grad(p)
+    assert float(g.abs().max()) < 1e-6

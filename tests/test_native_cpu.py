@@ -29,16 +29,19 @@ def test_library_exports_header_symbols():
     assert L.ctpwa_version().decode().startswith("ctpwa_amd")
 
 
-@pytest.mark.parametrize("name", ["jpsi_rho", "jpsi_3pi"])
-def test_generated_kernel_compiles_for_gfx950(name):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["jpsi_rho", "jpsi_3pi", "jpsi_kkpipi"])
+def test_generated_kernel_compiles_for_gfx950(name, precision):
     from ctpwa_amd import _native
     plan = plan_of(name)
-    src = _native.generate_source(plan.to_text())
+    src = _native.generate_source(plan.to_text(), precision)
     assert "ctpwa_eval" in src and "__launch_bounds__(256" in src
-    # every amplitude coefficient and every SL tensor appears
+    # every amplitude coefficient appears
     for a in range(plan.n_amp):
         assert "c%dr" % a in src
-    assert _native.compile_check(plan.to_text()) > 1000
+    # storage types: FP32 SL tensors only in the mixed form
+    assert ("const float2* __restrict__ sl" in src) == (precision == "mixed")
+    assert _native.compile_check(plan.to_text(), precision) > 1000
 
 
 def test_footprint_and_layout_formula():
@@ -49,6 +52,14 @@ def test_footprint_and_layout_formula():
     per_event = 4 * 2 * 16 + 7 * 8
     assert n >= 3_050_000 * per_event
     assert n < 3_050_000 * per_event * 1.2
+    # mixed: 8 B per complex SL value, 3 squared masses in FP64, 4 other invariants in FP32,
+    # plus construction-time staging of <= 2^20 events (momenta and one FP64 batch)
+    m = _native.footprint_bytes(plan.to_text(), 2_500_000, 500_000, 50_000, "mixed")
+    per_event_m = 4 * 2 * 8 + 3 * 8 + 4 * 4
+    stage = (1 << 20) * (3 * 32 + per_event)
+    assert 3_050_000 * per_event_m + stage <= m < 3_050_000 * per_event_m + stage + (1 << 20)
+    with pytest.raises(ValueError):
+        _native.footprint_bytes(plan.to_text(), 1, 1, 0, "half")
 
 
 def test_bad_plan_reports_error():
