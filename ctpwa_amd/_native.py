@@ -21,6 +21,7 @@ EXPORTS = [
     "ctpwa_hessian", "ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_time_eval_kernel",
     "ctpwa_footprint_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
+    "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
 ]
 
 _lib = None
@@ -59,6 +60,11 @@ def lib():
     L.ctpwa_generate_source.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, _ll]
     L.ctpwa_compile_check.restype = _ll
     L.ctpwa_compile_check.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.ctpwa_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.ctpwa_batch_destroy.argtypes = [_vp]
+    L.ctpwa_batch_destroy.restype = None
+    L.ctpwa_batch_nll_grad.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                       ctypes.c_int, _vp]
     L.ctpwa_last_error.restype = ctypes.c_char_p
     L.ctpwa_version.restype = ctypes.c_char_p
     _lib = L
@@ -201,3 +207,38 @@ class Handle:
 
     def kernel_source(self) -> str:
         return lib().ctpwa_kernel_source(self.h).decode()
+
+
+class Batch:
+    """Evaluates many handles of one model in one set of launches (ctpwa_batch_nll_grad)."""
+
+    def __init__(self, max_jobs: int, device: int = 0):
+        b = ctypes.c_void_p()
+        check(lib().ctpwa_batch_create(device, max_jobs, ctypes.byref(b)), "ctpwa_batch_create")
+        self.b = b
+        self.max_jobs = max_jobs
+        self._hs = (_vp * max_jobs)()
+        self._ps = (_vp * max_jobs)()
+        self._os = (_vp * max_jobs)()
+
+    def nll_grad(self, handles, params_ptrs, out_ptrs, stream: int):
+        n = len(handles)
+        if n > self.max_jobs:
+            raise ValueError("batch holds at most %d likelihoods" % self.max_jobs)
+        for j in range(n):
+            self._hs[j] = handles[j].h.value
+            self._ps[j] = params_ptrs[j]
+            self._os[j] = out_ptrs[j]
+        check(lib().ctpwa_batch_nll_grad(self.b, self._hs, self._ps, self._os, n, ctypes.c_void_p(stream)),
+              "ctpwa_batch_nll_grad")
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib().ctpwa_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

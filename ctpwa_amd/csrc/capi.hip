@@ -30,6 +30,9 @@ extern "C" hipError_t ctpwa_launch_to_mixed(const double2*, const double*, long 
                                              double*, float*, long long, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_prep_batch(const double* const*, double* const*, const ctpwa::PrepArgs*, int, int,
+                                              hipStream_t);
+extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
 
@@ -135,7 +138,7 @@ static void node_cg_table(const PNode& n, int S, int L, std::vector<double>& out
 static std::mutex g_mod_mu;
 static std::map<std::pair<int, std::string>, hipModule_t> g_mods;
 
-static hipFunction_t compile_eval(int device, const std::string& src) {
+static hipModule_t compile_eval(int device, const std::string& src) {
     std::lock_guard<std::mutex> lk(g_mod_mu);
     auto key = std::make_pair(device, src);
     auto it = g_mods.find(key);
@@ -164,9 +167,7 @@ static hipFunction_t compile_eval(int device, const std::string& src) {
         HIPCHECK(hipModuleLoadData(&mod, code.data()));
         g_mods[key] = mod;
     }
-    hipFunction_t fn;
-    HIPCHECK(hipModuleGetFunction(&fn, mod, "ctpwa_eval"));
-    return fn;
+    return mod;
 }
 
 // ---------------------------------------------------------------------------
@@ -192,7 +193,9 @@ struct ctpwa_handle {
     PrepArgs prep{};
     FinArgs fin{};
     AuxArgs aux{};
-    hipFunction_t fn_eval = nullptr;
+    hipModule_t mod = nullptr;
+    hipFunction_t fn_eval = nullptr;       // one likelihood
+    hipFunction_t fn_group = nullptr;      // many likelihoods of this model in one launch
     std::string src;
     // Hessian workspace (allocated on the first ctpwa_hessian call)
     bool hess_ready = false;
@@ -640,7 +643,9 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
                          d_amps, d_blocks, d_dens, d_dterms, d_terms, P.namp, P.npol};
 
         h->src = generate_eval_kernel(M);
-        h->fn_eval = compile_eval(device, h->src);
+        h->mod = compile_eval(device, h->src);
+        HIPCHECK(hipModuleGetFunction(&h->fn_eval, h->mod, "ctpwa_eval"));
+        HIPCHECK(hipModuleGetFunction(&h->fn_group, h->mod, "ctpwa_eval_group"));
         h->used_inv.assign(M.inv.size(), 0);
         for (const DenSpec& D : M.dens) {
             for (const Term& t : D.re) if (t.inv >= 0) h->used_inv[t.inv] = 1;
@@ -854,6 +859,117 @@ extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev,
     if (e != hipSuccess) return fail(std::string("ctpwa_time_eval_kernel: ") + hipGetErrorString(e));
     *avg_ms = ms / iters;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// many likelihoods of one model in one set of launches (continuous batching)
+// ---------------------------------------------------------------------------
+// mirrors `struct EvalJob` of the generated source (codegen.cpp)
+struct EvalJob {
+    const double* tab; const void* sl; const double* inv; const float* invf; const double* wts;
+    double* part; const int* wg_cls; const long long* wg_rng; long long ld; double w1, w2;
+    int nwg, pw1, pw2, wg_begin;
+};
+
+struct ctpwa_batch {
+    int device = 0;
+    int cap = 0;
+    size_t bytes = 0;
+    char* host = nullptr;          // pinned staging of the descriptors
+    char* dev = nullptr;           // device copy: [EvalJob cap][FinArgs cap][params ptr cap][table ptr cap]
+    hipEvent_t copied = nullptr;   // the staging buffer may be rewritten once this completes
+    bool pending = false;
+    size_t o_jobs = 0, o_fins = 0, o_params = 0, o_tables = 0;
+};
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+extern "C" int ctpwa_batch_create(int device, int max_jobs, ctpwa_batch** out) {
+    if (!out) return fail("ctpwa_batch_create: out is NULL");
+    *out = nullptr;
+    if (max_jobs <= 0 || max_jobs > 65535) return fail("ctpwa_batch_create: max_jobs must be in [1, 65535]");
+    ctpwa_batch* b = new ctpwa_batch();
+    DeviceGuard g(device);
+    try {
+        b->device = device;
+        b->cap = max_jobs;
+        b->o_jobs = 0;
+        b->o_fins = align256(sizeof(EvalJob) * max_jobs);
+        b->o_params = b->o_fins + align256(sizeof(FinArgs) * max_jobs);
+        b->o_tables = b->o_params + align256(sizeof(void*) * max_jobs);
+        b->bytes = b->o_tables + align256(sizeof(void*) * max_jobs);
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipHostMalloc((void**)&b->host, b->bytes, hipHostMallocDefault));
+        HIPCHECK(hipMalloc((void**)&b->dev, b->bytes));
+        HIPCHECK(hipEventCreateWithFlags(&b->copied, hipEventDisableTiming));
+        *out = b;
+        return 0;
+    } catch (const std::exception& e) {
+        if (b->host) (void)hipHostFree(b->host);
+        if (b->dev) (void)hipFree(b->dev);
+        delete b;
+        return fail(std::string("ctpwa_batch_create: ") + e.what());
+    }
+}
+
+extern "C" void ctpwa_batch_destroy(ctpwa_batch* b) {
+    if (!b) return;
+    DeviceGuard g(b->device);
+    if (b->pending) (void)hipEventSynchronize(b->copied);
+    (void)hipEventDestroy(b->copied);
+    (void)hipHostFree(b->host);
+    (void)hipFree(b->dev);
+    delete b;
+}
+
+extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, const double* const* params_dev,
+                                    double* const* out_dev, int n, void* stream) {
+    if (!b || !hs || !params_dev || !out_dev) return fail("ctpwa_batch_nll_grad: NULL argument");
+    if (n <= 0) return 0;
+    if (n > b->cap) return fail("ctpwa_batch_nll_grad: more likelihoods than the batch was created for");
+    DeviceGuard g(b->device);
+    hipStream_t s = (hipStream_t)stream;
+    try {
+        const ctpwa_handle* h0 = hs[0];
+        int max_nacc = 0;
+        for (int j = 0; j < n; ++j) {
+            const ctpwa_handle* h = hs[j];
+            if (!h || !params_dev[j] || !out_dev[j]) throw std::runtime_error("NULL handle, parameter or output pointer");
+            if (h->device != b->device) throw std::runtime_error("a handle lives on another device than the batch");
+            if (h->mod != h0->mod) throw std::runtime_error("the handles of one batch must share one model and precision");
+            max_nacc = std::max(max_nacc, h->nacc);
+        }
+        // the staging buffer is reused: wait for the previous upload from it
+        if (b->pending) HIPCHECK(hipEventSynchronize(b->copied));
+        EvalJob* jobs = (EvalJob*)(b->host + b->o_jobs);
+        FinArgs* fins = (FinArgs*)(b->host + b->o_fins);
+        const double** pp = (const double**)(b->host + b->o_params);
+        double** tp = (double**)(b->host + b->o_tables);
+        int wg = 0;
+        for (int j = 0; j < n; ++j) {
+            ctpwa_handle* h = hs[j];
+            jobs[j] = EvalJob{h->table, h->slp, h->inv, h->invf, h->wts, h->part, h->wg_cls, h->wg_rng, h->ntot,
+                              h->wc[1], h->wc[2], h->nwg, h->pw[1], h->pw[2], wg};
+            wg += h->nwg;
+            fins[j] = h->fin;
+            fins[j].params = params_dev[j];
+            fins[j].out = out_dev[j];
+            pp[j] = params_dev[j];
+            tp[j] = h->table;
+        }
+        HIPCHECK(hipMemcpyAsync(b->dev, b->host, b->bytes, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipEventRecord(b->copied, s));
+        b->pending = true;
+        HIPCHECK(ctpwa_launch_prep_batch((const double* const*)(b->dev + b->o_params), (double* const*)(b->dev + b->o_tables),
+                                         &h0->prep, h0->M.maxloc, n, s));
+        const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
+        void* args[] = {&d_jobs, &n};
+        HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+        HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, max_nacc, s));
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(std::string("ctpwa_batch_nll_grad: ") + e.what());
+    }
 }
 
 // per-event bytes of the SL tensors and the invariants a model stores at a precision

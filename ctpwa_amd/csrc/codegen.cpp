@@ -55,12 +55,14 @@ std::string generate_eval_kernel(const Model_& M) {
          "  double e = __builtin_fma(-x, r, 1.0); r = __builtin_fma(r, e, r);\n"
          "  e = __builtin_fma(-x, r, 1.0); return __builtin_fma(r, e, r);\n}\n";
     o << "#define OFFC " << M.off_c() << "LL\n#define OFFR " << M.off_rec() << "LL\n#define REC " << Model_::REC << "\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(256" << (M.min_waves > 0 ? ", " + std::to_string(M.min_waves) : std::string())
-      << ") ctpwa_eval(\n"
+    // one workgroup's share of the events: the body is shared by the single-likelihood
+    // kernel and the grouped kernel that evaluates many likelihoods of this model in one
+    // launch, so a likelihood's sums do not depend on how it was launched
+    o << "__device__ __forceinline__ void eval_body(\n"
          "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
-         "    double w1, double w2, int pw1, int pw2)\n{\n";
+         "    double w1, double w2, int pw1, int pw2, const int wg, double (*__restrict__ red)[" << NACC << "])\n{\n";
     // storage types: FP64, or FP32 SL tensors / FP32 invariants except s (mixed)
     const char* SLT = M.mixed ? "float2" : "double2";
     o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n";
@@ -74,8 +76,7 @@ std::string generate_eval_kernel(const Model_& M) {
     auto sl_conv = [&](const std::string& x) {
         return M.mixed ? "make_double2((double)" + x + ".x, (double)" + x + ".y)" : x;
     };
-    o << "  __shared__ double red[4][" << NACC << "];\n";
-    o << "  const int wg = blockIdx.x;\n  const int cls = wg_cls[wg];\n"
+    o << "  const int cls = wg_cls[wg];\n"
          "  const long long e0 = wg_rng[2 * wg], e1 = wg_rng[2 * wg + 1];\n"
          "  const double wc = (cls == 1) ? w1 : w2;\n  const int pw = (cls == 1) ? pw1 : ((cls == 2) ? pw2 : 0);\n"
          "  const double LOG_FLOOR = log(1e-10);\n";
@@ -235,6 +236,27 @@ std::string generate_eval_kernel(const Model_& M) {
     o << "  for (int k = threadIdx.x; k < " << NACC << "; k += 256)\n"
          "    part[(long long)k * nwg + wg] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);\n";
     o << "}\n";
+    std::string lb = "__launch_bounds__(256" + (M.min_waves > 0 ? ", " + std::to_string(M.min_waves) : std::string()) + ")";
+    o << "extern \"C\" __global__ void " << lb << " ctpwa_eval(\n"
+         "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
+         "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
+         "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
+         "    double w1, double w2, int pw1, int pw2)\n{\n"
+         "  __shared__ double red[4][" << NACC << "];\n"
+         "  eval_body(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red);\n}\n";
+    // grouped launch: jobs[j] owns workgroups [wg_begin_j, wg_begin_{j+1}); the layout of
+    // this struct is mirrored by EvalJob in capi.hip
+    o << "struct EvalJob {\n"
+         "  const double* tab; const void* sl; const double* inv; const float* invf; const double* wts;\n"
+         "  double* part; const int* wg_cls; const long long* wg_rng; long long ld; double w1, w2;\n"
+         "  int nwg, pw1, pw2, wg_begin;\n};\n";
+    o << "extern \"C\" __global__ void " << lb << " ctpwa_eval_group(const EvalJob* __restrict__ jobs, int njobs)\n{\n"
+         "  __shared__ double red[4][" << NACC << "];\n"
+         "  int lo = 0, hi = njobs - 1;\n"
+         "  while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (jobs[mid].wg_begin <= (int)blockIdx.x) lo = mid; else hi = mid - 1; }\n"
+         "  const EvalJob J = jobs[lo];\n"
+         "  eval_body(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
+         "            (int)blockIdx.x - J.wg_begin, red);\n}\n";
     return o.str();
 }
 

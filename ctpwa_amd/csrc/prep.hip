@@ -199,7 +199,7 @@ __device__ inline void store(double* t, const DJ<NV, H>& x) {
 }
 
 template <int NV, bool H>
-__global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
+__device__ __forceinline__ void prep_body(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
     using J = DJ<NV, H>;
     extern __shared__ uint4 smeta[];
     if (a.meta_bytes > 0) {
@@ -315,6 +315,19 @@ __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, 
     table[a.off_c + 2 * item + 1] = pi_ * K.v;
 }
 
+template <int NV, bool H>
+__global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
+    prep_body<NV, H>(params, table, a);
+}
+
+// many likelihoods of one model: blockIdx.y selects the likelihood (its parameters and
+// coefficient table); the model tables in `a` are the same for all of them
+template <int NV, bool H>
+__global__ void __launch_bounds__(64) k_prep_batch(const double* const* __restrict__ params, double* const* __restrict__ tables,
+                                                   PrepArgs a) {
+    prep_body<NV, H>(params[blockIdx.y], tables[blockIdx.y], a);
+}
+
 // ---------------------------------------------------------------------------
 // final assembly
 // ---------------------------------------------------------------------------
@@ -339,10 +352,28 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(FinArgs f) {
     if (threadIdx.x == 0) f.totals[pair] = sh[0];
 }
 
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __restrict__ fins) {
+    const FinArgs& f = fins[blockIdx.y];
+    if ((int)blockIdx.x >= 3 * f.nacc) return;      // the whole workgroup leaves together
+    __shared__ double sh[RED_THREADS];
+    int pair = blockIdx.x;
+    int c = pair / f.nacc, col = pair - c * f.nacc;
+    const double* src = f.part + (long long)col * f.nwg;
+    double s = 0.0;
+    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) f.totals[pair] = sh[0];
+}
+
 // stage 2: NLL and gradient from the class totals.  The class-combined accumulators
 // go to LDS first; the gradients are then CSR contractions over host-built entry lists
 // (independent loads, one wave per theta with a fixed-order butterfly sum).
-__global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) {
+__device__ __forceinline__ void assemble_body(const FinArgs& f) {
     extern __shared__ double C[];            // [nacc] = -data + bkg + cp * phsp
     int tid = threadIdx.x;
     const double* tot = f.totals;            // tot[c * nacc + col]
@@ -398,6 +429,12 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) {
             out[1 + 2 * f.nfree + j] = reset ? 0.0 : g;
         }
     }
+}
+
+__global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) { assemble_body(f); }
+
+__global__ void __launch_bounds__(ASM_THREADS) k_assemble_batch(const FinArgs* __restrict__ fins) {
+    assemble_body(fins[blockIdx.x]);
 }
 
 // ---------------------------------------------------------------------------
@@ -485,6 +522,39 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep(co
     if (nv <= 2) return hess ? launch_prep_t<2, true>(params, table, a, stream) : launch_prep_t<2, false>(params, table, a, stream);
     if (nv <= 4) return hess ? launch_prep_t<4, true>(params, table, a, stream) : launch_prep_t<4, false>(params, table, a, stream);
     return hess ? launch_prep_t<8, true>(params, table, a, stream) : launch_prep_t<8, false>(params, table, a, stream);
+}
+
+template <int NV, bool H>
+static hipError_t launch_prep_batch_t(const double* const* params, double* const* tables, const ctpwa::PrepArgs* a,
+                                      int njobs, hipStream_t stream) {
+    int items = a->nden + a->namp;
+    int bs = 64;
+    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3((items + bs - 1) / bs, njobs), dim3(bs), (size_t)a->meta_bytes,
+                       stream, params, tables, *a);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep_batch(const double* const* params,
+                                                                                    double* const* tables,
+                                                                                    const ctpwa::PrepArgs* a, int nv,
+                                                                                    int njobs, hipStream_t stream) {
+    if (njobs <= 0) return hipSuccess;
+    if (nv <= 1) return launch_prep_batch_t<1, false>(params, tables, a, njobs, stream);
+    if (nv <= 2) return launch_prep_batch_t<2, false>(params, tables, a, njobs, stream);
+    if (nv <= 4) return launch_prep_batch_t<4, false>(params, tables, a, njobs, stream);
+    return launch_prep_batch_t<8, false>(params, tables, a, njobs, stream);
+}
+
+// fins: device array [njobs]; max_nacc: the largest accumulator count among them
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs* fins, int njobs,
+                                                                                        int max_nacc, hipStream_t stream) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctpwa::k_reduce_batch, dim3(3 * max_nacc, njobs), dim3(ctpwa::RED_THREADS), 0, stream, fins);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(ctpwa::k_assemble_batch, dim3(njobs), dim3(ctpwa::ASM_THREADS), (size_t)max_nacc * sizeof(double),
+                       stream, fins);
+    return hipGetLastError();
 }
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs* f, hipStream_t stream) {

@@ -72,6 +72,20 @@ int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double* out_dev, v
 /* Hessian in the same layout: device [(2 n_free + n_theta)^2], row-major. */
 int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream);
 
+/* Many likelihoods of one model (same plan and precision; e.g. the toy fits of an
+ * ensemble) evaluated together: one batched prep launch, one grouped per-event launch
+ * whose workgroups are shared out among the likelihoods, one batched reduction.  Each
+ * likelihood's result is bit-identical to ctpwa_nll_grad on its own (same partition of
+ * its events, same summation order).  Replaces the per-fit NLLFunction::forward/backward
+ * calls (src/Analysis.cu:625-1261) when a service runs many fits at once; the reference
+ * has no batched form.  A batch object owns pinned and device descriptor space for up to
+ * `max_jobs` likelihoods; calls on one batch object must not overlap. */
+typedef struct ctpwa_batch ctpwa_batch;
+int ctpwa_batch_create(int device, int max_jobs, ctpwa_batch** out);
+void ctpwa_batch_destroy(ctpwa_batch* b);
+int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* handles, const double* const* params_dev,
+                         double* const* out_dev, int n, void* stream);
+
 /* per-event intensity of class `cls`: device [n_cls] */
 int ctpwa_intensity(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
