@@ -140,6 +140,29 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
     return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof}
 
 
+def run_serving(args, samples, plan, spec, W, precisions):
+    """Toy-MC ensemble through the fit service: data counts log-uniform in [5k, 500k],
+    phase space 5x data, Poisson arrivals; every fit is a bounded L-BFGS issuing
+    NLL+gradient requests that the service batches across the active fits."""
+    from ctpwa_amd.serve import FitService, toy_ensemble
+    x_nom = W.nominal_params(plan, spec["seed"])
+    res = {"jobs": args.serving_jobs, "arrival_rate_per_s": args.serving_rate, "slots": args.serving_slots,
+           "seed": 3, "data_per_fit": "log-uniform [5000, 500000]", "phsp_per_fit": "5 x data",
+           "events": "windows of the benchmark's data and phase-space samples"}
+    for prec in precisions:
+        jobs = toy_ensemble(plan, samples["phsp"], samples["data"], args.serving_jobs, seed=3,
+                            rate=args.serving_rate, x_nominal=x_nom)
+        svc = FitService(plan.to_text(), precision=prec, max_slots=args.serving_slots)
+        st = svc.run(jobs)
+        svc.close()
+        sm = st.summary()
+        res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sm.items()}
+        log("[bench] serving %s: %d fits in %.1f s, %.0f fits/h, p50 %.2f ms, p99 %.2f ms, mean batch %.1f"
+            % (prec, sm["fits"], sm["wall_s"], sm["fits_per_hour"], sm["latency_p50_ms"], sm["latency_p99_ms"],
+               sm["mean_batch"]))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +177,10 @@ def main():
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
                     help="both: time the mixed form (the reported value) and the FP64 form beside it")
+    ap.add_argument("--serving-jobs", type=int, default=256,
+                    help="toy fits run through the fit service after the timing (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=100.0, help="Poisson arrival rate of the toy fits per second")
+    ap.add_argument("--serving-slots", type=int, default=64, help="fits evaluated together at most")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
                          "known size that calibrates the HBM byte counters of a --pmc profile)")
@@ -204,6 +231,10 @@ def main():
                    "delta_nll_mixed_minus_fp64": r["nll"] - other["nll"],
                    "delta_nll_points": "NLL at the same last timed parameter point in both forms"}
 
+    serving = None
+    if args.serving_jobs > 0:
+        serving = run_serving(args, samples, plan, spec, W, precisions)
+
     if args.copy_probe_mb > 0:
         nb = args.copy_probe_mb << 20
         src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
@@ -239,6 +270,8 @@ def main():
     }
     if compare is not None:
         out["fp64"] = compare
+    if serving is not None:
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
     if tmp is not None:
         tmp.cleanup()

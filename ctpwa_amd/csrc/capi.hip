@@ -216,7 +216,8 @@ struct ctpwa_handle {
     T* alloc(size_t count) {
         void* p = nullptr;
         if (count == 0) count = 1;
-        HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+        // 256 bytes of slack after every buffer: whole 16-byte chunks never end past it
+        HIPCHECK(hipMalloc(&p, count * sizeof(T) + 256));
         allocs.push_back(p);
         return (T*)p;
     }
@@ -236,10 +237,15 @@ struct ctpwa_handle {
 };
 
 // code-generation options for the fused kernel (tuning; defaults are the measured best)
+// code-generation options; the environment overrides them for measurements
 static void apply_codegen_env(Model_& M) {
     if (const char* e = std::getenv("CTPWA_EVAL_WAVES")) M.min_waves = std::atoi(e);
-    if (const char* e = std::getenv("CTPWA_EVAL_RELOAD")) M.reload_coef = std::atoi(e) != 0;
-    if (const char* e = std::getenv("CTPWA_EVAL_PREFETCH")) M.prefetch = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CTPWA_EVAL_LDS")) M.lds_pipe = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CTPWA_EVAL_WGS")) {
+        int w = std::atoi(e);
+        if (w != 64 && w != 128 && w != 256) throw std::runtime_error("CTPWA_EVAL_WGS must be 64, 128 or 256");
+        M.wgs = w;
+    }
 }
 
 static void build_tables(ctpwa_handle* h, std::vector<DTopo>& topos, std::vector<DNode>& nodes,
@@ -320,14 +326,15 @@ static void check_q0_masses(const Model_& M) {
 }
 
 static void setup_partition(ctpwa_handle* h) {
-    long long target = std::min<long long>(1024, std::max<long long>(1, (h->ntot + 1023) / 1024));
+    const long long nev = h->n[0] + h->n[1] + h->n[2];
+    long long target = std::min<long long>(1024, std::max<long long>(1, (nev + 1023) / 1024));
     std::vector<int> cls;
     std::vector<long long> rng;
     for (int c = 0; c < 3; ++c) {
         h->g0[c] = (int)cls.size();
         long long n = h->n[c];
         if (n > 0) {
-            long long want = std::max<long long>(1, (long long)std::llround((double)target * n / h->ntot));
+            long long want = std::max<long long>(1, (long long)std::llround((double)target * n / nev));
             long long chunk = (n + want - 1) / want;
             chunk = ((chunk + 255) / 256) * 256;
             long long nw = (n + chunk - 1) / chunk;
@@ -383,10 +390,14 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         long long ns[3] = {n_phsp, n_data, n_bkg};
         for (int c = 0; c < 3; ++c) {
             h->n[c] = ns[c];
-            h->off[c] = h->ntot;
-            h->ntot += ns[c];
+            // class offsets and the leading dimension are multiples of 64 events, so
+            // every row of a tile starts 16-byte aligned (the pipelined kernel copies
+            // 16-byte chunks)
+            h->off[c] = (h->ntot + 63) / 64 * 64;
+            h->ntot = h->off[c] + ns[c];
             if (ns[c] > 0 && !moms[c]) throw std::runtime_error("event array is NULL");
         }
+        h->ntot = (h->ntot + 63) / 64 * 64;
         // weights: constant per class unless they vary
         std::vector<double> wall;
         bool any_var = false;
@@ -687,7 +698,7 @@ extern "C" int ctpwa_param_info(const ctpwa_handle* h, double* init, double* low
 static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s) {
     void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->wts, &h->part, &h->wg_cls, &h->wg_rng, &h->ntot,
                     &h->nwg, &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2]};
-    return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+    return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, h->M.wgs, 1, 1, 0, s, args, nullptr);
 }
 
 struct DeviceGuard {
@@ -964,7 +975,7 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
                                          &h0->prep, h0->M.maxloc, n, s));
         const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
         void* args[] = {&d_jobs, &n};
-        HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+        HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
         HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, max_nacc, s));
         return 0;
     } catch (const std::exception& e) {
@@ -1008,7 +1019,7 @@ extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long lo
     int used = 0;
     for (int u : h->used_inv) used += u;
     if (bytes_per_eval) {
-        long long b = h->ntot * event_bytes(h->M, &h->used_inv);
+        long long b = (h->n[0] + h->n[1] + h->n[2]) * event_bytes(h->M, &h->used_inv);
         for (int c = 1; c < 3; ++c) if (h->pw[c]) b += h->n[c] * 8;
         *bytes_per_eval = b;
     }

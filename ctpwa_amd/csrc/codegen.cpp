@@ -18,8 +18,9 @@
 //   accT_t += x_t Re R_d | y_t Im R_d        one per theta-dependent denominator term;
 //            dL/dtheta_j = -2 sum_t (dcoef_t/dtheta_j) accT_t is formed after the sum
 //   accV   += I (phase space) | w_e ln I (data, background)
-// The next event's inputs are loaded while the current one is computed, then a
-// deterministic wave/LDS reduction writes one partial row per workgroup.
+// Inputs are read straight from HBM, or (pipelined form) copied tile by tile into LDS
+// by the DMA path while the previous tile is computed; a deterministic wave/LDS
+// reduction then writes one partial row per workgroup.
 #include <sstream>
 #include <string>
 
@@ -48,6 +49,7 @@ std::string generate_eval_kernel(const Model_& M) {
         for (const Term& t : Dn.im) if (t.inv >= 0) used[t.inv] = 1;
     }
     std::ostringstream o;
+    long long tile_bytes = 16;       // LDS tile buffer of the pipelined form (per buffer)
     o << "// generated: fused likelihood kernel\n";
     // reciprocal: hardware estimate refined by two Newton steps (~1 ulp)
     o << "__device__ __forceinline__ double frcp(double x) {\n"
@@ -62,7 +64,8 @@ std::string generate_eval_kernel(const Model_& M) {
          "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
-         "    double w1, double w2, int pw1, int pw2, const int wg, double (*__restrict__ red)[" << NACC << "])\n{\n";
+         "    double w1, double w2, int pw1, int pw2, const int wg, double (*__restrict__ red)[" << NACC << "],\n"
+         "    unsigned char* __restrict__ lbuf)\n{\n";
     // storage types: FP64, or FP32 SL tensors / FP32 invariants except s (mixed)
     const char* SLT = M.mixed ? "float2" : "double2";
     o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n";
@@ -89,41 +92,73 @@ std::string generate_eval_kernel(const Model_& M) {
             for (int c = 0; c < M.dens[d].ncoef; ++c)
                 o << ind << "const double k" << d << "_" << c << " = tb[OFFR + " << (long long)(M.dens[d].coef_base + c) << "LL * REC];\n";
     };
-    if (!M.reload_coef) emit_coefs("  ");
+    emit_coefs("  ");
     o << "  double av = 0.0;\n";
     for (int a = 0; a < NA; ++a) o << "  double ax" << a << "r = 0.0, ax" << a << "i = 0.0;\n";
     for (int t = 0; t < NTH; ++t) o << "  double at" << t << " = 0.0;\n";
-    o << "  long long e = e0 + threadIdx.x;\n";
-    if (M.prefetch) {
-        // prefetch registers for the first event of this thread
-        o << "  long long ep = (e < e1) ? e : e0;\n";
+    const int WGS = M.wgs, NW = M.wgs / 64;
+    if (M.lds_pipe) {
+        // Tiles of WGS events: every per-event input row of the next tile is copied
+        // HBM -> LDS by the DMA path (global_load_lds_dwordx4, no registers held) while
+        // the current tile is computed from LDS.  Row r of a tile occupies
+        // [roff_r, roff_r + WGS * es_r) of the tile buffer; the event layout (class
+        // offsets and leading dimension multiples of 64 events) keeps every 16-byte chunk
+        // aligned and inside its row.
+        struct Row { std::string src; int es; long long off; std::string rd; };
+        std::vector<Row> rows;
+        long long tb = 0;
+        const int esl = M.mixed ? 8 : 16;
         for (int u = 0; u < NU; ++u)
-            for (int p = 0; p < NP; ++p)
-                o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + ep];\n";
-        for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "  double nx" << i << " = " << inv_load(i, "ep") << ";\n";
-    }
-    o << "  for (; e < e1; e += 256) {\n";
-    if (M.reload_coef) {
-        // keeps the coefficient loads inside the loop as scalar-cache loads
-        o << "    asm volatile(\"\" : \"+s\"(tb));\n";
-        emit_coefs("    ");
-    }
-    if (M.prefetch) {
-        for (int u = 0; u < NU; ++u)
-            for (int p = 0; p < NP; ++p)
-                o << "    const double2 s" << u << "_" << p << " = " << sl_conv("n" + std::to_string(u) + "_" + std::to_string(p)) << ";\n";
-        for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "    const double x" << i << " = nx" << i << ";\n";
-        // issue the next event's loads (index clamped to the chunk: always a valid address)
-        o << "    { const long long en = (e + 256 < e1) ? e + 256 : e;\n";
-        for (int u = 0; u < NU; ++u)
-            for (int p = 0; p < NP; ++p)
-                o << "      n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + en];\n";
-        for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "      nx" << i << " = " << inv_load(i, "en") << ";\n";
-        o << "    }\n";
+            for (int p = 0; p < NP; ++p) {
+                std::string nm = std::to_string(u) + "_" + std::to_string(p);
+                std::ostringstream src;
+                src << "(const unsigned char*)sl + " << (long long)u * NP + p << "LL * ld * " << esl;
+                std::ostringstream rd;
+                rd << "    const " << SLT << " g" << nm << " = ((const " << SLT << "*)(B + " << tb << "))[threadIdx.x];\n"
+                   << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
+                rows.push_back(Row{src.str(), esl, tb, rd.str()});
+                tb += (long long)WGS * esl;
+            }
+        for (size_t i = 0; i < M.inv.size(); ++i) {
+            if (!used[i]) continue;
+            std::ostringstream src, rd;
+            if (imap[i] >= 0) {
+                src << "(const unsigned char*)inv + " << imap[i] << "LL * ld * 8";
+                rd << "    const double x" << i << " = ((const double*)(B + " << tb << "))[threadIdx.x];\n";
+                rows.push_back(Row{src.str(), 8, tb, rd.str()});
+                tb += (long long)WGS * 8;
+            } else {
+                src << "(const unsigned char*)invf + " << (-imap[i] - 1) << "LL * ld * 4";
+                rd << "    const double x" << i << " = (double)((const float*)(B + " << tb << "))[threadIdx.x];\n";
+                rows.push_back(Row{src.str(), 4, tb, rd.str()});
+                tb += (long long)WGS * 4;
+            }
+        }
+        o << "  typedef __attribute__((address_space(3))) void* lds_ptr;\n";
+        o << "  const int ntile = (int)((e1 - e0 + " << WGS - 1 << ") / " << WGS << ");\n";
+        // issue the copies of the tile starting at event ts into buffer dst
+        o << "  auto issue = [&](unsigned char* dst, const long long ts) {\n"
+             "    const int t = threadIdx.x;\n";
+        for (const Row& r : rows) {
+            int nl = WGS * r.es / 16;
+            o << "    if (t < " << nl << " && ts + (long long)(t * " << 16 / r.es << ") < e1)\n"
+              << "      __builtin_amdgcn_global_load_lds((const void*)(" << r.src << " + ts * " << r.es << " + t * 16),\n"
+              << "                                     (lds_ptr)(dst + " << r.off << " + (t & ~63) * 16), 16, 0, 0);\n";
+        }
+        o << "  };\n";
+        o << "  issue(lbuf, e0);\n";
+        o << "  for (int it = 0; it < ntile; ++it) {\n"
+             "    const long long ts = e0 + (long long)it * " << WGS << ";\n"
+             "    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this thread's copies landed\n"
+             "    __syncthreads();                      // every thread's copies landed; previous tile consumed\n"
+             "    if (it + 1 < ntile) issue(lbuf + ((it + 1) & 1) * " << tb << ", ts + " << WGS << ");\n"
+             "    const unsigned char* B = lbuf + (it & 1) * " << tb << ";\n"
+             "    const long long e = ts + threadIdx.x;\n"
+             "    if (e < e1) {\n";
+        for (const Row& r : rows) o << r.rd;
+        tile_bytes = tb;
     } else {
+        o << "  for (long long e = e0 + threadIdx.x; e < e1; e += " << WGS << ") {\n";
         for (int u = 0; u < NU; ++u)
             for (int p = 0; p < NP; ++p) {
                 std::string nm = std::to_string(u) + "_" + std::to_string(p);
@@ -222,7 +257,7 @@ std::string generate_eval_kernel(const Model_& M) {
         }
         o << "    }\n";
     }
-    o << "  }\n";
+    o << (M.lds_pipe ? "    }\n  }\n" : "  }\n");
     // reduction
     o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
     auto red = [&](const std::string& v, int k) {
@@ -233,17 +268,24 @@ std::string generate_eval_kernel(const Model_& M) {
     for (int a = 0; a < NA; ++a) { red("ax" + std::to_string(a) + "r", 1 + 2 * a); red("ax" + std::to_string(a) + "i", 2 + 2 * a); }
     for (int t = 0; t < NTH; ++t) red("at" + std::to_string(t), 1 + 2 * NA + t);
     o << "  __syncthreads();\n";
-    o << "  for (int k = threadIdx.x; k < " << NACC << "; k += 256)\n"
-         "    part[(long long)k * nwg + wg] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);\n";
+    // fixed-order combination of the wave partials
+    std::string comb;
+    if (NW == 1) comb = "red[0][k]";
+    else if (NW == 2) comb = "red[0][k] + red[1][k]";
+    else comb = "(red[0][k] + red[1][k]) + (red[2][k] + red[3][k])";
+    o << "  for (int k = threadIdx.x; k < " << NACC << "; k += " << WGS << ")\n"
+         "    part[(long long)k * nwg + wg] = " << comb << ";\n";
     o << "}\n";
-    std::string lb = "__launch_bounds__(256" + (M.min_waves > 0 ? ", " + std::to_string(M.min_waves) : std::string()) + ")";
+    std::string lb = "__launch_bounds__(" + std::to_string(WGS) + (M.min_waves > 0 ? ", " + std::to_string(M.min_waves) : std::string()) + ")";
+    std::ostringstream decl;
+    decl << "  __shared__ double red[" << NW << "][" << NACC << "];\n"
+         << "  __shared__ __attribute__((aligned(16))) unsigned char lbuf[" << (M.lds_pipe ? 2 * tile_bytes : 16) << "];\n";
     o << "extern \"C\" __global__ void " << lb << " ctpwa_eval(\n"
          "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
-         "    double w1, double w2, int pw1, int pw2)\n{\n"
-         "  __shared__ double red[4][" << NACC << "];\n"
-         "  eval_body(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red);\n}\n";
+         "    double w1, double w2, int pw1, int pw2)\n{\n" << decl.str() <<
+         "  eval_body(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red, lbuf);\n}\n";
     // grouped launch: jobs[j] owns workgroups [wg_begin_j, wg_begin_{j+1}); the layout of
     // this struct is mirrored by EvalJob in capi.hip
     o << "struct EvalJob {\n"
@@ -251,12 +293,12 @@ std::string generate_eval_kernel(const Model_& M) {
          "  double* part; const int* wg_cls; const long long* wg_rng; long long ld; double w1, w2;\n"
          "  int nwg, pw1, pw2, wg_begin;\n};\n";
     o << "extern \"C\" __global__ void " << lb << " ctpwa_eval_group(const EvalJob* __restrict__ jobs, int njobs)\n{\n"
-         "  __shared__ double red[4][" << NACC << "];\n"
+      << decl.str() <<
          "  int lo = 0, hi = njobs - 1;\n"
          "  while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (jobs[mid].wg_begin <= (int)blockIdx.x) lo = mid; else hi = mid - 1; }\n"
          "  const EvalJob J = jobs[lo];\n"
          "  eval_body(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
-         "            (int)blockIdx.x - J.wg_begin, red);\n}\n";
+         "            (int)blockIdx.x - J.wg_begin, red, lbuf);\n}\n";
     return o.str();
 }
 

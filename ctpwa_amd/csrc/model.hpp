@@ -153,8 +153,8 @@ struct Model_ {
     int maxloc = 0;
     // code-generation options of the fused kernel
     int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
-    bool reload_coef = false; // reload coefficients per iteration (scalar loads) instead of pinning
-    bool prefetch = false;    // load the next event's inputs while computing the current one
+    int wgs = 256;           // threads (= events per tile) of a workgroup: 64, 128 or 256
+    bool lds_pipe = false;   // copy the next tile's inputs HBM -> LDS while computing the current one
     // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
     // invariants, FP64 squared masses)
     int mixed = 0;
