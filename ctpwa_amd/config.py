@@ -57,9 +57,35 @@ def _construct_mapping(loader, node, deep=False):
 _Loader.add_constructor(yaml.resolver.BaseResolver.DEFAULT_MAPPING_TAG, _construct_mapping)
 
 
+# top-level sections the likelihood does not read; yaml-cpp accepts some forms in them
+# that strict YAML rejects (e.g. ``bins: 100; range: [1.0, 2.6]`` in Plot)
+_UNUSED_SECTIONS = ("Plot",)
+
+
+def _drop_sections(text: str, names) -> str:
+    out, skip = [], False
+    for line in text.splitlines():
+        top = line and not line[0].isspace() and not line.startswith("#")
+        if top:
+            key = line.split(":", 1)[0].strip()
+            skip = key in names
+        if not skip:
+            out.append(line)
+    return "\n".join(out) + "\n"
+
+
+def parse_yaml_text(text: str):
+    """YAML of a config; on a syntax error the sections the likelihood does not use are
+    dropped and the text parsed again."""
+    try:
+        return yaml.load(text, Loader=_Loader)
+    except yaml.YAMLError:
+        return yaml.load(_drop_sections(text, _UNUSED_SECTIONS), Loader=_Loader)
+
+
 def load_yaml(path: str):
     with open(path, "r", encoding="utf-8") as f:
-        return yaml.load(f, Loader=_Loader)
+        return parse_yaml_text(f.read())
 
 
 def trans_j_value(s) -> float:
@@ -197,7 +223,7 @@ class ConfigParser:
         self._chain_filter: List[str] = []
 
         if text is not None:
-            cfg = yaml.load(text, Loader=_Loader)
+            cfg = parse_yaml_text(text)
         else:
             cfg = load_yaml(config_file)
         if not isinstance(cfg, dict):

@@ -246,6 +246,111 @@ def _polarization(particles: List[Particle]) -> Tuple[int, List[int]]:
     return npol, pm
 
 
+def _trans_folds(cfg: ConfigParser, chain_keys: List[str]) -> Dict[int, Tuple[int, float]]:
+    """``Constraints.trans`` pairs [A, B]: ratio (src/Parameters.cu:586-650,
+    src/Info.cu:322-335): every chain key containing B is folded onto the chain key
+    containing A that carries the same resonance (the text from the last '['), with the
+    real part of the first value as the coupling ratio (1 if none)."""
+    folds: Dict[int, Tuple[int, float]] = {}
+
+    def res_name(k: str) -> str:
+        i = k.rfind("[")
+        return k[i:] if i >= 0 else k
+
+    for c in cfg.constraints:
+        if c.type != "trans" or len(c.names) < 2:
+            continue
+        ratio = c.values[0].real if c.values else 1.0
+        a_name, b_name = c.names[0], c.names[1]
+        ca: Dict[str, int] = {}
+        cb: Dict[str, int] = {}
+        for ci, k in enumerate(chain_keys):
+            if a_name in k:
+                ca[res_name(k)] = ci
+            if b_name in k:
+                cb[res_name(k)] = ci
+        for res in sorted(cb):                           # std::map order
+            if res in ca:
+                folds[cb[res]] = (ca[res], ratio)
+    return folds
+
+
+def _vspace_map(cfg: ConfigParser, amp_chain_key: List[str], chain_keys: List[str]):
+    """Amplitude -> free coupling index and ratio in fit mode 1 (src/Analysis.cu:4382-4478)."""
+    n_amp = len(amp_chain_key)
+    folds = _trans_folds(cfg, chain_keys)
+    active = [ci for ci in range(len(chain_keys)) if ci not in folds]
+    amp_chain = [0] * n_amp
+    amp_ratio = [1.0] * n_amp
+    for a, ck in enumerate(amp_chain_key):
+        oc = chain_keys.index(ck)
+        fc, r = folds.get(oc, (oc, 1.0))
+        amp_chain[a] = active.index(fc)
+        amp_ratio[a] = r
+    owners: Dict[int, List[int]] = {}
+    for a in range(n_amp):
+        if abs(amp_ratio[a] - 1.0) < 1e-10:
+            owners.setdefault(amp_chain[a], []).append(a)
+    if not any(len(v) > 1 for v in owners.values()):
+        return amp_chain, amp_ratio, len(active)
+    # one free coupling per owner amplitude; folded chains map by position onto the
+    # free couplings of the chain they fold into
+    orig: Dict[str, List[int]] = {}
+    for a, ck in enumerate(amp_chain_key):
+        orig.setdefault(ck, []).append(a)
+    folded = {ck for ck, amps in orig.items() if abs(amp_ratio[min(amps)] - 1.0) > 1e-10}
+    amp_free = list(amp_chain)
+    nxt = 0
+    ac_free: Dict[int, List[int]] = {}
+    for ck in sorted(orig):
+        if ck in folded:
+            continue
+        amps = sorted(orig[ck])
+        ac = amp_chain[amps[0]]
+        for a in amps:
+            amp_free[a] = nxt
+            ac_free.setdefault(ac, []).append(nxt)
+            nxt += 1
+    for ck in sorted(orig):
+        if ck not in folded:
+            continue
+        amps = sorted(orig[ck])
+        fi = ac_free.get(amp_chain[amps[0]], [])
+        for pos, a in enumerate(amps):
+            if pos < len(fi):
+                amp_free[a] = fi[pos]
+    return amp_free, amp_ratio, nxt
+
+
+def _apply_var_equal(cfg: ConfigParser, blocks, theta) -> None:
+    """``Constraints.var_equal`` groups of parameter names share one slot: the first name
+    of a group that has a slot owns it, the other members' slots are removed and their
+    uses point at the owner (src/AmpGen.cu:2243-2303)."""
+    if not cfg.var_equal or not theta:
+        return
+    name_to_slot = {t.name: j for j, t in enumerate(theta)}
+    remap: Dict[int, int] = {}
+    for group in cfg.var_equal:
+        if len(group) < 2:
+            continue
+        owner = next((name_to_slot[n] for n in group if n in name_to_slot), None)
+        if owner is None or owner in remap:
+            continue
+        for n in group:
+            j = name_to_slot.get(n)
+            if j is None or j == owner or j in remap:
+                continue
+            remap[j] = owner
+    if not remap:
+        return
+    keep = [j for j in range(len(theta)) if j not in remap]
+    new_index = {j: i for i, j in enumerate(keep)}
+    for b in blocks:
+        for inst in b.res:
+            inst.slots = [(-1 if s < 0 else new_index[remap.get(s, s)]) for s in inst.slots]
+    theta[:] = [theta[j] for j in keep]
+
+
 def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
     if fit_mode != 1:
         raise ConfigError("only fit mode 1 (vspace) is built; chain x step mode is not")
@@ -260,10 +365,6 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
     for p in particles:
         if p.identical_group:
             raise ConfigError("identical-particle symmetrisation is not built")
-    if any(c.type == "trans" for c in cfg.constraints):
-        raise ConfigError("trans constraints are not built")
-    if cfg.var_equal:
-        raise ConfigError("var_equal is not built")
     npol, polmap = _polarization(particles)
 
     topologies: List[Topology] = []
@@ -414,28 +515,13 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
                 gls += len(sl_combos)
 
     n_amp = gls
-    # vspace layout (src/Analysis.cu:4382-4478) without trans folding
-    owners_per_chain: Dict[str, List[int]] = {}
-    for a, ck in enumerate(amp_chain_key):
-        owners_per_chain.setdefault(ck, []).append(a)
-    needs_split = any(len(v) > 1 for v in owners_per_chain.values())
-    amp_free = [0] * n_amp
-    if not needs_split:
-        for a, ck in enumerate(amp_chain_key):
-            amp_free[a] = chain_keys.index(ck)
-        n_free = len(chain_keys)
-    else:
-        nxt = 0
-        for ck in sorted(owners_per_chain):          # std::map order
-            for a in sorted(owners_per_chain[ck]):
-                amp_free[a] = nxt
-                nxt += 1
-        n_free = nxt
-    amp_ratio = [1.0] * n_amp
+    _apply_var_equal(cfg, blocks, theta)
+    amp_free, amp_ratio, n_free = _vspace_map(cfg, amp_chain_key, chain_keys)
     free_names: Dict[int, str] = {}
     for a in range(n_amp):
-        free_names[amp_free[a]] = amp_names[a]
-    param_names = [free_names[i] for i in sorted(free_names)] + [t.name for t in theta]
+        if abs(amp_ratio[a] - 1.0) < 1e-10:
+            free_names[amp_free[a]] = amp_names[a]
+    param_names = [free_names.get(i, "") for i in range(n_free)] + [t.name for t in theta]
 
     gauss = []
     for nm, sig in cfg.gauss_constr.items():

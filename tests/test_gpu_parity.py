@@ -81,3 +81,31 @@ def test_deterministic(oracle_lib):
     b = _gpu_nll_grad(h, p)
     assert a[0] == b[0]
     assert np.array_equal(a[1], b[1])
+
+
+def test_constrained_plan_parity(oracle_lib):
+    """trans [R_m0, R_p0]: -1 and var_equal [rho1450_width, rho1700_width]: shared
+    couplings with ratio -1 and a shared resonance parameter, against the oracle."""
+    from ctpwa_amd import _native
+    text = (W_JPSI_3PI + "Constraints:\n  trans:\n    - [R_m0, R_p0]: -1\n"
+            "  var_equal:\n    - [rho1450_width, rho1700_width]\n")
+    plan = plan_of(text)
+    ph, da, bk = samples(30000, 6000, 1500, 23)
+    wb = np.full(bk.shape[0], 0.8)
+    print("case constrained", flush=True)
+    h = _native.Handle(plan.to_text(), ph, da, bk, None, None, wb)
+    O = oracle_lib.Oracle(plan.to_text(), 3, ph, da, bk, w_bkg=wb)
+    rng = np.random.default_rng(4)
+    p = np.concatenate([rng.uniform(-1, 1, 2 * plan.n_free), [t.init for t in plan.theta]])
+    nll_o, g_o, H_o = O.hessian(p)
+    nll_g, g_g = _gpu_nll_grad(h, p)
+    t = torch.tensor(p, dtype=torch.float64, device="cuda")
+    H = torch.empty((h.n_params, h.n_params), dtype=torch.float64, device="cuda")
+    h.hessian(t.data_ptr(), H.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert abs(nll_g - nll_o) <= 1e-9 * abs(nll_o)
+    assert rel(g_g, g_o) <= 1e-7
+    assert rel(H.cpu().numpy(), H_o) <= 1e-6
+
+
+from ctpwa_amd.workloads import JPSI_3PI as W_JPSI_3PI  # noqa: E402

@@ -102,9 +102,57 @@ Resonances:
 
 def test_unsupported_features_raise():
     with pytest.raises(ConfigError):
-        _plan(W.JPSI_RHO + "Constraints:\n  trans:\n    - [a, b]: -1\n")
-    with pytest.raises(ConfigError):
         _plan(W.JPSI_RHO.replace("model: BWR", "model: Custom"))
+
+
+TRANS = W.JPSI_3PI + "Constraints:\n  trans:\n    - [R_m0, R_p0]: -1\n"
+
+
+def test_trans_folds_chains_onto_their_partner():
+    """[R_m0, R_p0]: -1 folds each rho(+) chain onto the rho(-) chain of the same
+    resonance with ratio -1 (src/Parameters.cu:586-650): 7 free couplings remain."""
+    plan = _plan(TRANS)
+    assert plan.n_amp == 10 and plan.n_free == 7
+    keys = [b.chain_key for b in plan.blocks if b.nsl]
+    # chain key: expanded chain name + "_" + intermediate + "[resonance]" (src/Info.cu:243-247)
+    assert keys[4:] == ["c1_R_m0_R_m0[rho770]", "c1_R_m0_R_m0[rho1450]", "c1_R_m0_R_m0[rho1700]",
+                        "c1_R_p0_R_p0[rho770]", "c1_R_p0_R_p0[rho1450]", "c1_R_p0_R_p0[rho1700]"]
+    assert plan.amp_free == [0, 1, 2, 3, 4, 5, 6, 4, 5, 6]
+    assert plan.amp_ratio == [1.0] * 7 + [-1.0] * 3
+    assert plan.param_names[4].startswith("Jpsi→pip+rho770")
+    assert len(plan.param_names) == 7 + 10
+
+
+def test_trans_with_split_chains():
+    """When a chain holds several amplitudes (J/psi -> K1 K: 2 SL combos x 2) every owner
+    amplitude gets its own coupling and the folded chain maps by position."""
+    text = W.JPSI_KKPIPI + "Constraints:\n  trans:\n    - [K1p, K1m]: [[0.5, 0.0]]\n"
+    plan = _plan(text)
+    base = _plan(W.JPSI_KKPIPI)
+    assert base.n_free == 20 and plan.n_free == 16
+    a_p = [a for a, n in enumerate(plan.amp_names) if "→Km+K1_1270" in n]
+    a_m = [a for a, n in enumerate(plan.amp_names) if "→Kp+K1_1270" in n]
+    assert len(a_p) == len(a_m) == 4
+    assert [plan.amp_free[a] for a in a_m] == [plan.amp_free[a] for a in a_p]
+    assert all(plan.amp_ratio[a] == 0.5 for a in a_m)
+    assert all(plan.amp_ratio[a] == 1.0 for a in a_p)
+    assert sorted(set(plan.amp_free)) == list(range(16))
+
+
+def test_var_equal_merges_slots():
+    text = W.JPSI_3PI + "Constraints:\n  var_equal:\n    - [rho1450_width, rho1700_width]\n"
+    plan = _plan(text)
+    base = _plan(W.JPSI_3PI)
+    assert plan.n_theta == base.n_theta - 1
+    names = [t.name for t in plan.theta]
+    assert "rho1700_width" not in names
+    w = names.index("rho1450_width")
+    for b in plan.blocks:
+        for r in b.res:
+            if r.name in ("rho1450", "rho1700"):
+                assert r.slots[1] == w
+            if r.name == "f2_1270":
+                assert r.slots == [names.index("f2_1270_mass"), names.index("f2_1270_width")]
 
 
 def test_barrier_scope_resolution():
