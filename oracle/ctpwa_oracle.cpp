@@ -1154,6 +1154,11 @@ void orc_flatte(double m, double m0, int nch, const double* g, const double* ch,
     CJ z = flatte_jet(m, Jet(m0), nch, gj.data(), ch);
     out[0] = z.re.v; out[1] = z.im.v;
 }
+void orc_gs(double m, double m0, double g0, double q, double q0, double mpi, double* out) {
+    g_nv = 0;
+    CJ z = gs_jet(m, Jet(m0), Jet(g0), q, Jet(q0), mpi);
+    out[0] = z.re.v; out[1] = z.im.v;
+}
 void orc_node_tensor(const double* p1, int dj1, const double* p2, int dj2, int dj, int dS, int L, double* out) {
     std::vector<cd> amp(dj * dj1 * dj2);
     pwa_amp(amp.data(), LV(p1[0], p1[1], p1[2], p1[3]), dj1, LV(p2[0], p2[1], p2[2], p2[3]), dj2, dj, dS, L);

@@ -60,6 +60,7 @@ def lib():
         L.orc_bwr.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_flatte.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_gs.argtypes = [ctypes.c_double] * 6 + [_dp]
         L.orc_node_tensor.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
@@ -175,6 +176,13 @@ def flatte(m, m0, g, ch):
     ch = np.ascontiguousarray(ch, dtype=np.float64).reshape(-1)
     out = np.zeros(2)
     lib().orc_flatte(m, m0, len(g), _ptr(g), _ptr(ch), _ptr(out))
+    return complex(out[0], out[1])
+
+
+def gs(m, m0, g0, q, q0, mpi):
+    """Gounaris-Sakurai propagator (without the barrier factor)."""
+    out = np.zeros(2)
+    lib().orc_gs(m, m0, g0, q, q0, mpi, _ptr(out))
     return complex(out[0], out[1])
 
 

@@ -109,3 +109,37 @@ def test_constrained_plan_parity(oracle_lib):
 
 
 from ctpwa_amd.workloads import JPSI_3PI as W_JPSI_3PI  # noqa: E402
+
+
+MODELS_3PI = W_JPSI_3PI.replace(
+    "  rho770:  {J: 1, P: -1, model: BWR, parameters: [0.775, 0.149], free: [0, 1]}",
+    "  rho770:  {J: 1, P: -1, model: GS, parameters: [0.775, 0.149], free: [0, 1], channels: [[0.13975, 0.13975]]}").replace(
+    "  rho1450: {J: 1, P: -1, model: BWR, parameters: [1.465, 0.40],  free: [0, 1]}",
+    "  rho1450: {J: 1, P: -1, model: BW, parameters: [1.465, 0.40],  free: [0, 1]}").replace(
+    "  rho1700: {J: 1, P: -1, model: BWR, parameters: [1.72, 0.25],   free: [0, 1]}",
+    "  rho1700: {J: 1, P: -1, model: ONE, parameters: [1.72]}") + "Constraints:\n  bf_d: 4.0\n"
+
+
+def test_all_builtin_lineshapes_parity(oracle_lib):
+    """GS, BW, ONE and BWR (Flatte is in the four-body cascade) with a non-default barrier
+    radius, against the oracle: NLL, gradient, Hessian."""
+    from ctpwa_amd import _native
+    plan = plan_of(MODELS_3PI)
+    models = sorted({r.model for b in plan.blocks for r in b.res})
+    assert models == ["BW", "BWR", "GS", "ONE"]
+    ph, da, bk = samples(30000, 6000, 1500, 25)
+    wb = np.full(bk.shape[0], 0.8)
+    print("case models", flush=True)
+    h = _native.Handle(plan.to_text(), ph, da, bk, None, None, wb)
+    O = oracle_lib.Oracle(plan.to_text(), 3, ph, da, bk, w_bkg=wb)
+    p = perturbed(W.nominal_params(plan, 1), plan, 15)
+    nll_o, g_o, H_o = O.hessian(p)
+    nll_g, g_g = _gpu_nll_grad(h, p)
+    t = torch.tensor(p, dtype=torch.float64, device="cuda")
+    H = torch.empty((h.n_params, h.n_params), dtype=torch.float64, device="cuda")
+    h.hessian(t.data_ptr(), H.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    print(nll_o, nll_g, rel(g_g, g_o), rel(H.cpu().numpy(), H_o), flush=True)
+    assert abs(nll_g - nll_o) <= 1e-9 * abs(nll_o)
+    assert rel(g_g, g_o) <= 1e-7
+    assert rel(H.cpu().numpy(), H_o) <= 1e-6

@@ -130,6 +130,53 @@ def test_bwr_off_pole_formula(oracle_lib):
     assert abs(oracle_lib.bwr(m, m0, g0, L, q, q0, d) - ref) < 1e-12 * abs(ref)
 
 
+def _breakup(m, m1, m2):
+    return math.sqrt(max((m * m - (m1 + m2) ** 2) * (m * m - (m1 - m2) ** 2), 0.0)) / (2 * m)
+
+
+def test_gs_on_pole_and_derivative_of_h(oracle_lib):
+    """Gounaris-Sakurai (Phys. Rev. Lett. 21 (1968) 244): at m = m0 the dispersive term
+    vanishes and F = i (1 + d g0/m0) / (m0 g0); h'(m0) = dh/dm^2 at m0 (checked against a
+    central difference of h)."""
+    mpi, m0, g0 = 0.13957, 0.775, 0.149
+    q0 = _breakup(m0, mpi, mpi)
+    d = (3 / math.pi) * mpi ** 2 / q0 ** 2 * math.log((m0 + 2 * q0) / (2 * mpi)) + m0 / (2 * math.pi * q0) \
+        - mpi ** 2 * m0 / (math.pi * q0 ** 3)
+    got = oracle_lib.gs(m0, m0, g0, q0, q0, mpi)
+    ref = 1j * (1 + d * g0 / m0) / (m0 * g0)
+    assert abs(got - ref) < 1e-12 * abs(ref)
+
+    def h(m):
+        q = _breakup(m, mpi, mpi)
+        return (2 / math.pi) * (q / m) * math.log((m + 2 * q) / (2 * mpi))
+    eps = 1e-5
+    dh_num = (h(math.sqrt(m0 ** 2 + eps)) - h(math.sqrt(m0 ** 2 - eps))) / (2 * eps)
+    h0 = h(m0)
+    dh_closed = h0 * (1 / (8 * q0 ** 2) - 1 / (2 * m0 ** 2)) + 1 / (2 * math.pi * m0 ** 2)
+    assert abs(dh_num - dh_closed) < 1e-8 * abs(dh_closed)
+
+
+@pytest.mark.parametrize("m", [0.45, 0.7, 0.9, 1.3])
+def test_gs_matches_textbook_form(oracle_lib, m):
+    """Independent evaluation of the GS propagator: the dispersive term written with the
+    numerical derivative of h, Gamma(m) = g0 (q/q0)^3 m0/m."""
+    mpi, m0, g0 = 0.13957, 0.775, 0.149
+    q0, q = _breakup(m0, mpi, mpi), _breakup(m, mpi, mpi)
+
+    def h(x):
+        k = _breakup(x, mpi, mpi)
+        return (2 / math.pi) * (k / x) * math.log((x + 2 * k) / (2 * mpi))
+    eps = 1e-6
+    dh = (h(math.sqrt(m0 ** 2 + eps)) - h(math.sqrt(m0 ** 2 - eps))) / (2 * eps)
+    f = g0 * m0 ** 2 / q0 ** 3 * (q ** 2 * (h(m) - h(m0)) + (m0 ** 2 - m ** 2) * q0 ** 2 * dh)
+    d = (3 / math.pi) * mpi ** 2 / q0 ** 2 * math.log((m0 + 2 * q0) / (2 * mpi)) + m0 / (2 * math.pi * q0) \
+        - mpi ** 2 * m0 / (math.pi * q0 ** 3)
+    gam = g0 * (q / q0) ** 3 * m0 / m
+    ref = (1 + d * g0 / m0) / (m0 ** 2 - m ** 2 + f - 1j * m0 * gam)
+    got = oracle_lib.gs(m, m0, g0, q, q0, mpi)
+    assert abs(got - ref) < 2e-7 * abs(ref)
+
+
 def test_flatte_formula(oracle_lib):
     m, m0 = 1.02, 0.99
     g = np.array([0.2, 0.4])
