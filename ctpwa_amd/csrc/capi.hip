@@ -196,6 +196,7 @@ struct ctpwa_handle {
     hipModule_t mod = nullptr;
     hipFunction_t fn_eval = nullptr;       // one likelihood
     hipFunction_t fn_group = nullptr;      // many likelihoods of this model in one launch
+    hipFunction_t fn_aux = nullptr;        // amplitudes / intensities (models with Custom lineshapes)
     std::string src;
     // Hessian workspace (allocated on the first ctpwa_hessian call)
     bool hess_ready = false;
@@ -241,6 +242,7 @@ struct ctpwa_handle {
 static void apply_codegen_env(Model_& M) {
     if (const char* e = std::getenv("CTPWA_EVAL_WAVES")) M.min_waves = std::atoi(e);
     if (const char* e = std::getenv("CTPWA_EVAL_LDS")) M.lds_pipe = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CTPWA_EVAL_PREFETCH")) M.prefetch_sl = std::atoi(e) != 0;
     if (const char* e = std::getenv("CTPWA_EVAL_WGS")) {
         int w = std::atoi(e);
         if (w != 64 && w != 128 && w != 256) throw std::runtime_error("CTPWA_EVAL_WGS must be 64, 128 or 256");
@@ -570,7 +572,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->prep = PrepArgs{d_blocks, d_res, d_nodes2, d_topos2, d_slcL, d_slc0, d_dens, d_amps,
                            (int)dens.size(), P.namp, P.nfree, P.ntheta,
                            (long long)M.off_v(), (long long)M.off_c(), (long long)M.off_rec(),
-                           nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                           (long long)M.off_th(), nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         {
             std::vector<char> blob;
             auto put = [&](const void* p, size_t n) -> int {
@@ -623,6 +625,12 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
                     int li = local(amps[a].block, j);
                     if (li >= 0) the.push_back(DGEnt{-a - 1, amps[a].rec * REC + 1 + li, 2.0});
                 }
+                // Custom factors: accumulated Re(z dPsi/dtheta_j) directly (no coefficient record)
+                for (size_t t = 0; t < M.cterms.size(); ++t) {
+                    const CTerm& T = M.cterms[t];
+                    if (M.bvars[M.customs[T.custom].block][T.local] == j)
+                        the.push_back(DGEnt{1 + 2 * P.namp + (int)th.size() + (int)t, -1, 2.0});
+                }
                 thp.push_back((int)the.size());
             }
             for (int fi = 0; fi < P.nfree; ++fi) {
@@ -657,6 +665,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->mod = compile_eval(device, h->src);
         HIPCHECK(hipModuleGetFunction(&h->fn_eval, h->mod, "ctpwa_eval"));
         HIPCHECK(hipModuleGetFunction(&h->fn_group, h->mod, "ctpwa_eval_group"));
+        if (!M.customs.empty()) HIPCHECK(hipModuleGetFunction(&h->fn_aux, h->mod, "ctpwa_aux"));
         h->used_inv.assign(M.inv.size(), 0);
         for (const DenSpec& D : M.dens) {
             for (const Term& t : D.re) if (t.inv >= 0) h->used_inv[t.inv] = 1;
@@ -770,6 +779,7 @@ static void hess_setup(ctpwa_handle* h) {
 
 extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream) {
     if (!h || !params_dev || !hess_dev) return fail("ctpwa_hessian: NULL argument");
+    if (!h->M.customs.empty()) return fail("ctpwa_hessian: the Hessian of models with Custom lineshapes is not built");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     try {
@@ -838,7 +848,17 @@ static int aux_call(ctpwa_handle* h, const double* params_dev, int cls, double* 
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
-    if (e == hipSuccess) e = ctpwa_launch_aux(&h->aux, h->off[cls], h->n[cls], out_dev, intensity, s);
+    if (e == hipSuccess && h->fn_aux) {
+        // generated kernel of the model (Custom lineshapes)
+        long long e0 = h->off[cls], n = h->n[cls];
+        if (n > 0) {
+            int flag = intensity;
+            void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->ntot, &e0, &n, &out_dev, &flag};
+            e = hipModuleLaunchKernel(h->fn_aux, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, s, args, nullptr);
+        }
+    } else if (e == hipSuccess) {
+        e = ctpwa_launch_aux(&h->aux, h->off[cls], h->n[cls], out_dev, intensity, s);
+    }
     if (e != hipSuccess) return fail(hipGetErrorString(e));
     return 0;
 }

@@ -38,6 +38,229 @@ static std::string D(double x) {
 }
 
 
+// Complex numbers carrying first derivatives over NCV block-local parameters (forward
+// mode), for Custom lineshapes; the operations and functions follow the expression
+// language's complex semantics (principal log, exp(b log a) powers, principal square
+// root, real q0/Bf/BW/BWR model functions as in the built-in lineshapes).
+static const char* CD_PREAMBLE = R"(
+struct CD { double r, i; double dr[NCV], di[NCV]; };
+__device__ __forceinline__ CD cd_c(double r, double i) {
+  CD z; z.r = r; z.i = i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = 0.0; z.di[k] = 0.0; }
+  return z;
+}
+__device__ __forceinline__ CD cd_var(double v, int j) { CD z = cd_c(v, 0.0); z.dr[j] = 1.0; return z; }
+__device__ __forceinline__ CD cd_add(const CD& a, const CD& b) {
+  CD z; z.r = a.r + b.r; z.i = a.i + b.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = a.dr[k] + b.dr[k]; z.di[k] = a.di[k] + b.di[k]; }
+  return z;
+}
+__device__ __forceinline__ CD cd_sub(const CD& a, const CD& b) {
+  CD z; z.r = a.r - b.r; z.i = a.i - b.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = a.dr[k] - b.dr[k]; z.di[k] = a.di[k] - b.di[k]; }
+  return z;
+}
+__device__ __forceinline__ CD cd_neg(const CD& a) {
+  CD z; z.r = -a.r; z.i = -a.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = -a.dr[k]; z.di[k] = -a.di[k]; }
+  return z;
+}
+__device__ __forceinline__ CD cd_mul(const CD& a, const CD& b) {
+  CD z; z.r = a.r * b.r - a.i * b.i; z.i = a.r * b.i + a.i * b.r;
+  for (int k = 0; k < NCV; ++k) {
+    z.dr[k] = a.dr[k] * b.r - a.di[k] * b.i + a.r * b.dr[k] - a.i * b.di[k];
+    z.di[k] = a.dr[k] * b.i + a.di[k] * b.r + a.r * b.di[k] + a.i * b.dr[k];
+  }
+  return z;
+}
+// value f(a) with f'(a) = (pr + i pi): derivative chain rule
+__device__ __forceinline__ CD cd_chain(double fr, double fi, double pr, double pi, const CD& a) {
+  CD z; z.r = fr; z.i = fi;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = pr * a.dr[k] - pi * a.di[k]; z.di[k] = pr * a.di[k] + pi * a.dr[k]; }
+  return z;
+}
+__device__ __forceinline__ CD cd_inv(const CD& a) {
+  const double q = 1.0 / (a.r * a.r + a.i * a.i);
+  const double vr = a.r * q, vi = -a.i * q;                 // 1/a
+  return cd_chain(vr, vi, -(vr * vr - vi * vi), -(2.0 * vr * vi), a);   // d(1/a) = -da/a^2
+}
+__device__ __forceinline__ CD cd_div(const CD& a, const CD& b) { return cd_mul(a, cd_inv(b)); }
+__device__ __forceinline__ CD cd_exp(const CD& a) {
+  const double e = exp(a.r), vr = e * cos(a.i), vi = e * sin(a.i);
+  return cd_chain(vr, vi, vr, vi, a);
+}
+__device__ __forceinline__ CD cd_log(const CD& a) {
+  const double q = 1.0 / (a.r * a.r + a.i * a.i);
+  return cd_chain(0.5 * log(a.r * a.r + a.i * a.i), atan2(a.i, a.r), a.r * q, -a.i * q, a);
+}
+__device__ __forceinline__ CD cd_pow(const CD& a, const CD& b) { return cd_exp(cd_mul(b, cd_log(a))); }
+__device__ __forceinline__ CD cd_sin(const CD& a) {
+  const double ep = exp(a.i), em = exp(-a.i);
+  const double sr = sin(a.r) * (ep + em) * 0.5, si = cos(a.r) * (ep - em) * 0.5;
+  const double cr = cos(a.r) * (ep + em) * 0.5, ci = -sin(a.r) * (ep - em) * 0.5;
+  return cd_chain(sr, si, cr, ci, a);
+}
+__device__ __forceinline__ CD cd_cos(const CD& a) {
+  const double ep = exp(a.i), em = exp(-a.i);
+  const double sr = sin(a.r) * (ep + em) * 0.5, si = cos(a.r) * (ep - em) * 0.5;
+  const double cr = cos(a.r) * (ep + em) * 0.5, ci = -sin(a.r) * (ep - em) * 0.5;
+  return cd_chain(cr, ci, -sr, -si, a);
+}
+__device__ __forceinline__ CD cd_sqrt(const CD& a) {
+  double vr, vi;
+  if (a.i == 0.0) {
+    if (a.r >= 0.0) { vr = sqrt(a.r); vi = 0.0; } else { vr = 0.0; vi = sqrt(-a.r); }
+  } else {
+    const double r = sqrt(a.r * a.r + a.i * a.i);
+    vr = sqrt((r + a.r) * 0.5);
+    vi = sqrt((r - a.r) * 0.5);
+    if (a.i < 0.0) vi = -vi;
+  }
+  // d sqrt(a) = da / (2 sqrt(a))
+  const double q = 0.5 / (vr * vr + vi * vi);
+  return cd_chain(vr, vi, vr * q, -vi * q, a);
+}
+__device__ __forceinline__ CD cd_abs(const CD& a) {
+  const double v = sqrt(a.r * a.r + a.i * a.i);
+  CD z = cd_c(v, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = (a.r * a.dr[k] + a.i * a.di[k]) / v;
+  return z;
+}
+__device__ __forceinline__ CD cd_re(const CD& a) {
+  CD z = cd_c(a.r, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = a.dr[k];
+  return z;
+}
+__device__ __forceinline__ CD cd_im(const CD& a) {
+  CD z = cd_c(a.i, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = a.di[k];
+  return z;
+}
+__device__ __forceinline__ CD cd_conj(const CD& a) {
+  CD z; z.r = a.r; z.i = -a.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = a.dr[k]; z.di[k] = -a.di[k]; }
+  return z;
+}
+// real parts only from here on: breakup at x (computeQ0AD), 0 below threshold
+__device__ __forceinline__ CD cd_q0(const CD& x, const CD& m1, const CD& m2) {
+  const double s = m1.r + m2.r, d = m1.r - m2.r, x2 = x.r * x.r;
+  const double q2 = (x2 - s * s) * (x2 - d * d) / (4.0 * x2);
+  if (q2 < 0.0) return cd_c(0.0, 0.0);
+  const double q = sqrt(q2);
+  // dq2/dx = (x^4 - s^2 d^2) / (2 x^3)
+  const double dq2 = (x2 * x2 - s * s * d * d) / (2.0 * x2 * x.r);
+  CD z = cd_c(q, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = (q > 0.0) ? dq2 / (2.0 * q) * x.dr[k] : 0.0;
+  return z;
+}
+__device__ __forceinline__ void bfpoly_d(int L, double z, double& n, double& dn) {
+  // N_L(z) with z the argument q d, and dN/dz
+  const double z2 = z * z;
+  double p, dp;   // polynomial in z2 and its derivative
+  switch (L) {
+  case 1: p = 1.0 + z2; dp = 1.0; break;
+  case 2: p = 9.0 + z2 * (3.0 + z2); dp = 3.0 + 2.0 * z2; break;
+  case 3: p = 225.0 + z2 * (45.0 + z2 * (6.0 + z2)); dp = 45.0 + z2 * (12.0 + 3.0 * z2); break;
+  case 4: p = 11025.0 + z2 * (1575.0 + z2 * (135.0 + z2 * (10.0 + z2)));
+          dp = 1575.0 + z2 * (270.0 + z2 * (30.0 + 4.0 * z2)); break;
+  case 5: p = 893025.0 + z2 * (99225.0 + z2 * (6300.0 + z2 * (315.0 + z2 * (15.0 + z2))));
+          dp = 99225.0 + z2 * (12600.0 + z2 * (945.0 + z2 * (60.0 + 5.0 * z2))); break;
+  case 6: p = 540326025.0 + z2 * (6185025.0 + z2 * (363825.0 + z2 * (17325.0 + z2 * (630.0 + z2 * (21.0 + z2)))));
+          dp = 6185025.0 + z2 * (727650.0 + z2 * (51975.0 + z2 * (2520.0 + z2 * (105.0 + 6.0 * z2)))); break;
+  default: p = 1.0; dp = 0.0;
+  }
+  n = p;
+  dn = dp * 2.0 * z;
+}
+// Bf(L, q, q0, d) = sqrt(N_L(q0 d) / N_L(q d))
+__device__ __forceinline__ CD cd_bf(int L, const CD& q, const CD& q0, const CD& d) {
+  if (L == 0 || L > 6) return cd_c(1.0, 0.0);
+  double n0, dn0, n1, dn1;
+  bfpoly_d(L, q0.r * d.r, n0, dn0);
+  bfpoly_d(L, q.r * d.r, n1, dn1);
+  const double v = sqrt(n0 / n1);
+  CD z = cd_c(v, 0.0);
+  for (int k = 0; k < NCV; ++k) {
+    const double dn0k = dn0 * (q0.dr[k] * d.r + q0.r * d.dr[k]);
+    const double dn1k = dn1 * (q.dr[k] * d.r + q.r * d.dr[k]);
+    z.dr[k] = 0.5 * v * (dn0k / n0 - dn1k / n1);
+  }
+  return z;
+}
+// 1 / (m0^2 - m^2 - i m0 g0)
+__device__ __forceinline__ CD cd_bw(const CD& m, const CD& m0, const CD& g0) {
+  return cd_inv(cd_sub(cd_sub(cd_mul(cd_re(m0), cd_re(m0)), cd_mul(cd_re(m), cd_re(m))),
+                       cd_mul(cd_c(0.0, 1.0), cd_mul(cd_re(m0), cd_re(g0)))));
+}
+// 1 / (m0^2 - m^2 - i m0 g0 (q/q0)^(2L+1) (m0/m) Bf^2)
+__device__ __forceinline__ CD cd_bwr(int L, const CD& m, const CD& m0, const CD& g0, const CD& q, const CD& q0,
+                                     const CD& d) {
+  const CD bf = cd_bf(L, q, q0, d);
+  CD r = cd_div(cd_re(q), cd_re(q0));
+  CD p = cd_c(1.0, 0.0);
+  for (int k = 0; k < 2 * L + 1; ++k) p = cd_mul(p, r);
+  const CD gam = cd_mul(cd_mul(cd_mul(cd_re(g0), p), cd_div(cd_re(m0), cd_re(m))), cd_mul(bf, bf));
+  return cd_inv(cd_sub(cd_sub(cd_mul(cd_re(m0), cd_re(m0)), cd_mul(cd_re(m), cd_re(m))),
+                       cd_mul(cd_c(0.0, 1.0), cd_mul(cd_re(m0), gam))));
+}
+)";
+
+// code of one Custom expression (prefix tokens) as a nested CD expression
+static std::string custom_code(const std::vector<std::string>& t, size_t& i, const CustomSpec& C,
+                               const std::vector<int>& vars) {
+    if (i >= t.size()) throw std::runtime_error("Custom expression is truncated");
+    const std::string k = t[i++];
+    auto sub = [&]() { return custom_code(t, i, C, vars); };
+    auto num = [](const std::string& x) {
+        std::ostringstream s;
+        s.precision(17);
+        s << std::stod(x);
+        std::string r = s.str();
+        if (r.find_first_of(".eEn") == std::string::npos) r += ".0";
+        return r;
+    };
+    if (k.rfind("n:", 0) == 0) return "cd_c(" + num(k.substr(2)) + ", 0.0)";
+    if (k == "I") return "cd_c(0.0, 1.0)";
+    if (k == "v:m") return "cd_c(xm, 0.0)";
+    if (k == "v:q") return "cd_c(xq, 0.0)";
+    if (k.rfind("p:", 0) == 0) {
+        int pk = std::stoi(k.substr(2));
+        if (pk < 0 || pk >= (int)C.param_val.size()) throw std::runtime_error("Custom parameter index out of range");
+        int s = C.param_slot[pk];
+        if (s < 0) return "cd_c(" + num(std::to_string(C.param_val[pk])) + ", 0.0)";
+        for (size_t li = 0; li < vars.size(); ++li)
+            if (vars[li] == s) return "cd_var(tb[OFFT + " + std::to_string(s) + "], " + std::to_string(li) + ")";
+        throw std::runtime_error("Custom parameter slot is not a block variable");
+    }
+    if (k == "+") { std::string a = sub(), b = sub(); return "cd_add(" + a + ", " + b + ")"; }
+    if (k == "-") { std::string a = sub(), b = sub(); return "cd_sub(" + a + ", " + b + ")"; }
+    if (k == "*") { std::string a = sub(), b = sub(); return "cd_mul(" + a + ", " + b + ")"; }
+    if (k == "/") { std::string a = sub(), b = sub(); return "cd_div(" + a + ", " + b + ")"; }
+    if (k == "^") { std::string a = sub(), b = sub(); return "cd_pow(" + a + ", " + b + ")"; }
+    if (k == "neg") return "cd_neg(" + sub() + ")";
+    if (k.rfind("f:", 0) == 0) {
+        std::string f = k.substr(2);
+        static const char* ok[] = {"exp", "log", "sin", "cos", "sqrt", "abs", "re", "im", "conj"};
+        bool known = false;
+        for (const char* o : ok) if (f == o) known = true;
+        if (!known) throw std::runtime_error("Custom expression: unknown function " + f);
+        return "cd_" + f + "(" + sub() + ")";
+    }
+    if (k == "q0") { std::string a = sub(), b = sub(), c = sub(); return "cd_q0(" + a + ", " + b + ", " + c + ")"; }
+    if (k.rfind("bf:", 0) == 0) {
+        std::string L = std::to_string(std::stoi(k.substr(3)));
+        std::string a = sub(), b = sub(), c = sub();
+        return "cd_bf(" + L + ", " + a + ", " + b + ", " + c + ")";
+    }
+    if (k == "bw") { std::string a = sub(), b = sub(), c = sub(); return "cd_bw(" + a + ", " + b + ", " + c + ")"; }
+    if (k.rfind("bwr:", 0) == 0) {
+        std::string L = std::to_string(std::stoi(k.substr(4)));
+        std::string a = sub(), b = sub(), c = sub(), d = sub(), e = sub(), f = sub();
+        return "cd_bwr(" + L + ", " + a + ", " + b + ", " + c + ", " + d + ", " + e + ", " + f + ")";
+    }
+    throw std::runtime_error("Custom expression: unknown token " + k);
+}
+
 std::string generate_eval_kernel(const Model_& M) {
     const Plan& P = M.P;
     int NU = (int)M.us.size(), NP = P.npol, NA = P.namp;
@@ -48,9 +271,15 @@ std::string generate_eval_kernel(const Model_& M) {
         for (const Term& t : Dn.re) if (t.inv >= 0) used[t.inv] = 1;
         for (const Term& t : Dn.im) if (t.inv >= 0) used[t.inv] = 1;
     }
+    for (const CustomSpec& C : M.customs) { used[C.inv_m] = 1; used[C.inv_q] = 1; }
     std::ostringstream o;
     long long tile_bytes = 16;       // LDS tile buffer of the pipelined form (per buffer)
     o << "// generated: fused likelihood kernel\n";
+    if (!M.customs.empty()) {
+        size_t ncv = 1;
+        for (const CustomSpec& C : M.customs) ncv = std::max(ncv, M.bvars[C.block].size());
+        o << "#define NCV " << ncv << "\n#define OFFT " << M.off_th() << "LL\n" << CD_PREAMBLE;
+    }
     // reciprocal: hardware estimate refined by two Newton steps (~1 ulp)
     o << "__device__ __forceinline__ double frcp(double x) {\n"
          "  double r = __builtin_amdgcn_rcp(x);\n"
@@ -96,6 +325,8 @@ std::string generate_eval_kernel(const Model_& M) {
     o << "  double av = 0.0;\n";
     for (int a = 0; a < NA; ++a) o << "  double ax" << a << "r = 0.0, ax" << a << "i = 0.0;\n";
     for (int t = 0; t < NTH; ++t) o << "  double at" << t << " = 0.0;\n";
+    const int NCT = (int)M.cterms.size();
+    for (int t = 0; t < NCT; ++t) o << "  double ct" << t << " = 0.0;\n";
     const int WGS = M.wgs, NW = M.wgs / 64;
     if (M.lds_pipe) {
         // Tiles of WGS events: every per-event input row of the next tile is copied
@@ -158,40 +389,81 @@ std::string generate_eval_kernel(const Model_& M) {
         for (const Row& r : rows) o << r.rd;
         tile_bytes = tb;
     } else {
-        o << "  for (long long e = e0 + threadIdx.x; e < e1; e += " << WGS << ") {\n";
-        for (int u = 0; u < NU; ++u)
-            for (int p = 0; p < NP; ++p) {
-                std::string nm = std::to_string(u) + "_" + std::to_string(p);
-                o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
-                o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
-            }
+        if (M.prefetch_sl) {
+            // the SL values of the thread's next event are loaded one iteration ahead
+            // (registers), so their latency overlaps the current event's arithmetic
+            o << "  long long e = e0 + threadIdx.x;\n  const long long ep = (e < e1) ? e : e0;\n";
+            for (int u = 0; u < NU; ++u)
+                for (int p = 0; p < NP; ++p)
+                    o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + ep];\n";
+            o << "  for (; e < e1; e += " << WGS << ") {\n";
+            for (int u = 0; u < NU; ++u)
+                for (int p = 0; p < NP; ++p) {
+                    std::string nm = std::to_string(u) + "_" + std::to_string(p);
+                    o << "    const double2 s" << nm << " = " << sl_conv("n" + nm) << ";\n";
+                }
+            o << "    { const long long en = (e + " << WGS << " < e1) ? e + " << WGS << " : e;\n";
+            for (int u = 0; u < NU; ++u)
+                for (int p = 0; p < NP; ++p)
+                    o << "      n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + en];\n";
+            o << "    }\n";
+        } else {
+            o << "  for (long long e = e0 + threadIdx.x; e < e1; e += " << WGS << ") {\n";
+            for (int u = 0; u < NU; ++u)
+                for (int p = 0; p < NP; ++p) {
+                    std::string nm = std::to_string(u) + "_" + std::to_string(p);
+                    o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+                    o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
+                }
+        }
         for (size_t i = 0; i < M.inv.size(); ++i)
             if (used[i]) o << "    const double x" << i << " = " << inv_load(i, "e") << ";\n";
     }
-    auto term_expr = [&](int d, const Term& t) {
-        std::string coef = (t.coef < 0) ? D(t.cval) : ("k" + std::to_string(d) + "_" + std::to_string(t.coef));
-        if (t.inv < 0) return coef;
-        return coef + " * x" + std::to_string(t.inv);
-    };
-    for (size_t d = 0; d < M.dens.size(); ++d) {
-        const DenSpec& Dn = M.dens[d];
-        o << "    double dr" << d << " = 0.0, di" << d << " = 0.0;\n";
-        for (const Term& t : Dn.re) o << "    dr" << d << " += " << term_expr((int)d, t) << ";\n";
-        for (const Term& t : Dn.im) o << "    di" << d << " += " << term_expr((int)d, t) << ";\n";
-        o << "    const double q" << d << " = frcp(dr" << d << " * dr" << d << " + di" << d << " * di" << d << ");\n";
-        o << "    const double pr" << d << " = dr" << d << " * q" << d << ", pi" << d << " = di" << d << " * q" << d << ";\n";
-    }
-    for (size_t b = 0; b < P.blocks.size(); ++b) {
-        const std::vector<int>& ds = M.block_dens[b];
-        if (P.blocks[b].nsl == 0) continue;
-        if (ds.empty()) { o << "    const double br" << b << " = 1.0, bi" << b << " = 0.0;\n"; continue; }
-        o << "    double br" << b << " = pr" << ds[0] << ", bi" << b << " = pi" << ds[0] << ";\n";
-        for (size_t k = 1; k < ds.size(); ++k) {
-            int d = ds[k];
-            o << "    { const double tr = br" << b << " * pr" << d << " - bi" << b << " * pi" << d
-              << "; bi" << b << " = br" << b << " * pi" << d << " + bi" << b << " * pr" << d << "; br" << b << " = tr; }\n";
+    // per-event core: denominators, Custom factors, block products (shared by the fused
+    // kernel and the amplitude/intensity kernel of models with Custom lineshapes)
+    auto emit_core = [&](std::ostringstream& o) {
+        auto term_expr = [&](int d, const Term& t) {
+            std::string coef = (t.coef < 0) ? D(t.cval) : ("k" + std::to_string(d) + "_" + std::to_string(t.coef));
+            if (t.inv < 0) return coef;
+            return coef + " * x" + std::to_string(t.inv);
+        };
+        for (size_t d = 0; d < M.dens.size(); ++d) {
+            const DenSpec& Dn = M.dens[d];
+            o << "    double dr" << d << " = 0.0, di" << d << " = 0.0;\n";
+            for (const Term& t : Dn.re) o << "    dr" << d << " += " << term_expr((int)d, t) << ";\n";
+            for (const Term& t : Dn.im) o << "    di" << d << " += " << term_expr((int)d, t) << ";\n";
+            o << "    const double q" << d << " = frcp(dr" << d << " * dr" << d << " + di" << d << " * di" << d << ");\n";
+            o << "    const double pr" << d << " = dr" << d << " * q" << d << ", pi" << d << " = di" << d << " * q" << d << ";\n";
         }
-    }
+        // Custom lineshape factors with their derivatives over the block's local variables
+        for (size_t c = 0; c < M.customs.size(); ++c) {
+            const CustomSpec& C = M.customs[c];
+            size_t ti = 0;
+            std::string code = custom_code(C.expr, ti, C, M.bvars[C.block]);
+            if (ti != C.expr.size()) throw std::runtime_error("Custom expression has trailing tokens");
+            o << "    CD F" << c << ";\n    { const double xm = x" << C.inv_m << ", xq = x" << C.inv_q << ";\n"
+              << "      F" << c << " = " << code << "; }\n";
+        }
+        for (size_t b = 0; b < P.blocks.size(); ++b) {
+            const std::vector<int>& ds = M.block_dens[b];
+            if (P.blocks[b].nsl == 0) continue;
+            if (ds.empty()) o << "    double br" << b << " = 1.0, bi" << b << " = 0.0;\n";
+            else o << "    double br" << b << " = pr" << ds[0] << ", bi" << b << " = pi" << ds[0] << ";\n";
+            for (size_t k = 1; k < ds.size(); ++k) {
+                int d = ds[k];
+                o << "    { const double tr = br" << b << " * pr" << d << " - bi" << b << " * pi" << d
+                  << "; bi" << b << " = br" << b << " * pi" << d << " + bi" << b << " * pr" << d << "; br" << b << " = tr; }\n";
+            }
+            if (!M.block_customs[b].empty()) {
+                // product of the denominators alone (for the derivatives through Custom factors)
+                o << "    const double pdr" << b << " = br" << b << ", pdi" << b << " = bi" << b << ";\n";
+                for (int c : M.block_customs[b])
+                    o << "    { const double tr = br" << b << " * F" << c << ".r - bi" << b << " * F" << c << ".i; bi" << b
+                      << " = br" << b << " * F" << c << ".i + bi" << b << " * F" << c << ".r; br" << b << " = tr; }\n";
+            }
+        }
+    };
+    emit_core(o);
     for (int u = 0; u < NU; ++u) o << "    double y" << u << "r = 0.0, y" << u << "i = 0.0;\n";
     for (int a = 0; a < NA; ++a) {
         int b = M.amps[a].block, u = M.amps[a].u;
@@ -231,10 +503,13 @@ std::string generate_eval_kernel(const Model_& M) {
     // theta terms: R_d = Z_b Psi_b Psi_d
     std::vector<std::vector<int>> den_terms(M.dens.size());
     for (int t = 0; t < NTH; ++t) den_terms[M.thterms[t].den].push_back(t);
+    std::vector<std::vector<int>> custom_terms(M.customs.size());
+    for (int t = 0; t < NCT; ++t) custom_terms[M.cterms[t].custom].push_back(t);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const PBlock& B = P.blocks[b];
         bool any = false;
         for (int d : M.block_dens[b]) if (!den_terms[d].empty()) any = true;
+        for (int c : M.block_customs[b]) if (!custom_terms[c].empty()) any = true;
         if (B.nsl == 0 || !any) continue;
         o << "    {\n      double zr = 0.0, zi = 0.0;\n";
         for (int c = 0; c < B.nsl; ++c) {
@@ -255,6 +530,22 @@ std::string generate_eval_kernel(const Model_& M) {
             }
             o << "      }\n";
         }
+        // Custom factors: d Psi_b / d theta_j = (product of the block's other factors) F'_c,j
+        for (int c : M.block_customs[b]) {
+            if (custom_terms[c].empty()) continue;
+            o << "      {\n        double or_ = pdr" << b << ", oi_ = pdi" << b << ";\n";
+            for (int c2 : M.block_customs[b]) {
+                if (c2 == c) continue;
+                o << "        { const double tr = or_ * F" << c2 << ".r - oi_ * F" << c2 << ".i; oi_ = or_ * F" << c2
+                  << ".i + oi_ * F" << c2 << ".r; or_ = tr; }\n";
+            }
+            o << "        const double wr = zr * or_ - zi * oi_, wi = zr * oi_ + zi * or_;\n";
+            for (int t : custom_terms[c]) {
+                int li = M.cterms[t].local;
+                o << "        ct" << t << " += wr * F" << c << ".dr[" << li << "] - wi * F" << c << ".di[" << li << "];\n";
+            }
+            o << "      }\n";
+        }
         o << "    }\n";
     }
     o << (M.lds_pipe ? "    }\n  }\n" : "  }\n");
@@ -267,6 +558,7 @@ std::string generate_eval_kernel(const Model_& M) {
     red("av", 0);
     for (int a = 0; a < NA; ++a) { red("ax" + std::to_string(a) + "r", 1 + 2 * a); red("ax" + std::to_string(a) + "i", 2 + 2 * a); }
     for (int t = 0; t < NTH; ++t) red("at" + std::to_string(t), 1 + 2 * NA + t);
+    for (int t = 0; t < NCT; ++t) red("ct" + std::to_string(t), 1 + 2 * NA + NTH + t);
     o << "  __syncthreads();\n";
     // fixed-order combination of the wave partials
     std::string comb;
@@ -299,6 +591,55 @@ std::string generate_eval_kernel(const Model_& M) {
          "  const EvalJob J = jobs[lo];\n"
          "  eval_body(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
          "            (int)blockIdx.x - J.wg_begin, red, lbuf);\n}\n";
+    if (!M.customs.empty()) {
+        // per-event amplitudes (without couplings) or intensities of a range of events:
+        // models with Custom lineshapes evaluate them with the same generated core
+        o << "extern \"C\" __global__ void __launch_bounds__(256) ctpwa_aux(\n"
+             "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
+             "    const float* __restrict__ invf, long long ld, long long e0, long long n, double* __restrict__ out,\n"
+             "    int intensity)\n{\n";
+        o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n  const double* tb = tab;\n";
+        emit_coefs("  ");
+        o << "  const long long ie = (long long)blockIdx.x * 256 + threadIdx.x;\n  if (ie >= n) return;\n"
+             "  const long long e = e0 + ie;\n  {\n";
+        for (int u = 0; u < NU; ++u)
+            for (int p = 0; p < NP; ++p) {
+                std::string nm = std::to_string(u) + "_" + std::to_string(p);
+                o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+                o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
+            }
+        for (size_t i = 0; i < M.inv.size(); ++i)
+            if (used[i]) o << "    const double x" << i << " = " << inv_load(i, "e") << ";\n";
+        emit_core(o);
+        o << "    if (intensity) {\n";
+        for (int u = 0; u < NU; ++u) o << "      double y" << u << "r = 0.0, y" << u << "i = 0.0;\n";
+        for (int a = 0; a < NA; ++a) {
+            int b = M.amps[a].block, u = M.amps[a].u;
+            o << "      y" << u << "r += c" << a << "r * br" << b << " - c" << a << "i * bi" << b << "; y" << u
+              << "i += c" << a << "r * bi" << b << " + c" << a << "i * br" << b << ";\n";
+        }
+        o << "      double I = 0.0;\n";
+        for (int p = 0; p < NP; ++p) {
+            o << "      { double sr = 0.0, si = 0.0;\n";
+            for (int u = 0; u < NU; ++u)
+                o << "        sr += y" << u << "r * s" << u << "_" << p << ".x - y" << u << "i * s" << u << "_" << p
+                  << ".y; si += y" << u << "r * s" << u << "_" << p << ".y + y" << u << "i * s" << u << "_" << p << ".x;\n";
+            o << "        I += sr * sr + si * si; }\n";
+        }
+        o << "      out[ie] = I;\n    } else {\n";
+        for (int a = 0; a < NA; ++a) {
+            int b = M.amps[a].block, u = M.amps[a].u;
+            o << "      { const double K = tb[OFFR + " << (long long)M.amps[a].coef_base << "LL * REC];\n";
+            for (int p = 0; p < NP; ++p) {
+                o << "        { const double fr = K * br" << b << ", fi = K * bi" << b << ";\n"
+                  << "          const long long oo = ((ie * " << NP << " + " << p << ") * " << NA << " + " << a << ") * 2;\n"
+                  << "          out[oo] = fr * s" << u << "_" << p << ".x - fi * s" << u << "_" << p << ".y;\n"
+                  << "          out[oo + 1] = fr * s" << u << "_" << p << ".y + fi * s" << u << "_" << p << ".x; }\n";
+            }
+            o << "      }\n";
+        }
+        o << "    }\n  }\n}\n";
+    }
     return o.str();
 }
 

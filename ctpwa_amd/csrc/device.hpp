@@ -70,6 +70,7 @@ struct PrepArgs {
     const DAmp* amps;
     int nden, namp, nfree, ntheta;
     long long off_v, off_c, off_rec;
+    long long off_th;        // theta values (read by Custom lineshapes in the fused kernel)
     // the same tables packed in one blob (16-byte aligned sections) that the prep kernel
     // copies to LDS before use; meta_bytes == 0: read the tables from global memory
     const void* meta;

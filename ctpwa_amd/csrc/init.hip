@@ -380,6 +380,8 @@ k_init_events(const double* __restrict__ mom, long long N, int nfinal,
         }
         case 6: val = q * q; break;
         case 7: val = q * q * q / mm; break;
+        case 8: val = mm; break;                                           // m (Custom)
+        case 9: val = q; break;                                            // q (Custom)
         default: val = 1.0;
         }
         inv[(size_t)i * ld + eo] = val;

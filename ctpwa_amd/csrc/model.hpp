@@ -35,13 +35,19 @@
 
 namespace ctpwa {
 
-enum Model { M_BWR = 0, M_BW = 1, M_ONE = 2, M_FLATTE = 3, M_GS = 4 };
+enum Model { M_BWR = 0, M_BW = 1, M_ONE = 2, M_FLATTE = 3, M_GS = 4, M_CUSTOM = 5 };
 enum InvKind { INV_ONE = 0, INV_S = 1, INV_E = 2, INV_RHO_RE = 3, INV_RHO_IM = 4,
-               INV_GS_Q2H = 5, INV_GS_Q2 = 6, INV_GS_Q3M = 7 };
+               INV_GS_Q2H = 5, INV_GS_Q2 = 6, INV_GS_Q3M = 7, INV_M = 8, INV_Q = 9 };
 
 struct PNode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md1, md2; };
 struct PTopo { std::vector<PNode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
-struct PRes { int node, model; std::vector<double> params; std::vector<int> slots; std::vector<double> ch; };
+struct PRes {
+    int node, model;
+    std::vector<double> params;
+    std::vector<int> slots;
+    std::vector<double> ch;
+    std::vector<std::string> expr;   // Custom: expression tokens in prefix order
+};
 struct PBlock { int topo, site, nsl; std::vector<PRes> res; };
 
 struct Plan {
@@ -93,6 +99,13 @@ inline Plan parse_plan(const std::string& text) {
             r.slots.resize(np); for (auto& x : r.slots) ls >> x;
             ls >> nch; r.ch.resize(2 * nch); for (auto& x : r.ch) ls >> x;
             P.blocks.back().res.push_back(r);
+        } else if (key == "cexpr") {
+            if (P.blocks.empty() || P.blocks.back().res.empty()) throw std::runtime_error("cexpr before res");
+            int n; ls >> n;
+            if (n <= 0 || n > 100000) throw std::runtime_error("bad cexpr length");
+            std::vector<std::string> toks(n);
+            for (auto& t : toks) if (!(ls >> t)) throw std::runtime_error("cexpr is truncated");
+            P.blocks.back().res.back().expr = toks;
         } else if (key == "ntheta") ls >> P.ntheta;
         else if (key == "theta") { int j; double a, lo, hi; ls >> j >> a >> lo >> hi; P.th_init.push_back(a); P.th_lo.push_back(lo); P.th_hi.push_back(hi); }
         else if (key == "namp") ls >> P.namp >> P.nfree;
@@ -139,6 +152,18 @@ struct AmpSpec { int block, slc, u; int coef_base; };
 // sum_e y_t(e) Im R_d(e) (imaginary term) of a theta-dependent denominator coefficient
 struct ThTerm { int den, coef, im, inv; };
 
+// one Custom lineshape factor of a block: evaluated per event with forward-mode
+// derivatives over the block's local variables
+struct CustomSpec {
+    int block, res, node;
+    int inv_m, inv_q;                  // invariant rows of m and q at the node
+    std::vector<std::string> expr;
+    std::vector<int> param_slot;       // parameter k -> theta slot (-1: fixed value)
+    std::vector<double> param_val;
+};
+// one accumulator of the theta gradient through a Custom factor: sum_e Re(z F'_j other)
+struct CTerm { int custom, local; };
+
 struct Model_ {
     Plan P;
     std::vector<KinNode> kin;
@@ -150,11 +175,15 @@ struct Model_ {
     std::vector<AmpSpec> amps;                    // [namp]
     std::vector<std::vector<int>> bvars;          // [block] -> local var -> global slot
     std::vector<ThTerm> thterms;                  // theta-gradient accumulators
+    std::vector<CustomSpec> customs;
+    std::vector<std::vector<int>> block_customs;  // [block] -> custom indices
+    std::vector<CTerm> cterms;                    // Custom theta-gradient accumulators
     int maxloc = 0;
     // code-generation options of the fused kernel
     int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
     int wgs = 256;           // threads (= events per tile) of a workgroup: 64, 128 or 256
     bool lds_pipe = false;   // copy the next tile's inputs HBM -> LDS while computing the current one
+    bool prefetch_sl = false; // load the next event's SL values one iteration ahead (registers)
     // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
     // invariants, FP64 squared masses)
     int mixed = 0;
@@ -177,8 +206,9 @@ struct Model_ {
     size_t off_v() const { return 0; }
     size_t off_c() const { return 2 * (size_t)P.namp; }
     size_t off_rec() const { return 4 * (size_t)P.namp; }
-    size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC; }
-    int nacc() const { return 1 + 2 * P.namp + (int)thterms.size(); }
+    size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC + (size_t)P.ntheta; }
+    int nacc() const { return 1 + 2 * P.namp + (int)thterms.size() + (int)cterms.size(); }
+    size_t off_th() const { return off_rec() + (size_t)n_coef_rec * REC; }   // theta values (for Custom)
 };
 
 // final-state subset of a momentum reference
@@ -247,6 +277,7 @@ inline Model_ derive(const Plan& P) {
     std::map<std::tuple<int, int, uint32_t>, int> u_idx;
     int rec = 0;
     M.block_dens.resize(P.blocks.size());
+    M.block_customs.resize(P.blocks.size());
     M.amps.resize(P.namp);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const PBlock& B = P.blocks[b];
@@ -273,7 +304,7 @@ inline Model_ derive(const Plan& P) {
             if (k == 0 || res_at[k] < 0) fold = T.nodes[k].has_bf;
             else {
                 int mdl = B.res[res_at[k]].model;
-                fold = T.nodes[k].has_bf && mdl != M_BW;
+                fold = T.nodes[k].has_bf && mdl != M_BW && mdl != M_CUSTOM;
             }
             if (fold) mask |= 1u << k;
         }
@@ -288,6 +319,19 @@ inline Model_ derive(const Plan& P) {
             auto ONE = add_inv({kin, INV_ONE, 0, 0.0, 0.0, 0.0});
             switch (R.model) {
             case M_ONE: continue;
+            case M_CUSTOM: {
+                CustomSpec C;
+                C.block = (int)b; C.res = (int)r; C.node = k;
+                C.inv_m = add_inv({kin, INV_M, 0, 0.0, 0.0, 0.0});
+                C.inv_q = add_inv({kin, INV_Q, 0, 0.0, 0.0, 0.0});
+                C.expr = R.expr;
+                if (C.expr.empty()) throw std::runtime_error("Custom lineshape without expression");
+                C.param_slot = R.slots;
+                C.param_val = R.params;
+                M.block_customs[b].push_back((int)M.customs.size());
+                M.customs.push_back(C);
+                continue;
+            }
             case M_BWR: {
                 if (R.params.size() < 2) throw std::runtime_error("BWR needs mass and width");
                 int E = add_inv({kin, INV_E, n.lmin, n.has_bf ? n.bf_d : 0.0, 0.0, 0.0});
@@ -355,6 +399,14 @@ inline Model_ derive(const Plan& P) {
         if (M.bvars[Dn.block].empty()) continue;
         for (const Term& t : Dn.re) if (t.coef >= 0) M.thterms.push_back(ThTerm{(int)d, t.coef, 0, t.inv});
         for (const Term& t : Dn.im) if (t.coef >= 0) M.thterms.push_back(ThTerm{(int)d, t.coef, 1, t.inv});
+    }
+    for (size_t c = 0; c < M.customs.size(); ++c) {
+        const std::vector<int>& vars = M.bvars[M.customs[c].block];
+        for (size_t li = 0; li < vars.size(); ++li) {
+            bool uses = false;
+            for (int s : M.customs[c].param_slot) if (s == vars[li]) uses = true;
+            if (uses) M.cterms.push_back(CTerm{(int)c, (int)li});
+        }
     }
     // amplitude records follow the denominator records
     M.amp_rec0 = rec;

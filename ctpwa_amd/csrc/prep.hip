@@ -219,6 +219,8 @@ __device__ __forceinline__ void prep_body(const double* __restrict__ params, dou
     }
     int item = blockIdx.x * blockDim.x + threadIdx.x;
     const double* theta = params + 2 * a.nfree;
+    if (blockIdx.x == 0)
+        for (int j = threadIdx.x; j < a.ntheta; j += blockDim.x) table[a.off_th + j] = theta[j];
     if (item < a.nden) {
         const DDen& D = a.dens[item];
         const DBlock& B = a.blocks[D.block];
@@ -292,7 +294,7 @@ __device__ __forceinline__ void prep_body(const double* __restrict__ params, dou
             continue;
         }
         const DRes& R = a.res[B.res0 + r];
-        if (R.model == 1) continue;                        // BW carries no barrier
+        if (R.model == 1 || R.model == 5) continue;        // BW and Custom carry no automatic barrier
         J m0 = (R.np > 0) ? res_param<NV, H>(R, 0, B, theta) : jc<NV, H>(1.0);
         J q0 = jq0ad(m0, m1, m2);
         if (n.has_bf && L >= 1 && L <= 6) K = jmul(K, jsqrt(jbfpoly(L, q0, n.bf_d)));
@@ -413,7 +415,7 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
         double g = 0.0;
         for (int k = f.thp[j] + lane; k < f.thp[j + 1]; k += 64) {
             DGEnt e = f.the[k];
-            double d = f.table[f.off_rec + e.rec];
+            double d = (e.rec < 0) ? 1.0 : f.table[f.off_rec + e.rec];
             if (e.idx >= 0) {
                 g += e.scale * d * C[e.idx];
             } else {

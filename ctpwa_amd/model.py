@@ -28,7 +28,7 @@ from typing import Dict, List, Optional, Tuple
 from .config import ConfigError, ConfigParser, Particle, ResonanceConfig
 
 # lineshape model codes shared with the native library
-MODEL_CODES = {"BWR": 0, "BW": 1, "ONE": 2, "Flatte": 3, "GS": 4}
+MODEL_CODES = {"BWR": 0, "BW": 1, "ONE": 2, "Flatte": 3, "GS": 4, "Custom": 5}
 # ordered parameter names per model (src/Resonance.cu:14-75)
 _MODEL_PARAM_SPECS = {
     "BWR": [("mass", False), ("width", False), ("r", True)],
@@ -36,6 +36,7 @@ _MODEL_PARAM_SPECS = {
     "ONE": [("mass", False)],
     "Flatte": [("mass", False)],
     "GS": [("mass", False), ("width", False)],
+    "Custom": [],
 }
 _EXTRA_PREFIX = {"Flatte": "g"}
 
@@ -125,6 +126,7 @@ class ResInst:
     channels: List[Tuple[float, float]]
     node: int                # node of the topology whose mother is ``tag``
     slots: List[int] = field(default_factory=list)   # theta slot per param or -1
+    expr: List[str] = field(default_factory=list)    # Custom: serialized expression
 
 
 @dataclass
@@ -197,6 +199,8 @@ class Plan:
                 for ch in r.channels:
                     toks += [repr(float(ch[0])), repr(float(ch[1]))]
                 out.append(" ".join(toks))
+                if r.model == "Custom":
+                    out.append("cexpr %d %s" % (len(r.expr), " ".join(r.expr)))
         out.append("ntheta %d" % len(self.theta))
         for j, th in enumerate(self.theta):
             out.append("theta %d %r %r %r" % (j, th.init, th.lower, th.upper))
@@ -449,7 +453,15 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
                 range_lists: List[List[List[float]]] = []
                 for key, rconf in zip(comb, rcombo):
                     model = canonical_model(rconf.type)
-                    vals, names = ordered_params(model, rconf.parameters)
+                    if model == "Custom":
+                        vals = [float(x) for x in rconf.parameters]
+                        pn = rconf.options.get("params")
+                        names = pn.split(",") if pn else ["p%d" % i for i in range(len(vals))]
+                        if len(names) != len(vals):
+                            raise ConfigError("Custom %s: %d paramNames for %d paramValues"
+                                              % (rconf.name, len(names), len(vals)))
+                    else:
+                        vals, names = ordered_params(model, rconf.parameters)
                     # Constraints.fix_var overrides values (src/Analysis.cu:4632-4650)
                     for pi, pn in enumerate(names):
                         nm = rconf.name + "_" + pn
@@ -462,8 +474,18 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
                     if model == "Flatte" and len(rconf.channels) < len(vals) - 1:
                         raise ConfigError("Flatte %s needs one channel per coupling" % rconf.name)
                     chans = [(float(c[0]), float(c[1])) for c in rconf.channels]
-                    insts.append(ResInst(rconf.name, key[0], model, vals, names, chans, node_k,
-                                         [-1] * len(vals)))
+                    inst = ResInst(rconf.name, key[0], model, vals, names, chans, node_k, [-1] * len(vals))
+                    if model == "Custom":
+                        from . import customexpr
+                        text = rconf.options.get("expression", rconf.options.get("expr"))
+                        if not text:
+                            raise ConfigError("Custom %s has no expression" % rconf.name)
+                        nd = nodes[node_k]
+                        ctx = customexpr.Context(names, [c[node_k][1] for c in sl_combos], nd.bf_d,
+                                                 nd.masses[1] if nd.masses[1] > 0 else None,
+                                                 nd.masses[2] if nd.masses[2] > 0 else None)
+                        inst.expr = customexpr.serialize(customexpr.parse(text, ctx))
+                    insts.append(inst)
                     if rconf.free:
                         if len(rconf.free) == 1 and rconf.free[0] == -1:
                             fidx = list(range(len(vals)))

@@ -472,7 +472,8 @@ static CJ gs_jet(double m, const Jet& m0, const Jet& g0, double q, const Jet& q0
 // ===========================================================================
 struct ONode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md1, md2; };
 struct OTopo { std::vector<ONode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
-struct ORes { int node, model; std::vector<double> params; std::vector<int> slots; std::vector<double> ch; };
+struct ORes { int node, model; std::vector<double> params; std::vector<int> slots; std::vector<double> ch;
+              std::vector<std::string> expr; };
 struct OBlock { int topo, site, nsl; std::vector<ORes> res; };
 struct OPlan {
     int nfinal = 0, npol = 0;
@@ -520,6 +521,11 @@ static OPlan parse_plan(const char* text) {
             r.slots.resize(np); for (auto& x : r.slots) ls >> x;
             int nch; ls >> nch; r.ch.resize(2 * nch); for (auto& x : r.ch) ls >> x;
             P.blocks.back().res.push_back(r);
+        } else if (key == "cexpr") {
+            int n; ls >> n;
+            std::vector<std::string> toks(n);
+            for (auto& t : toks) ls >> t;
+            P.blocks.back().res.back().expr = toks;
         } else if (key == "theta") { int j; double a, lo, hi; ls >> j >> a >> lo >> hi; P.th_init.push_back(a); }
         else if (key == "ntheta") ls >> P.ntheta;
         else if (key == "namp") ls >> P.namp >> P.nfree;
@@ -673,6 +679,111 @@ static Jet res_param(const ORes& r, int p, const std::vector<int>& vars, const s
     return Jet(theta[s]);
 }
 
+// ---------------------------------------------------------------------------
+// Custom lineshape expressions (src/CustomExpr.cu:95-228 semantics), evaluated on
+// complex jets from the plan's prefix token list
+// ---------------------------------------------------------------------------
+// real function of two jets, with its first and second partial derivatives
+static Jet jfun2(const Jet& x, const Jet& y, double f, double fx, double fy, double fxx, double fyy, double fxy) {
+    Jet r(f);
+    for (int i = 0; i < g_nv; i++) {
+        r.g[i] = fx * x.g[i] + fy * y.g[i];
+        for (int k = 0; k < g_nv; k++)
+            r.h[i][k] = fx * x.h[i][k] + fy * y.h[i][k] + fxx * x.g[i] * x.g[k] + fyy * y.g[i] * y.g[k]
+                        + fxy * (x.g[i] * y.g[k] + y.g[i] * x.g[k]);
+    }
+    return r;
+}
+static Jet jexp(const Jet& a) { double e = std::exp(a.v); return jfun(a, e, e, e); }
+static Jet jsin(const Jet& a) { return jfun(a, std::sin(a.v), std::cos(a.v), -std::sin(a.v)); }
+static Jet jcos(const Jet& a) { return jfun(a, std::cos(a.v), -std::sin(a.v), -std::cos(a.v)); }
+static Jet jatan2(const Jet& y, const Jet& x) {
+    double r2 = x.v * x.v + y.v * y.v, r4 = r2 * r2;
+    return jfun2(x, y, std::atan2(y.v, x.v), -y.v / r2, x.v / r2, 2 * x.v * y.v / r4, -2 * x.v * y.v / r4,
+                 (y.v * y.v - x.v * x.v) / r4);
+}
+static bool jzero(const Jet& a) {
+    if (a.v != 0.0) return false;
+    for (int i = 0; i < g_nv; i++) if (a.g[i] != 0.0) return false;
+    return true;
+}
+static CJ cadd(const CJ& a, const CJ& b) { return CJ{a.re + b.re, a.im + b.im}; }
+static CJ csub(const CJ& a, const CJ& b) { return CJ{a.re - b.re, a.im - b.im}; }
+static CJ cdiv(const CJ& a, const CJ& b) {
+    Jet d = b.re * b.re + b.im * b.im;
+    return CJ{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+static CJ cexpj(const CJ& a) { Jet e = jexp(a.re); return CJ{e * jcos(a.im), e * jsin(a.im)}; }
+static CJ clogj(const CJ& a) { return CJ{0.5 * jlog(a.re * a.re + a.im * a.im), jatan2(a.im, a.re)}; }
+static CJ csqrtj(const CJ& a) {
+    // principal square root; a real argument keeps a real (or purely imaginary) root
+    if (jzero(a.im)) {
+        if (a.re.v >= 0) return CJ{jsqrt(a.re), Jet(0.0)};
+        return CJ{Jet(0.0), jsqrt(Jet(0.0) - a.re)};
+    }
+    Jet r = jsqrt(a.re * a.re + a.im * a.im);
+    Jet re = jsqrt((r + a.re) * 0.5), im = jsqrt((r - a.re) * 0.5);
+    if (a.im.v < 0) im = Jet(0.0) - im;
+    return CJ{re, im};
+}
+
+struct CExpr {
+    const std::vector<std::string>& t;
+    size_t i = 0;
+    const ORes& R;
+    const std::vector<int>& vars;
+    const std::vector<double>& theta;
+    double m, q;
+    CJ eval() {
+        const std::string& k = t.at(i++);
+        auto real = [&](const Jet& x) { return CJ{x, Jet(0.0)}; };
+        if (k.rfind("n:", 0) == 0) return real(Jet(std::stod(k.substr(2))));
+        if (k == "I") return CJ{Jet(0.0), Jet(1.0)};
+        if (k == "v:m") return real(Jet(m));
+        if (k == "v:q") return real(Jet(q));
+        if (k.rfind("p:", 0) == 0) return real(res_param(R, std::stoi(k.substr(2)), vars, theta));
+        if (k == "+") { CJ a = eval(), b = eval(); return cadd(a, b); }
+        if (k == "-") { CJ a = eval(), b = eval(); return csub(a, b); }
+        if (k == "*") { CJ a = eval(), b = eval(); return cmul(a, b); }
+        if (k == "/") { CJ a = eval(), b = eval(); return cdiv(a, b); }
+        if (k == "^") { CJ a = eval(), b = eval(); return cexpj(cmul(b, clogj(a))); }
+        if (k == "neg") { CJ a = eval(); return CJ{Jet(0.0) - a.re, Jet(0.0) - a.im}; }
+        if (k.rfind("f:", 0) == 0) {
+            std::string f = k.substr(2);
+            CJ a = eval();
+            if (f == "exp") return cexpj(a);
+            if (f == "log") return clogj(a);
+            if (f == "sin") {   // sin(x + iy) = sin x cosh y + i cos x sinh y
+                Jet ep = jexp(a.im), em = jexp(Jet(0.0) - a.im);
+                return CJ{jsin(a.re) * (ep + em) * 0.5, jcos(a.re) * (ep - em) * 0.5};
+            }
+            if (f == "cos") {
+                Jet ep = jexp(a.im), em = jexp(Jet(0.0) - a.im);
+                return CJ{jcos(a.re) * (ep + em) * 0.5, Jet(0.0) - jsin(a.re) * (ep - em) * 0.5};
+            }
+            if (f == "sqrt") return csqrtj(a);
+            if (f == "abs") return real(jsqrt(a.re * a.re + a.im * a.im));
+            if (f == "re") return real(a.re);
+            if (f == "im") return real(a.im);
+            if (f == "conj") return CJ{a.re, Jet(0.0) - a.im};
+            throw std::runtime_error("custom expression: unknown function " + f);
+        }
+        if (k == "q0") { CJ x = eval(), a = eval(), b = eval(); return real(q0ad_jet(x.re, a.re, b.re)); }
+        if (k.rfind("bf:", 0) == 0) {
+            int L = std::stoi(k.substr(3));
+            CJ qq = eval(), qz = eval(), d = eval();
+            return real(bf_jet(L, qq.re, qz.re, d.re.v));
+        }
+        if (k == "bw") { CJ mm = eval(), m0 = eval(), g0 = eval(); return bw_jet(mm.re, m0.re, g0.re); }
+        if (k.rfind("bwr:", 0) == 0) {
+            int L = std::stoi(k.substr(4));
+            CJ mm = eval(), m0 = eval(), g0 = eval(), qq = eval(), qz = eval(), d = eval();
+            return bwr_jet(mm.re, m0.re, g0.re, L, qq.re, qz.re, d.re.v);
+        }
+        throw std::runtime_error("custom expression: unknown token " + k);
+    }
+};
+
 static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin,
                        const std::vector<int>& vars, const std::vector<double>& theta) {
     const OTopo& T = P.topos[B.topo];
@@ -740,6 +851,12 @@ static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin
             double mpi = R.ch.empty() ? 0.1396 : R.ch[0];
             f = gs_jet(mm, m0, g0, qq, q0, mpi);
             if (n.has_bf) f = cscale(f, bf_jet(L, q, q0, n.bf_d));
+            break;
+        }
+        case 5: {   // Custom expression: no vertex barrier besides what it contains
+            CExpr ce{R.expr, 0, R, vars, theta, mm, qq};
+            f = ce.eval();
+            if (ce.i != R.expr.size()) throw std::runtime_error("custom expression: trailing tokens");
             break;
         }
         default: throw std::runtime_error("model not supported by the oracle");

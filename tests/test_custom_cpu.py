@@ -1,0 +1,104 @@
+"""Custom lineshape expressions: parser, and the oracle's evaluation pinned by
+equivalences with built-in lineshapes and by finite differences (CPU)."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import plan_of, samples, perturbed, rel
+from ctpwa_amd import customexpr as ce
+from ctpwa_amd import workloads as W
+from ctpwa_amd.config import ConfigError
+
+RHO_BWR = "  rho770:  {J: 1, P: -1, model: BWR, parameters: [0.775, 0.149], free: [0, 1]}"
+
+
+def _custom(expr, names="[p0, p1]"):
+    return W.JPSI_3PI.replace(
+        RHO_BWR, '  rho770:  {J: 1, P: -1, model: Custom, paramValues: [0.775, 0.149], paramNames: %s, '
+                 'free: [0, 1], expression: "%s"}' % (names, expr))
+
+
+def _ctx():
+    return ce.Context(["a", "b"], [1], 3.0, 0.14, 0.14)
+
+
+@pytest.mark.parametrize("text,expected", [
+    ("1 + 2 * 3", ["+", "n:1.0", "*", "n:2.0", "n:3.0"]),
+    ("2 ^ 3 ^ 2", ["^", "n:2.0", "^", "n:3.0", "n:2.0"]),          # right associative
+    ("-a * b", ["*", "neg", "p:0", "p:1"]),
+    ("(1 - a) / b - 1j", ["-", "/", "-", "n:1.0", "p:0", "p:1", "I"]),
+    ("exp(i*m) + Re(q)", ["+", "f:exp", "*", "I", "v:m", "f:re", "v:q"]),
+    ("L * d", ["*", "n:1.0", "n:3.0"]),
+    ("q0", ["q0", "p:0", "n:0.14", "n:0.14"]),
+])
+def test_parser(text, expected):
+    assert ce.serialize(ce.parse(text, _ctx())) == expected
+
+
+@pytest.mark.parametrize("text", ["a +", "foo(1)", "p1p * 2", "Flatte(a, b, 1, 2)", "BWR(a, b, c)", "(a", "a $ b"])
+def test_parser_rejects(text):
+    with pytest.raises(ConfigError):
+        ce.parse(text, _ctx())
+
+
+def test_L_must_be_unique_at_the_node():
+    with pytest.raises(ConfigError):
+        ce.parse("L", ce.Context(["a"], [0, 2], 3.0, 0.14, 0.14))
+
+
+def _oracles(oracle_lib, texts, seed=3):
+    ph, da, bk = samples(5000, 1200, 300, seed)
+    wb = np.full(bk.shape[0], 0.8)
+    out = []
+    for t in texts:
+        plan = plan_of(t)
+        out.append((plan, oracle_lib.Oracle(plan.to_text(), 3, ph, da, bk, w_bkg=wb)))
+    return out
+
+
+def test_custom_bwr_equals_builtin(oracle_lib):
+    """BWR(p0, p1, 1, 3.0) * Bf(1, q, q0(p0), 3.0) is the built-in BWR with its vertex
+    barrier (rho -> pi pi has L = 1): same NLL, gradient and Hessian."""
+    (pc, oc), (pb, ob) = _oracles(oracle_lib, [_custom("BWR(p0, p1, 1, 3.0) * Bf(1, q, q0(p0), 3.0)"),
+                                              W.JPSI_3PI])
+    p = perturbed(W.nominal_params(pb, 1), pb, 2)
+    nc, gc, Hc = oc.hessian(p)
+    nb, gb, Hb = ob.hessian(p)
+    assert abs(nc - nb) <= 1e-12 * abs(nb)
+    assert rel(gc, gb) <= 1e-10
+    assert rel(Hc, Hb) <= 1e-9
+
+
+def test_custom_bw_is_scaled_builtin(oracle_lib):
+    """p0^2/(p0^2 - m^2 - i*p0*p1) = p0^2 x BW(p0, p1): the rho(770) amplitudes differ by
+    the factor p0^2, the others not at all."""
+    bw = W.JPSI_3PI.replace(RHO_BWR, "  rho770:  {J: 1, P: -1, model: BW, parameters: [0.775, 0.149], free: [0, 1]}")
+    (pc, oc), (pb, ob) = _oracles(oracle_lib, [_custom("p0^2/(p0^2 - m^2 - i*p0*p1)"), bw])
+    p = perturbed(W.nominal_params(pb, 1), pb, 5)
+    Ac, Ab = oc.amplitudes(p, 1), ob.amplitudes(p, 1)
+    m0 = p[2 * pb.n_free + [t.name for t in pb.theta].index("rho770_mass")]
+    rho = [a for a, n in enumerate(pb.amp_names) if "rho770" in n]
+    other = [a for a in range(pb.n_amp) if a not in rho]
+    assert np.allclose(Ac[:, :, rho], m0 ** 2 * Ab[:, :, rho], rtol=1e-12, atol=0)
+    assert np.array_equal(Ac[:, :, other], Ab[:, :, other])
+
+
+def test_custom_functions_finite_differences(oracle_lib):
+    """Every operator and function of the language in one expression; the oracle's
+    gradient and Hessian against central differences of its NLL and gradient."""
+    expr = ("exp(-p1*m) * (cos(p0*m) + i*sin(p1)) / sqrt(p0^2 + m^2) * abs(p0 - i*p1) "
+            "+ log(p0) * conj(m + i*p1)^2 - Im(p0 * (q + 1j)) + Re(csqrt(p1 - q0))")
+    ((pc, oc),) = _oracles(oracle_lib, [_custom(expr)])
+    p = perturbed(W.nominal_params(pc, 1), pc, 7)
+    _, g, H = oc.hessian(p)
+    j0 = 2 * pc.n_free + [t.name for t in pc.theta].index("rho770_p0")
+    for j in (j0, j0 + 1):
+        h = 1e-6 * abs(p[j])
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        fp, gp = oc.nll_grad(pp)
+        fm, gm = oc.nll_grad(pm)
+        assert abs((fp - fm) / (2 * h) - g[j]) <= 1e-6 * max(abs(g[j]), 1.0)
+        assert np.max(np.abs((gp - gm) / (2 * h) - H[j])) <= 1e-5 * np.max(np.abs(H[j]))

@@ -1,0 +1,78 @@
+"""Custom lineshapes (hipRTC-compiled expressions with forward-mode derivatives) on the
+GPU against the oracle's interpreter: NLL, gradient, amplitudes and intensities, in both
+precision forms; the Custom spelling of the built-in BWR gives the built-in's NLL."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import perturbed, plan_of, rel, samples
+from test_custom_cpu import _custom
+from ctpwa_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+EXPRS = [
+    "BWR(p0, p1, 1, 3.0) * Bf(1, q, q0(p0), 3.0)",
+    "p0^2/(p0^2 - m^2 - i*p0*p1)",
+    ("exp(-p1*m) * (cos(p0*m) + i*sin(p1)) / sqrt(p0^2 + m^2) * abs(p0 - i*p1) "
+     "+ log(p0) * conj(m + i*p1)^2 - Im(p0 * (q + 1j)) + Re(csqrt(p1 - q0))"),
+]
+
+
+def _run(h, p):
+    t = torch.tensor(p, dtype=torch.float64, device="cuda")
+    out = torch.empty(1 + h.n_params, dtype=torch.float64, device="cuda")
+    h.nll_grad(t.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    o = out.cpu().numpy()
+    return o[0], o[1:]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("k", range(len(EXPRS)))
+def test_custom_parity(k, precision, oracle_lib):
+    from ctpwa_amd import _native
+    plan = plan_of(_custom(EXPRS[k]))
+    ph, da, bk = samples(20000, 4000, 1000, 61 + k)
+    wb = np.full(bk.shape[0], 0.8)
+    print("case custom", k, precision, flush=True)
+    h = _native.Handle(plan.to_text(), ph, da, bk, None, None, wb, precision=precision)
+    O = oracle_lib.Oracle(plan.to_text(), 3, ph, da, bk, w_bkg=wb)
+    p = perturbed(W.nominal_params(plan, 1), plan, 30 + k)
+    nll_o, g_o = O.nll_grad(p)
+    nll_g, g_g = _run(h, p)
+    tol_n, tol_g, tol_a = (1e-9, 1e-7, 1e-10) if precision == "fp64" else (1e-9, 1e-5, 1e-6)
+    print(nll_o, nll_g, rel(g_g, g_o), flush=True)
+    assert abs(nll_g - nll_o) <= tol_n * abs(nll_o)
+    assert rel(g_g, g_o) <= tol_g
+    t = torch.tensor(p, dtype=torch.float64, device="cuda")
+    n = h.n_events[1]
+    A = torch.empty((n, plan.npol, plan.n_amp, 2), dtype=torch.float64, device="cuda")
+    I = torch.empty(n, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    h.amplitudes(t.data_ptr(), 1, A.data_ptr(), s)
+    h.intensity(t.data_ptr(), 1, I.data_ptr(), s)
+    torch.cuda.synchronize()
+    A = A.cpu().numpy()
+    A = A[..., 0] + 1j * A[..., 1]
+    Ao = O.amplitudes(p, 1)
+    assert np.max(np.abs(A - Ao)) <= tol_a * np.max(np.abs(Ao))
+    assert rel(I.cpu().numpy(), O.intensity(p, 1)) <= 10 * tol_a
+    if precision == "fp64":
+        H = torch.empty((h.n_params, h.n_params), dtype=torch.float64, device="cuda")
+        with pytest.raises(_native.NativeError):
+            h.hessian(t.data_ptr(), H.data_ptr(), s)
+
+
+def test_custom_bwr_matches_builtin_on_gpu():
+    from ctpwa_amd import _native
+    ph, da, bk = samples(20000, 4000, 1000, 71)
+    wb = np.full(bk.shape[0], 0.8)
+    pc, pb = plan_of(_custom(EXPRS[0])), plan_of(W.JPSI_3PI)
+    hc = _native.Handle(pc.to_text(), ph, da, bk, None, None, wb)
+    hb = _native.Handle(pb.to_text(), ph, da, bk, None, None, wb)
+    p = perturbed(W.nominal_params(pb, 1), pb, 33)
+    nc, gc = _run(hc, p)
+    nb, gb = _run(hb, p)
+    assert abs(nc - nb) <= 1e-11 * abs(nb)
+    assert rel(gc, gb) <= 1e-9
