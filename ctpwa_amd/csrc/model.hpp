@@ -185,14 +185,16 @@ struct Model_ {
     bool lds_pipe = false;   // copy the next tile's inputs HBM -> LDS while computing the current one
     bool prefetch_sl = false; // load the next event's SL values one iteration ahead (registers)
     // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
-    // invariants, FP64 squared masses)
+    // invariants, FP64 squared masses and Custom-expression m, q)
     int mixed = 0;
     // storage row of invariant i: >= 0 row of the FP64 store, < 0: -(row of the FP32 store) - 1
     std::vector<int> imap() const {
         std::vector<int> m(inv.size());
         int nd = 0, nf = 0;
         for (size_t i = 0; i < inv.size(); ++i) {
-            if (!mixed || inv[i].kind == INV_S) m[i] = nd++;
+            // squared masses, and the m and q of Custom expressions (arbitrary formulas
+            // that may cancel like m0^2 - m^2), stay FP64
+            if (!mixed || inv[i].kind == INV_S || inv[i].kind == INV_M || inv[i].kind == INV_Q) m[i] = nd++;
             else m[i] = -(nf++) - 1;
         }
         return m;

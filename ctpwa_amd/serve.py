@@ -1,25 +1,27 @@
 """Resident fit service with continuous batching of NLL+gradient requests.
 
 Many fits of one model (multi-start minimisations, toy-MC ensembles with uneven event
-counts) share the GPU.  Every fit is a minimiser written as a generator: it yields the
-next parameter point and receives (NLL, gradient), so the service drives all active fits
-in lock step without threads:
+counts) share the GPU.  All active fits advance in lock step:
 
     round:  gather the pending point of every active fit
             -> one batched evaluation (ctpwa_batch_nll_grad: batched prep, one grouped
                per-event launch whose workgroups are shared out among the fits, batched
-               reduction)
-            -> hand each fit its result; finished fits retire and free their slot and HBM
-    between rounds: admit arrived fits while a slot is free and their device footprint
-    (ctpwa_footprint_bytes) fits the HBM budget.
+               reduction) on the service's stream
+            -> one step of the minimiser for all fits at once (BatchLBFGS, numpy over
+               the fits' rows); finished fits retire and free their slot and HBM
+    meanwhile a worker thread admits arrived fits in order: it waits for HBM
+    (ctpwa_footprint_bytes against the budget) and builds each fit's handle (upload,
+    SL tensors) while the rounds go on; ready fits take free slots between rounds.
 
 Each fit's events stay resident on the device for its lifetime.  A fit's results are
 bit-identical whether it runs alone or in a batch (the batched evaluation reproduces the
-single one exactly), so a fit's trajectory does not depend on what else is running.
+single one exactly, and the minimiser reduces every fit's row on its own), so a fit's
+trajectory does not depend on what else is running.
 
 The reference has no service; its fit driver (example/fit.py:84-260) runs one fit per
 process with torch's L-BFGS.  The minimiser here is a bounded L-BFGS (two-loop recursion,
-projected gradient, Armijo backtracking) kept on the host in float64 numpy.
+projected gradient, Armijo backtracking) kept on the host in float64 numpy; ``lbfgs`` is
+the same algorithm as a generator for a single fit.
 """
 from __future__ import annotations
 
@@ -143,6 +145,184 @@ def lbfgs(x0: np.ndarray, lower: np.ndarray, upper: np.ndarray, m: int = 10, max
 
 
 # ---------------------------------------------------------------------------
+# the same minimiser for many fits at once (what the service runs)
+# ---------------------------------------------------------------------------
+def _rowdot(a, b):
+    """Row-wise dot products; each row is reduced on its own, so a fit's numbers do not
+    depend on which other fits share the arrays."""
+    return np.add.reduce(a * b, axis=-1)
+
+
+class BatchLBFGS:
+    """Bounded L-BFGS (the algorithm of :func:`lbfgs`) advanced for many fits per call,
+    with numpy operations over the fits' rows.  ``start`` places a fit in a slot;
+    ``step(slots, f, g)`` consumes the results of the slots' pending points and returns
+    the slots that finished (with their Outcome); ``xt[slot]`` is each remaining slot's
+    next point."""
+
+    def __init__(self, cap: int, n: int, m: int = 10, pgtol: float = 1e-5, ftol: float = 1e-11):
+        self.cap, self.n, self.m, self.pgtol, self.ftol = cap, n, m, pgtol, ftol
+        z = lambda *sh: np.zeros(sh)
+        self.x, self.g, self.xt, self.d = z(cap, n), z(cap, n), z(cap, n), z(cap, n)
+        self.lo, self.hi = z(cap, n), z(cap, n)
+        self.f, self.t = z(cap), z(cap)
+        self.S, self.Y = z(cap, m, n), z(cap, m, n)
+        self.hl = np.zeros(cap, dtype=np.int64)
+        self.phase = np.zeros(cap, dtype=np.int64)        # 0: first point pending, 1: trial pending
+        self.n_evals = np.zeros(cap, dtype=np.int64)
+        self.n_iter = np.zeros(cap, dtype=np.int64)
+        self.max_evals = np.zeros(cap, dtype=np.int64)
+        self.restarted = np.zeros(cap, dtype=bool)
+
+    def start(self, slot: int, x0, lower, upper, max_evals: int = 400) -> np.ndarray:
+        self.lo[slot], self.hi[slot] = lower, upper
+        self.xt[slot] = np.clip(np.asarray(x0, dtype=np.float64), lower, upper)
+        self.phase[slot] = 0
+        self.hl[slot] = 0
+        self.n_evals[slot] = 0
+        self.n_iter[slot] = 0
+        self.max_evals[slot] = max_evals
+        self.restarted[slot] = False
+        return self.xt[slot]
+
+    def _projected(self, idx):
+        x, g, lo, hi = self.x[idx], self.g[idx], self.lo[idx], self.hi[idx]
+        pg = np.where(lo < hi, g, 0.0)
+        pg = np.where((x <= lo) & (pg > 0), 0.0, pg)
+        pg = np.where((x >= hi) & (pg < 0), 0.0, pg)
+        return pg
+
+    def _directions(self, idx, finished):
+        """New search directions for the slots idx (x, f, g current); slots that converge
+        are added to ``finished``."""
+        if len(idx) == 0:
+            return
+        pg = self._projected(idx)
+        scale = np.maximum(1.0, np.abs(self.f[idx]))
+        conv = np.max(np.abs(pg), axis=1) <= self.pgtol * scale
+        for k in np.nonzero(conv)[0]:
+            finished[int(idx[k])] = True
+        keep = ~conv
+        idx, pg = idx[keep], pg[keep]
+        if len(idx) == 0:
+            return
+        act = pg == 0.0
+        q = pg.copy()
+        hl = self.hl[idx]
+        alpha = np.zeros((len(idx), self.m))
+        for k in range(self.m - 1, -1, -1):
+            valid = k < hl
+            s_ = np.where(act, 0.0, self.S[idx, k])
+            y_ = np.where(act, 0.0, self.Y[idx, k])
+            sy = _rowdot(y_, s_)
+            use = valid & (sy > 0)
+            a = np.where(use, _rowdot(s_, q) / np.where(use, sy, 1.0), 0.0)
+            alpha[:, k] = a
+            q = q - a[:, None] * y_
+        last = np.maximum(hl - 1, 0)
+        s_ = np.where(act, 0.0, self.S[idx, last])
+        y_ = np.where(act, 0.0, self.Y[idx, last])
+        sy, yy = _rowdot(s_, y_), _rowdot(y_, y_)
+        gamma = np.where((hl > 0) & (yy > 0) & (sy > 0), sy / np.where(yy > 0, yy, 1.0), 1.0)
+        first = 1.0 / np.maximum(1.0, np.sqrt(_rowdot(pg, pg)))
+        q = q * np.where(hl > 0, gamma, first)[:, None]
+        for k in range(self.m):
+            valid = k < hl
+            s_ = np.where(act, 0.0, self.S[idx, k])
+            y_ = np.where(act, 0.0, self.Y[idx, k])
+            sy = _rowdot(y_, s_)
+            use = valid & (sy > 0)
+            b = np.where(use, _rowdot(y_, q) / np.where(use, sy, 1.0), 0.0)
+            q = q + np.where(use, alpha[:, k] - b, 0.0)[:, None] * s_
+        d = -np.where(act, 0.0, q)
+        bad = _rowdot(pg, d) >= 0
+        if np.any(bad):
+            self.hl[idx[bad]] = 0
+            d[bad] = -pg[bad] / np.maximum(1.0, np.sqrt(_rowdot(pg[bad], pg[bad])))[:, None]
+        self.d[idx] = d
+        self.t[idx] = 1.0
+        self.xt[idx] = np.clip(self.x[idx] + d, self.lo[idx], self.hi[idx])
+
+    def step(self, slots, f, g) -> Dict[int, Outcome]:
+        slots = np.asarray(slots, dtype=np.int64)
+        f = np.asarray(f, dtype=np.float64)
+        g = np.asarray(g, dtype=np.float64)
+        self.n_evals[slots] += 1
+        finished: Dict[int, bool] = {}
+        need_dir = []
+        # first evaluations
+        ini = self.phase[slots] == 0
+        si = slots[ini]
+        self.x[si], self.f[si], self.g[si] = self.xt[si], f[ini], g[ini]
+        self.phase[si] = 1
+        need_dir.append(si)
+        # trial evaluations
+        tr = ~ini
+        st, ft, gt = slots[tr], f[tr], g[tr]
+        if len(st):
+            pg = self._projected(st)
+            ok = np.isfinite(ft) & (ft <= self.f[st] + 1e-4 * _rowdot(pg, self.xt[st] - self.x[st]))
+            # accepted: history update, move, convergence on the decrease
+            sa = st[ok]
+            if len(sa):
+                s_ = self.xt[sa] - self.x[sa]
+                y_ = gt[ok] - self.g[sa]
+                good = _rowdot(s_, y_) > 1e-12 * np.sqrt(_rowdot(s_, s_)) * np.sqrt(_rowdot(y_, y_))
+                for k in np.nonzero(good)[0]:
+                    sl = int(sa[k])
+                    h = int(self.hl[sl])
+                    if h == self.m:
+                        self.S[sl, :-1] = self.S[sl, 1:]
+                        self.Y[sl, :-1] = self.Y[sl, 1:]
+                        h -= 1
+                    self.S[sl, h] = s_[k]
+                    self.Y[sl, h] = y_[k]
+                    self.hl[sl] = h + 1
+                df = self.f[sa] - ft[ok]
+                self.x[sa], self.f[sa], self.g[sa] = self.xt[sa], ft[ok], gt[ok]
+                self.n_iter[sa] += 1
+                self.restarted[sa] = False
+                small = (df >= 0) & (df <= self.ftol * np.maximum(1.0, np.abs(self.f[sa])))
+                for k in np.nonzero(small)[0]:
+                    finished[int(sa[k])] = False
+                need_dir.append(sa[~small])
+            # rejected: backtrack, or restart once from steepest descent
+            sr = st[~ok]
+            if len(sr):
+                self.t[sr] *= 0.5
+                tiny = self.t[sr] < 1e-12
+                back = sr[~tiny]
+                self.xt[back] = np.clip(self.x[back] + self.t[back, None] * self.d[back], self.lo[back], self.hi[back])
+                for sl in sr[tiny]:
+                    sl = int(sl)
+                    if self.hl[sl] > 0 and not self.restarted[sl]:
+                        self.hl[sl] = 0
+                        self.restarted[sl] = True
+                        need_dir.append(np.array([sl], dtype=np.int64))
+                    else:
+                        finished[sl] = None
+        conv: Dict[int, bool] = {}
+        nd = np.concatenate(need_dir) if need_dir else np.zeros(0, dtype=np.int64)
+        self._directions(nd, conv)
+        out: Dict[int, Outcome] = {}
+        for sl, c in conv.items():
+            out[sl] = Outcome(self.x[sl].copy(), float(self.f[sl]), int(self.n_evals[sl]), int(self.n_iter[sl]), True)
+        for sl, flag in finished.items():
+            if flag is None:
+                c = False
+            else:
+                pgn = self._projected(np.array([sl]))[0]
+                c = bool(np.max(np.abs(pgn)) <= 1e3 * self.pgtol * max(1.0, abs(self.f[sl])))
+            out[sl] = Outcome(self.x[sl].copy(), float(self.f[sl]), int(self.n_evals[sl]), int(self.n_iter[sl]), c)
+        # evaluation budget
+        for sl in slots:
+            sl = int(sl)
+            if sl not in out and self.n_evals[sl] >= self.max_evals[sl]:
+                out[sl] = Outcome(self.x[sl].copy(), float(self.f[sl]), int(self.n_evals[sl]), int(self.n_iter[sl]), False)
+        return out
+
+
+# ---------------------------------------------------------------------------
 # service
 # ---------------------------------------------------------------------------
 @dataclass
@@ -209,85 +389,127 @@ class FitService:
         self.clock = clock
         self.batch = _native.Batch(max_slots, device=device)
         self.np = None
-        self.stream = torch.cuda.current_stream(device)
 
     def footprint(self, n3) -> int:
         return _native.footprint_bytes(self.plan_text, n3[0], n3[1], n3[2], self.precision)
 
     def run(self, jobs: List[FitJob]) -> ServiceStats:
+        """Serve ``jobs`` until every fit has finished.
+
+        A worker thread admits fits in arrival order: it waits for the arrival time and
+        for HBM (ctpwa_footprint_bytes against the budget), then builds the fit's handle
+        (upload, SL tensors) while the main loop keeps evaluating the active fits on its
+        own stream.  The main loop moves ready fits into free slots between rounds."""
+        import queue as _queue
+        import threading
         st = ServiceStats()
-        queue = sorted(jobs, key=lambda j: j.arrival)
-        qi = 0
-        active: Dict[int, dict] = {}           # slot -> state
-        free_slots = list(range(self.max_slots))[::-1]
-        mem_used = 0
+        order = sorted(jobs, key=lambda j: j.arrival)
         dev = torch.device("cuda", self.device)
-        P = O = P_host = O_host = None
+        stream = torch.cuda.Stream(device=dev)
+        ready: "_queue.Queue" = _queue.Queue()
+        cond = threading.Condition()
+        mem = {"used": 0, "stop": False}
         t0 = self.clock()
-        while qi < len(queue) or active:
-            now = self.clock() - t0
-            # admission: arrived fits while a slot and memory are free
-            while qi < len(queue) and queue[qi].arrival <= now and free_slots:
-                job = queue[qi]
-                smp = job.samples() if callable(job.samples) else job.samples
-                ph, da, bk = smp.get("phsp"), smp.get("data"), smp.get("bkg")
-                n3 = (len(ph), len(da), 0 if bk is None else len(bk))
-                need = self.footprint(n3)
-                if need > self.hbm_budget:
-                    raise RuntimeError("fit %d needs %d bytes, more than the HBM budget" % (job.job_id, need))
-                if mem_used + need > self.hbm_budget:
-                    break
-                ta = self.clock()
-                h = _native.Handle(self.plan_text, ph, da, bk, smp.get("phsp_weights"), smp.get("data_weights"),
-                                   smp.get("bkg_weights"), device=self.device, precision=self.precision)
-                st.admissions_seconds += self.clock() - ta
-                if self.np is None:
-                    self.np = h.n_params
-                    P = torch.zeros((self.max_slots, self.np), dtype=torch.float64, device=dev)
-                    O = torch.zeros((self.max_slots, 1 + self.np), dtype=torch.float64, device=dev)
-                    P_host = torch.zeros((self.max_slots, self.np), dtype=torch.float64).pin_memory()
-                    O_host = torch.zeros((self.max_slots, 1 + self.np), dtype=torch.float64).pin_memory()
-                gen = lbfgs(job.x0, job.lower, job.upper, max_evals=job.max_evals)
-                x = next(gen)
-                slot = free_slots.pop()
-                mem_used += need
-                active[slot] = dict(job=job, h=h, gen=gen, x=x, t_req=self.clock(), need=need,
-                                    admitted=self.clock() - t0, n_events=sum(n3))
-                qi += 1
-                now = self.clock() - t0
-            if not active:
-                if qi < len(queue):
-                    wait = queue[qi].arrival - (self.clock() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 0.01))
-                continue
-            # one round: every active fit's pending point in one batched evaluation
-            tr = self.clock()
-            slots = sorted(active)
-            for s in slots:
-                P_host[s].numpy()[:] = active[s]["x"]
-            P.copy_(P_host, non_blocking=True)
-            self.batch.nll_grad([active[s]["h"] for s in slots], [P[s].data_ptr() for s in slots],
-                                [O[s].data_ptr() for s in slots], self.stream.cuda_stream)
-            O_host.copy_(O, non_blocking=True)
-            self.stream.synchronize()
-            tdone = self.clock()
-            st.round_seconds.append(tdone - tr)
-            st.batch_sizes.append(len(slots))
-            res = O_host.numpy()
-            for s in slots:
-                a = active[s]
-                st.latencies.append(tdone - a["t_req"])
-                try:
-                    a["x"] = a["gen"].send((float(res[s, 0]), res[s, 1:].copy()))
-                    a["t_req"] = self.clock()
-                except StopIteration as fin:
-                    st.fits.append(FitRecord(a["job"].job_id, fin.value, a["n_events"], a["admitted"],
-                                             self.clock() - t0))
+
+        def admit():
+            try:
+                for job in order:
+                    while not mem["stop"]:
+                        wait = job.arrival - (self.clock() - t0)
+                        if wait <= 0:
+                            break
+                        time.sleep(min(wait, 0.005))
+                    smp = job.samples() if callable(job.samples) else job.samples
+                    ph, da, bk = smp.get("phsp"), smp.get("data"), smp.get("bkg")
+                    n3 = (len(ph), len(da), 0 if bk is None else len(bk))
+                    need = self.footprint(n3)
+                    if need > self.hbm_budget:
+                        raise RuntimeError("fit %d needs %d bytes, more than the HBM budget" % (job.job_id, need))
+                    with cond:
+                        while mem["used"] + need > self.hbm_budget and not mem["stop"]:
+                            cond.wait(0.05)
+                        if mem["stop"]:
+                            return
+                        mem["used"] += need
+                    ta = self.clock()
+                    h = _native.Handle(self.plan_text, ph, da, bk, smp.get("phsp_weights"), smp.get("data_weights"),
+                                       smp.get("bkg_weights"), device=self.device, precision=self.precision)
+                    st.admissions_seconds += self.clock() - ta
+                    ready.put((job, h, need, sum(n3), self.clock() - t0))
+            except BaseException as exc:          # surfaces in the main loop
+                ready.put(exc)
+                return
+            ready.put(None)
+
+        worker = threading.Thread(target=admit, daemon=True)
+        worker.start()
+        opt = None
+        P = O = P_host = O_host = None
+        active: Dict[int, dict] = {}
+        free_slots = list(range(self.max_slots))[::-1]
+        admitting = True
+        try:
+            while admitting or active:
+                # fits whose handles are ready take the free slots
+                while admitting and free_slots:
+                    try:
+                        item = ready.get(block=not active, timeout=None if not active else 0)
+                    except _queue.Empty:
+                        break
+                    if item is None:
+                        admitting = False
+                        break
+                    if isinstance(item, BaseException):
+                        raise item
+                    job, h, need, nev, t_adm = item
+                    if opt is None:
+                        self.np = h.n_params
+                        opt = BatchLBFGS(self.max_slots, self.np)
+                        P = torch.zeros((self.max_slots, self.np), dtype=torch.float64, device=dev)
+                        O = torch.zeros((self.max_slots, 1 + self.np), dtype=torch.float64, device=dev)
+                        P_host = torch.zeros((self.max_slots, self.np), dtype=torch.float64).pin_memory()
+                        O_host = torch.zeros((self.max_slots, 1 + self.np), dtype=torch.float64).pin_memory()
+                    slot = free_slots.pop()
+                    opt.start(slot, job.x0, job.lower, job.upper, job.max_evals)
+                    active[slot] = dict(job=job, h=h, need=need, t_req=self.clock(), admitted=t_adm, n_events=nev)
+                if not active:
+                    continue
+                # one round: every active fit's pending point in one batched evaluation
+                tr = self.clock()
+                slots = sorted(active)
+                Ph = P_host.numpy()
+                Ph[slots] = opt.xt[slots]
+                with torch.cuda.stream(stream):
+                    P.copy_(P_host, non_blocking=True)
+                    self.batch.nll_grad([active[s]["h"] for s in slots], [P[s].data_ptr() for s in slots],
+                                        [O[s].data_ptr() for s in slots], stream.cuda_stream)
+                    O_host.copy_(O, non_blocking=True)
+                stream.synchronize()
+                tdone = self.clock()
+                st.round_seconds.append(tdone - tr)
+                st.batch_sizes.append(len(slots))
+                res = O_host.numpy()
+                done = opt.step(slots, res[slots, 0], res[slots, 1:])
+                tnext = self.clock()
+                for s in slots:
+                    a = active[s]
+                    st.latencies.append(tdone - a["t_req"])
+                    a["t_req"] = tnext
+                for s, outcome in done.items():
+                    a = active.pop(s)
+                    st.fits.append(FitRecord(a["job"].job_id, outcome, a["n_events"], a["admitted"], tnext - t0))
                     a["h"].close()
-                    mem_used -= a["need"]
+                    with cond:
+                        mem["used"] -= a["need"]
+                        cond.notify_all()
                     free_slots.append(s)
-                    del active[s]
+        finally:
+            with cond:
+                mem["stop"] = True
+                cond.notify_all()
+            worker.join()
+            for a in active.values():
+                a["h"].close()
         st.wall = self.clock() - t0
         return st
 
