@@ -138,6 +138,12 @@ static void node_cg_table(const PNode& n, int S, int L, std::vector<double>& out
 static std::mutex g_mod_mu;
 static std::map<std::pair<int, std::string>, hipModule_t> g_mods;
 
+// hipRTC options of the generated kernels: gfx950, and no packed FP32 instructions
+// (the same target-feature switch as the library's own hipcc build)
+static const char* RTC_OPTS[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
+                                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+static const int RTC_NOPTS = (int)(sizeof(RTC_OPTS) / sizeof(RTC_OPTS[0]));
+
 static hipModule_t compile_eval(int device, const std::string& src) {
     std::lock_guard<std::mutex> lk(g_mod_mu);
     auto key = std::make_pair(device, src);
@@ -149,8 +155,7 @@ static hipModule_t compile_eval(int device, const std::string& src) {
         hiprtcProgram prog;
         if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
             throw std::runtime_error("hiprtcCreateProgram failed");
-        const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast"};
-        hiprtcResult r = hiprtcCompileProgram(prog, 4, opts);
+        hiprtcResult r = hiprtcCompileProgram(prog, RTC_NOPTS, RTC_OPTS);
         if (r != HIPRTC_SUCCESS) {
             size_t ls = 0;
             hiprtcGetProgramLogSize(prog, &ls);
@@ -197,6 +202,8 @@ struct ctpwa_handle {
     hipFunction_t fn_eval = nullptr;       // one likelihood
     hipFunction_t fn_group = nullptr;      // many likelihoods of this model in one launch
     hipFunction_t fn_aux = nullptr;        // amplitudes / intensities (models with Custom lineshapes)
+    hipFunction_t fn_custom = nullptr;     // Custom factors and derivatives per event (models with Custom lineshapes)
+    double* cfb = nullptr;                 // their per-event rows, [ncustom * ncf][ntot]
     std::string src;
     // Hessian workspace (allocated on the first ctpwa_hessian call)
     bool hess_ready = false;
@@ -665,7 +672,11 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->mod = compile_eval(device, h->src);
         HIPCHECK(hipModuleGetFunction(&h->fn_eval, h->mod, "ctpwa_eval"));
         HIPCHECK(hipModuleGetFunction(&h->fn_group, h->mod, "ctpwa_eval_group"));
-        if (!M.customs.empty()) HIPCHECK(hipModuleGetFunction(&h->fn_aux, h->mod, "ctpwa_aux"));
+        if (!M.customs.empty()) {
+            HIPCHECK(hipModuleGetFunction(&h->fn_aux, h->mod, "ctpwa_aux"));
+            HIPCHECK(hipModuleGetFunction(&h->fn_custom, h->mod, "ctpwa_custom"));
+            h->cfb = h->alloc<double>((size_t)M.customs.size() * M.ncf() * h->ntot);
+        }
         h->used_inv.assign(M.inv.size(), 0);
         for (const DenSpec& D : M.dens) {
             for (const Term& t : D.re) if (t.inv >= 0) h->used_inv[t.inv] = 1;
@@ -704,9 +715,17 @@ extern "C" int ctpwa_param_info(const ctpwa_handle* h, double* init, double* low
     return 0;
 }
 
+// Custom factors of every stored event at the parameter point the table holds
+static hipError_t launch_custom(ctpwa_handle* h, hipStream_t s) {
+    if (!h->fn_custom) return hipSuccess;
+    void* args[] = {&h->table, &h->inv, &h->invf, &h->ntot, &h->cfb};
+    unsigned grid = (unsigned)std::min<long long>(4096, (h->ntot + 255) / 256);
+    return hipModuleLaunchKernel(h->fn_custom, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+}
+
 static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s) {
     void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->wts, &h->part, &h->wg_cls, &h->wg_rng, &h->ntot,
-                    &h->nwg, &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2]};
+                    &h->nwg, &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2], &h->cfb};
     return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, h->M.wgs, 1, 1, 0, s, args, nullptr);
 }
 
@@ -721,6 +740,7 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess) e = launch_eval(h, s);
     if (e == hipSuccess) {
         h->fin.params = params_dev;
@@ -848,12 +868,13 @@ static int aux_call(ctpwa_handle* h, const double* params_dev, int cls, double* 
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess && h->fn_aux) {
         // generated kernel of the model (Custom lineshapes)
         long long e0 = h->off[cls], n = h->n[cls];
         if (n > 0) {
             int flag = intensity;
-            void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->ntot, &e0, &n, &out_dev, &flag};
+            void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->ntot, &e0, &n, &out_dev, &flag, &h->cfb};
             e = hipModuleLaunchKernel(h->fn_aux, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, s, args, nullptr);
         }
     } else if (e == hipSuccess) {
@@ -879,6 +900,7 @@ extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev,
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess) e = hipEventRecord(a, s);
     for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_eval(h, s);
     if (e == hipSuccess) e = hipEventRecord(b, s);
@@ -898,7 +920,7 @@ extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev,
 // mirrors `struct EvalJob` of the generated source (codegen.cpp)
 struct EvalJob {
     const double* tab; const void* sl; const double* inv; const float* invf; const double* wts;
-    double* part; const int* wg_cls; const long long* wg_rng; long long ld; double w1, w2;
+    double* part; const int* wg_cls; const long long* wg_rng; const double* cf; long long ld; double w1, w2;
     int nwg, pw1, pw2, wg_begin;
 };
 
@@ -979,7 +1001,7 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
         int wg = 0;
         for (int j = 0; j < n; ++j) {
             ctpwa_handle* h = hs[j];
-            jobs[j] = EvalJob{h->table, h->slp, h->inv, h->invf, h->wts, h->part, h->wg_cls, h->wg_rng, h->ntot,
+            jobs[j] = EvalJob{h->table, h->slp, h->inv, h->invf, h->wts, h->part, h->wg_cls, h->wg_rng, h->cfb, h->ntot,
                               h->wc[1], h->wc[2], h->nwg, h->pw[1], h->pw[2], wg};
             wg += h->nwg;
             fins[j] = h->fin;
@@ -993,6 +1015,7 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
         b->pending = true;
         HIPCHECK(ctpwa_launch_prep_batch((const double* const*)(b->dev + b->o_params), (double* const*)(b->dev + b->o_tables),
                                          &h0->prep, h0->M.maxloc, n, s));
+        for (int j = 0; j < n; ++j) HIPCHECK(launch_custom(hs[j], s));
         const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
         void* args[] = {&d_jobs, &n};
         HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
@@ -1009,6 +1032,7 @@ static long long event_bytes(const Model_& M, const std::vector<int>* used) {
     long long b = (long long)M.us.size() * M.P.npol * (M.mixed ? 8 : 16);
     for (size_t i = 0; i < M.inv.size(); ++i)
         if (!used || (*used)[i]) b += (imap[i] >= 0) ? 8 : 4;
+    b += (long long)M.customs.size() * M.ncf() * 8;      // Custom factor rows
     return b;
 }
 
@@ -1083,8 +1107,7 @@ extern "C" long long ctpwa_compile_check(const char* plan_text, int precision) {
         hiprtcProgram prog;
         if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
             throw std::runtime_error("hiprtcCreateProgram failed");
-        const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast"};
-        hiprtcResult r = hiprtcCompileProgram(prog, 4, opts);
+        hiprtcResult r = hiprtcCompileProgram(prog, RTC_NOPTS, RTC_OPTS);
         size_t ls = 0;
         hiprtcGetProgramLogSize(prog, &ls);
         std::string log(ls + 1, '\0');

@@ -271,15 +271,12 @@ std::string generate_eval_kernel(const Model_& M) {
         for (const Term& t : Dn.re) if (t.inv >= 0) used[t.inv] = 1;
         for (const Term& t : Dn.im) if (t.inv >= 0) used[t.inv] = 1;
     }
-    for (const CustomSpec& C : M.customs) { used[C.inv_m] = 1; used[C.inv_q] = 1; }
     std::ostringstream o;
     long long tile_bytes = 16;       // LDS tile buffer of the pipelined form (per buffer)
     o << "// generated: fused likelihood kernel\n";
-    if (!M.customs.empty()) {
-        size_t ncv = 1;
-        for (const CustomSpec& C : M.customs) ncv = std::max(ncv, M.bvars[C.block].size());
-        o << "#define NCV " << ncv << "\n#define OFFT " << M.off_th() << "LL\n" << CD_PREAMBLE;
-    }
+    if (!M.customs.empty())
+        o << "#define NCV " << M.ncv() << "\n#define NCF " << M.ncf() << "\n#define OFFT " << M.off_th() << "LL\n"
+          << CD_PREAMBLE;
     // reciprocal: hardware estimate refined by two Newton steps (~1 ulp)
     o << "__device__ __forceinline__ double frcp(double x) {\n"
          "  double r = __builtin_amdgcn_rcp(x);\n"
@@ -294,7 +291,7 @@ std::string generate_eval_kernel(const Model_& M) {
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
          "    double w1, double w2, int pw1, int pw2, const int wg, double (*__restrict__ red)[" << NACC << "],\n"
-         "    unsigned char* __restrict__ lbuf)\n{\n";
+         "    unsigned char* __restrict__ lbuf, const double* __restrict__ cfb)\n{\n";
     // storage types: FP64, or FP32 SL tensors / FP32 invariants except s (mixed)
     const char* SLT = M.mixed ? "float2" : "double2";
     o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n";
@@ -435,14 +432,12 @@ std::string generate_eval_kernel(const Model_& M) {
             o << "    const double q" << d << " = frcp(dr" << d << " * dr" << d << " + di" << d << " * di" << d << ");\n";
             o << "    const double pr" << d << " = dr" << d << " * q" << d << ", pi" << d << " = di" << d << " * q" << d << ";\n";
         }
-        // Custom lineshape factors with their derivatives over the block's local variables
+        // Custom lineshape factors as written for this parameter point by ctpwa_custom (their
+        // derivatives are loaded where the gradient accumulators use them)
         for (size_t c = 0; c < M.customs.size(); ++c) {
-            const CustomSpec& C = M.customs[c];
-            size_t ti = 0;
-            std::string code = custom_code(C.expr, ti, C, M.bvars[C.block]);
-            if (ti != C.expr.size()) throw std::runtime_error("Custom expression has trailing tokens");
-            o << "    CD F" << c << ";\n    { const double xm = x" << C.inv_m << ", xq = x" << C.inv_q << ";\n"
-              << "      F" << c << " = " << code << "; }\n";
+            const long long r0 = (long long)c * M.ncf();
+            o << "    const double F" << c << "r = cfb[" << r0 << "LL * ld + e], F" << c << "i = cfb[" << r0 + 1
+              << "LL * ld + e];\n";
         }
         for (size_t b = 0; b < P.blocks.size(); ++b) {
             const std::vector<int>& ds = M.block_dens[b];
@@ -458,8 +453,8 @@ std::string generate_eval_kernel(const Model_& M) {
                 // product of the denominators alone (for the derivatives through Custom factors)
                 o << "    const double pdr" << b << " = br" << b << ", pdi" << b << " = bi" << b << ";\n";
                 for (int c : M.block_customs[b])
-                    o << "    { const double tr = br" << b << " * F" << c << ".r - bi" << b << " * F" << c << ".i; bi" << b
-                      << " = br" << b << " * F" << c << ".i + bi" << b << " * F" << c << ".r; br" << b << " = tr; }\n";
+                    o << "    { const double tr = br" << b << " * F" << c << "r - bi" << b << " * F" << c << "i; bi" << b
+                      << " = br" << b << " * F" << c << "i + bi" << b << " * F" << c << "r; br" << b << " = tr; }\n";
             }
         }
     };
@@ -536,13 +531,14 @@ std::string generate_eval_kernel(const Model_& M) {
             o << "      {\n        double or_ = pdr" << b << ", oi_ = pdi" << b << ";\n";
             for (int c2 : M.block_customs[b]) {
                 if (c2 == c) continue;
-                o << "        { const double tr = or_ * F" << c2 << ".r - oi_ * F" << c2 << ".i; oi_ = or_ * F" << c2
-                  << ".i + oi_ * F" << c2 << ".r; or_ = tr; }\n";
+                o << "        { const double tr = or_ * F" << c2 << "r - oi_ * F" << c2 << "i; oi_ = or_ * F" << c2
+                  << "i + oi_ * F" << c2 << "r; or_ = tr; }\n";
             }
             o << "        const double wr = zr * or_ - zi * oi_, wi = zr * oi_ + zi * or_;\n";
             for (int t : custom_terms[c]) {
                 int li = M.cterms[t].local;
-                o << "        ct" << t << " += wr * F" << c << ".dr[" << li << "] - wi * F" << c << ".di[" << li << "];\n";
+                const long long rr = (long long)c * M.ncf() + 2 + li, ri = rr + M.ncv();
+                o << "        ct" << t << " += wr * cfb[" << rr << "LL * ld + e] - wi * cfb[" << ri << "LL * ld + e];\n";
             }
             o << "      }\n";
         }
@@ -576,13 +572,13 @@ std::string generate_eval_kernel(const Model_& M) {
          "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
-         "    double w1, double w2, int pw1, int pw2)\n{\n" << decl.str() <<
-         "  eval_body(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red, lbuf);\n}\n";
+         "    double w1, double w2, int pw1, int pw2, const double* __restrict__ cfb)\n{\n" << decl.str() <<
+         "  eval_body(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red, lbuf, cfb);\n}\n";
     // grouped launch: jobs[j] owns workgroups [wg_begin_j, wg_begin_{j+1}); the layout of
     // this struct is mirrored by EvalJob in capi.hip
     o << "struct EvalJob {\n"
          "  const double* tab; const void* sl; const double* inv; const float* invf; const double* wts;\n"
-         "  double* part; const int* wg_cls; const long long* wg_rng; long long ld; double w1, w2;\n"
+         "  double* part; const int* wg_cls; const long long* wg_rng; const double* cf; long long ld; double w1, w2;\n"
          "  int nwg, pw1, pw2, wg_begin;\n};\n";
     o << "extern \"C\" __global__ void " << lb << " ctpwa_eval_group(const EvalJob* __restrict__ jobs, int njobs)\n{\n"
       << decl.str() <<
@@ -590,14 +586,36 @@ std::string generate_eval_kernel(const Model_& M) {
          "  while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (jobs[mid].wg_begin <= (int)blockIdx.x) lo = mid; else hi = mid - 1; }\n"
          "  const EvalJob J = jobs[lo];\n"
          "  eval_body(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
-         "            (int)blockIdx.x - J.wg_begin, red, lbuf);\n}\n";
+         "            (int)blockIdx.x - J.wg_begin, red, lbuf, J.cf);\n}\n";
     if (!M.customs.empty()) {
+        // Custom factors and their derivatives for every stored event at the current
+        // parameter point, rows [c * NCF + (r, i, dr[0..NCV), di[0..NCV))] of cfb.  A kernel
+        // of its own: the complex-dual expression code then never shares registers with
+        // the fused pass (which only loads the results).
+        o << "extern \"C\" __global__ void __launch_bounds__(256) ctpwa_custom(\n"
+             "    const double* __restrict__ tab, const double* __restrict__ inv, const float* __restrict__ invf,\n"
+             "    long long ld, double* __restrict__ cfb)\n{\n"
+             "  const double* tb = tab;\n"
+             "  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < ld; e += (long long)gridDim.x * 256) {\n";
+        for (size_t c = 0; c < M.customs.size(); ++c) {
+            const CustomSpec& C = M.customs[c];
+            size_t ti = 0;
+            std::string code = custom_code(C.expr, ti, C, M.bvars[C.block]);
+            if (ti != C.expr.size()) throw std::runtime_error("Custom expression has trailing tokens");
+            const long long r0 = (long long)c * M.ncf();
+            o << "    {\n      const double xm = " << inv_load(C.inv_m, "e") << ", xq = " << inv_load(C.inv_q, "e") << ";\n"
+              << "      const CD F = " << code << ";\n"
+              << "      cfb[" << r0 << "LL * ld + e] = F.r; cfb[" << r0 + 1 << "LL * ld + e] = F.i;\n"
+              << "      for (int k = 0; k < NCV; ++k) { cfb[(" << r0 + 2 << "LL + k) * ld + e] = F.dr[k]; cfb[(" << r0 + 2
+              << "LL + NCV + k) * ld + e] = F.di[k]; }\n    }\n";
+        }
+        o << "  }\n}\n";
         // per-event amplitudes (without couplings) or intensities of a range of events:
         // models with Custom lineshapes evaluate them with the same generated core
         o << "extern \"C\" __global__ void __launch_bounds__(256) ctpwa_aux(\n"
              "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
              "    const float* __restrict__ invf, long long ld, long long e0, long long n, double* __restrict__ out,\n"
-             "    int intensity)\n{\n";
+             "    int intensity, const double* __restrict__ cfb)\n{\n";
         o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n  const double* tb = tab;\n";
         emit_coefs("  ");
         o << "  const long long ie = (long long)blockIdx.x * 256 + threadIdx.x;\n  if (ie >= n) return;\n"

@@ -181,7 +181,7 @@ struct Model_ {
     int maxloc = 0;
     // code-generation options of the fused kernel
     int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
-    int wgs = 256;           // threads (= events per tile) of a workgroup: 64, 128 or 256
+    int wgs = 128;           // threads (= events per tile) of a workgroup: 64, 128 or 256
     bool lds_pipe = false;   // copy the next tile's inputs HBM -> LDS while computing the current one
     bool prefetch_sl = false; // load the next event's SL values one iteration ahead (registers)
     // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
@@ -210,6 +210,14 @@ struct Model_ {
     size_t off_rec() const { return 4 * (size_t)P.namp; }
     size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC + (size_t)P.ntheta; }
     int nacc() const { return 1 + 2 * P.namp + (int)thterms.size() + (int)cterms.size(); }
+    // Custom factors: derivative directions (the largest block among Custom blocks) and
+    // per-event rows of one factor in the factor buffer (value, then dr[], di[])
+    int ncv() const {
+        size_t n = 1;
+        for (const CustomSpec& C : customs) n = std::max(n, bvars[C.block].size());
+        return (int)n;
+    }
+    int ncf() const { return 2 + 2 * ncv(); }
     size_t off_th() const { return off_rec() + (size_t)n_coef_rec * REC; }   // theta values (for Custom)
 };
 

@@ -41,7 +41,10 @@ def test_custom_parity(k, precision, oracle_lib):
     p = perturbed(W.nominal_params(plan, 1), plan, 30 + k)
     nll_o, g_o = O.nll_grad(p)
     nll_g, g_g = _run(h, p)
-    tol_n, tol_g, tol_a = (1e-9, 1e-7, 1e-10) if precision == "fp64" else (1e-9, 1e-5, 1e-6)
+    # mixed: the FP32-stored SL tensors carry 2^-24 relative rounding per entry; over
+    # 4k data events the NLL moves by a few 1e-9 relative (the |dNLL| <= 1e-4 gate of the
+    # mixed form at the full size is far looser)
+    tol_n, tol_g, tol_a = (1e-9, 1e-7, 1e-10) if precision == "fp64" else (1e-8, 1e-5, 1e-6)
     print(nll_o, nll_g, rel(g_g, g_o), flush=True)
     assert abs(nll_g - nll_o) <= tol_n * abs(nll_o)
     assert rel(g_g, g_o) <= tol_g

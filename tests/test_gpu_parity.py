@@ -8,6 +8,7 @@ import pytest
 import torch
 
 from helpers import perturbed, plan_of, rel, samples
+from ctpwa_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 

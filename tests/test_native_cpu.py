@@ -35,7 +35,7 @@ def test_generated_kernel_compiles_for_gfx950(name, precision):
     from ctpwa_amd import _native
     plan = plan_of(name)
     src = _native.generate_source(plan.to_text(), precision)
-    assert "ctpwa_eval" in src and "__launch_bounds__(256" in src
+    assert "ctpwa_eval" in src and "__launch_bounds__(128" in src
     # every amplitude coefficient appears
     for a in range(plan.n_amp):
         assert "c%dr" % a in src
