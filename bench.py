@@ -142,24 +142,28 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
 
 def run_serving(args, samples, plan, spec, W, precisions):
     """Toy-MC ensemble through the fit service: data counts log-uniform in [5k, 500k],
-    phase space 5x data, Poisson arrivals; every fit is a bounded L-BFGS issuing
-    NLL+gradient requests that the service batches across the active fits."""
+    phase space 5x data, Poisson arrivals; every fit is a bounded Newton minimisation with
+    the GPU Hessian (or L-BFGS) whose NLL+gradient requests the service batches across the
+    active fits, run until EDM < 1e-5 or 400 evaluations."""
     from ctpwa_amd.serve import FitService, toy_ensemble
     x_nom = W.nominal_params(plan, spec["seed"])
     res = {"jobs": args.serving_jobs, "arrival_rate_per_s": args.serving_rate, "slots": args.serving_slots,
+           "method": args.serving_method, "stop": "EDM < 1e-5 (Newton) or 400 evaluations",
            "seed": 3, "data_per_fit": "log-uniform [5000, 500000]", "phsp_per_fit": "5 x data",
            "events": "windows of the benchmark's data and phase-space samples"}
     for prec in precisions:
         jobs = toy_ensemble(plan, samples["phsp"], samples["data"], args.serving_jobs, seed=3,
                             rate=args.serving_rate, x_nominal=x_nom)
-        svc = FitService(plan.to_text(), precision=prec, max_slots=args.serving_slots)
+        svc = FitService(plan.to_text(), precision=prec, max_slots=args.serving_slots, method=args.serving_method)
         st = svc.run(jobs)
         svc.close()
         sm = st.summary()
         res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sm.items()}
-        log("[bench] serving %s: %d fits in %.1f s, %.0f fits/h, p50 %.2f ms, p99 %.2f ms, mean batch %.1f"
-            % (prec, sm["fits"], sm["wall_s"], sm["fits_per_hour"], sm["latency_p50_ms"], sm["latency_p99_ms"],
-               sm["mean_batch"]))
+        log("[bench] serving %s: %d fits (%d converged, median %d evaluations, %d Hessians) in %.1f s, %.0f fits/h, "
+            "evaluation p50 %.2f ms p99 %.2f ms, fit p50 %.2f s p99 %.2f s, mean batch %.1f"
+            % (prec, sm["fits"], sm["converged"], sm["evals_per_fit_median"], sm["hessians"], sm["wall_s"],
+               sm["fits_per_hour"], sm["latency_p50_ms"], sm["latency_p99_ms"], sm["fit_latency_p50_s"],
+               sm["fit_latency_p99_s"], sm["mean_batch"]))
     return res
 
 
@@ -179,7 +183,11 @@ def main():
                     help="both: time the mixed form (the reported value) and the FP64 form beside it")
     ap.add_argument("--serving-jobs", type=int, default=256,
                     help="toy fits run through the fit service after the timing (0: skip)")
-    ap.add_argument("--serving-rate", type=float, default=100.0, help="Poisson arrival rate of the toy fits per second")
+    ap.add_argument("--serving-rate", type=float, default=1000.0,
+                    help="Poisson arrival rate of the toy fits per second (above the service's capacity, so "
+                         "fits/hour measures the service)")
+    ap.add_argument("--serving-method", choices=["auto", "newton", "lbfgs"], default="auto",
+                    help="minimiser of the toy fits (auto: Newton with the GPU Hessian)")
     ap.add_argument("--serving-slots", type=int, default=64, help="fits evaluated together at most")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "

@@ -19,7 +19,7 @@ _ll = ctypes.c_longlong
 EXPORTS = [
     "ctpwa_create", "ctpwa_destroy", "ctpwa_dims", "ctpwa_param_info", "ctpwa_nll_grad",
     "ctpwa_hessian", "ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_time_eval_kernel",
-    "ctpwa_footprint_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
+    "ctpwa_footprint_bytes", "ctpwa_hessian_workspace_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
     "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
 ]
@@ -52,6 +52,8 @@ def lib():
     L.ctpwa_time_eval_kernel.argtypes = [_vp, _vp, ctypes.c_int, _vp, _dp]
     L.ctpwa_footprint_bytes.restype = _ll
     L.ctpwa_footprint_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll, ctypes.c_int]
+    L.ctpwa_hessian_workspace_bytes.restype = _ll
+    L.ctpwa_hessian_workspace_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll]
     L.ctpwa_layout.argtypes = [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll), ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.ctpwa_kernel_source.restype = ctypes.c_char_p
@@ -113,6 +115,13 @@ def compile_check(plan_text: str, precision=FP64) -> int:
 
 def footprint_bytes(plan_text: str, n_phsp: int, n_data: int, n_bkg: int, precision=FP64) -> int:
     n = lib().ctpwa_footprint_bytes(plan_text.encode(), n_phsp, n_data, n_bkg, precision_code(precision))
+    if n < 0:
+        raise NativeError(last_error())
+    return int(n)
+
+
+def hessian_workspace_bytes(plan_text: str, n_phsp: int, n_data: int, n_bkg: int) -> int:
+    n = lib().ctpwa_hessian_workspace_bytes(plan_text.encode(), n_phsp, n_data, n_bkg)
     if n < 0:
         raise NativeError(last_error())
     return int(n)

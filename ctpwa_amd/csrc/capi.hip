@@ -758,18 +758,37 @@ static std::vector<int2> upper_triangle(int n) {
     return v;
 }
 
-static void hess_setup(ctpwa_handle* h) {
-    const Model_& M = h->M;
+// Hessian workspace: Jacobian columns C, u-vector length NU, second-part scalars NZ per event
+struct HessDims { int C, NU, NZ; };
+static HessDims hess_dims(const Model_& M) {
     const Plan& P = M.P;
-    h->hC = P.namp + P.ntheta;
-    h->hNU = 2 * P.namp + P.ntheta;
-    int nz = 1;
+    HessDims d{P.namp + P.ntheta, 2 * P.namp + P.ntheta, 1};
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         if (P.blocks[b].nsl == 0) continue;
         int nl = (int)M.bvars[b].size();
-        nz += P.blocks[b].nsl * 2 * nl + nl * (nl + 1) / 2;
+        d.NZ += P.blocks[b].nsl * 2 * nl + nl * (nl + 1) / 2;
     }
-    h->hNZ = nz;
+    return d;
+}
+static const long long HESS_CHUNK = 1 << 19;     // events per Jacobian pass
+static const int HESS_NWG = 512;                 // split-K workgroups of the Gram
+static long long hess_workspace_bytes(const Model_& M, long long maxn) {
+    HessDims d = hess_dims(M);
+    long long chunk = std::min<long long>(maxn, HESS_CHUNK);
+    long long nG = (long long)(2 * d.C) * (2 * d.C + 1) / 2, nUu = (long long)d.NU * (d.NU + 1) / 2;
+    long long maxcol = std::max<long long>(2048, std::max(d.NU, d.NZ));
+    long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + d.NZ + 2) + HESS_NWG * maxcol + 3 * (nG + nUu + d.NZ + d.NU) +
+                  (long long)d.NU * d.NU;
+    return w * 8 + (nG + nUu) * 8 + 9 * 256;     // + the index lists and allocation slack
+}
+
+static void hess_setup(ctpwa_handle* h) {
+    const Model_& M = h->M;
+    const Plan& P = M.P;
+    const HessDims hd = hess_dims(M);
+    h->hC = hd.C;
+    h->hNU = hd.NU;
+    h->hNZ = hd.NZ;
     if (2 * h->hC > 500) throw std::runtime_error("Hessian: more than 250 Jacobian columns is not built");
     {
         std::vector<int> mult(P.nfree, 0);
@@ -783,8 +802,8 @@ static void hess_setup(ctpwa_handle* h) {
     h->hklG = h->upload(klG);
     h->hklU = h->upload(klU);
     long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
-    h->hchunk = std::min<long long>(maxn, 1 << 19);
-    h->hnwg = 512;
+    h->hchunk = std::min<long long>(maxn, HESS_CHUNK);
+    h->hnwg = HESS_NWG;
     h->hJ = h->alloc<double>((size_t)h->hchunk * P.npol * 2 * h->hC);
     h->hU = h->alloc<double>((size_t)h->hchunk * h->hNU);
     h->hZ = h->alloc<double>((size_t)h->hchunk * h->hNZ);
@@ -1052,6 +1071,17 @@ extern "C" long long ctpwa_footprint_bytes(const char* plan_text, long long n_ph
         return b;
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_footprint_bytes: ") + e.what();
+        return -1;
+    }
+}
+
+extern "C" long long ctpwa_hessian_workspace_bytes(const char* plan_text, long long n_phsp, long long n_data,
+                                                   long long n_bkg) {
+    try {
+        Model_ M = derive(parse_plan(plan_text));
+        return hess_workspace_bytes(M, std::max(n_phsp, std::max(n_data, n_bkg)));
+    } catch (const std::exception& e) {
+        g_err = std::string("ctpwa_hessian_workspace_bytes: ") + e.what();
         return -1;
     }
 }

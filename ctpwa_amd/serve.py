@@ -7,8 +7,9 @@ counts) share the GPU.  All active fits advance in lock step:
             -> one batched evaluation (ctpwa_batch_nll_grad: batched prep, one grouped
                per-event launch whose workgroups are shared out among the fits, batched
                reduction) on the service's stream
-            -> one step of the minimiser for all fits at once (BatchLBFGS, numpy over
-               the fits' rows); finished fits retire and free their slot and HBM
+            -> one step of the minimiser for all fits at once; finished fits retire and
+               free their slot and HBM
+            -> (Newton) the exact Hessian (ctpwa_hessian) of each fit that moved
     meanwhile a worker thread admits arrived fits in order: it waits for HBM
     (ctpwa_footprint_bytes against the budget) and builds each fit's handle (upload,
     SL tensors) while the rounds go on; ready fits take free slots between rounds.
@@ -19,9 +20,17 @@ single one exactly, and the minimiser reduces every fit's row on its own), so a 
 trajectory does not depend on what else is running.
 
 The reference has no service; its fit driver (example/fit.py:84-260) runs one fit per
-process with torch's L-BFGS.  The minimiser here is a bounded L-BFGS (two-loop recursion,
-projected gradient, Armijo backtracking) kept on the host in float64 numpy; ``lbfgs`` is
-the same algorithm as a generator for a single fit.
+process with torch's L-BFGS and takes parameter errors from getHessian at the end.  The
+service's default minimiser, BatchNewton, uses that exact Hessian at every iterate: a
+bounded, Levenberg-Marquardt-damped Newton method that stops on the estimated distance to
+the minimum EDM = g^T H^-1 g / 2 (the MINUIT criterion) and returns the errors
+sqrt(diag H^-1) of its last step.  Partial-wave likelihoods have long, flat valleys
+(broad overlapping resonances); quasi-Newton steps crawl along them, so an L-BFGS fit of a
+few thousand events is still lowering the NLL after hundreds of evaluations (scipy's
+L-BFGS-B behaves the same), while Newton steps take a handful.  Models whose Hessian is not
+built (Custom lineshapes) use BatchLBFGS: a bounded L-BFGS (two-loop recursion, projected
+gradient, Armijo backtracking) kept on the host in float64 numpy; ``lbfgs`` is the same
+algorithm as a generator for a single fit.
 """
 from __future__ import annotations
 
@@ -46,6 +55,9 @@ class Outcome:
     n_evals: int
     n_iter: int
     converged: bool
+    n_hess: int = 0
+    edm: float = float("nan")
+    errors: Optional[np.ndarray] = None      # sqrt(diag H^-1); 0 for fixed, nan for flat parameters
 
 
 def lbfgs(x0: np.ndarray, lower: np.ndarray, upper: np.ndarray, m: int = 10, max_evals: int = 400,
@@ -322,6 +334,152 @@ class BatchLBFGS:
         return out
 
 
+class BatchNewton:
+    """Bounded damped Newton with exact Hessians, for many fits.
+
+    ``start`` places a fit in a slot; ``step(slots, f, g)`` consumes the results of the
+    slots' pending points and returns (finished, need_hessian): the slots whose point was
+    accepted (or was the first one) need the Hessian there, given back with
+    ``set_hessians(slots, H)``, which computes each slot's next trial point ``xt[slot]``
+    or finishes it.  Every slot is handled on its own rows, so a fit's trajectory does not
+    depend on the other slots.
+
+    Direction: coordinates that are fixed (lower == upper), held at a bound by a gradient
+    pointing out of the box, or flat (zero Hessian row, e.g. a resonance with no allowed
+    wave) are left out; on the rest, (H + lam diag|H|) d = -g with lam = 0 when H is
+    positive definite, else the smallest lam in 1e-8 * 10^k that makes it so
+    (Levenberg-Marquardt); lam shrinks tenfold after every accepted step.  Armijo
+    backtracking halves the step; after four halvings lam grows tenfold.  Converged: the
+    undamped H positive definite on the free coordinates and EDM = g^T H^-1 g / 2 < edm_tol
+    (checked at every Hessian, whatever the damping).  The errors are sqrt(diag H^-1) there."""
+
+    def __init__(self, cap: int, n: int, edm_tol: float = 1e-5, c1: float = 1e-4):
+        self.cap, self.n, self.edm_tol, self.c1 = cap, n, edm_tol, c1
+        z = lambda *sh: np.zeros(sh)
+        self.x, self.g, self.xt, self.d = z(cap, n), z(cap, n), z(cap, n), z(cap, n)
+        self.lo, self.hi = z(cap, n), z(cap, n)
+        self.H = z(cap, n, n)
+        self.f, self.t, self.lam, self.edm = z(cap), z(cap), z(cap), z(cap)
+        self.phase = np.zeros(cap, dtype=np.int64)        # 0: first point pending, 1: trial pending
+        self.halvings = np.zeros(cap, dtype=np.int64)
+        self.n_evals = np.zeros(cap, dtype=np.int64)
+        self.n_hess = np.zeros(cap, dtype=np.int64)
+        self.n_iter = np.zeros(cap, dtype=np.int64)
+        self.max_evals = np.zeros(cap, dtype=np.int64)
+
+    def start(self, slot: int, x0, lower, upper, max_evals: int = 400) -> np.ndarray:
+        self.lo[slot], self.hi[slot] = lower, upper
+        self.xt[slot] = np.clip(np.asarray(x0, dtype=np.float64), lower, upper)
+        self.phase[slot] = 0
+        self.lam[slot] = 0.0
+        self.edm[slot] = np.nan
+        self.halvings[slot] = 0
+        self.n_evals[slot] = self.n_hess[slot] = self.n_iter[slot] = 0
+        self.max_evals[slot] = max_evals
+        return self.xt[slot]
+
+    def _outcome(self, sl: int, converged: bool, errors=None) -> Outcome:
+        return Outcome(self.x[sl].copy(), float(self.f[sl]), int(self.n_evals[sl]), int(self.n_iter[sl]), converged,
+                       int(self.n_hess[sl]), float(self.edm[sl]), errors)
+
+    def step(self, slots, f, g) -> Tuple[Dict[int, Outcome], List[int]]:
+        out: Dict[int, Outcome] = {}
+        need_h: List[int] = []
+        for k, sl in enumerate(np.asarray(slots, dtype=np.int64)):
+            sl = int(sl)
+            self.n_evals[sl] += 1
+            fk, gk = float(f[k]), np.asarray(g[k], dtype=np.float64)
+            if self.phase[sl] == 0:
+                self.x[sl], self.f[sl], self.g[sl] = self.xt[sl], fk, gk
+                need_h.append(sl)
+                continue
+            ok = np.isfinite(fk) and fk <= self.f[sl] + self.c1 * float(self.g[sl] @ (self.xt[sl] - self.x[sl]))
+            if ok:
+                self.x[sl], self.f[sl], self.g[sl] = self.xt[sl], fk, gk
+                self.n_iter[sl] += 1
+                self.lam[sl] *= 0.1 if self.lam[sl] > 1e-7 else 0.0
+                need_h.append(sl)
+            else:
+                self.halvings[sl] += 1
+                if self.halvings[sl] > 40:
+                    out[sl] = self._outcome(sl, False)
+                    continue
+                if self.halvings[sl] % 4 == 0:
+                    self.lam[sl] = max(10.0 * self.lam[sl], 1e-6)
+                    r = self._direction(sl)
+                    if r is not None:
+                        out[sl] = r
+                        continue
+                else:
+                    self.t[sl] *= 0.5
+                    self.xt[sl] = np.clip(self.x[sl] + self.t[sl] * self.d[sl], self.lo[sl], self.hi[sl])
+            if sl not in out and self.n_evals[sl] >= self.max_evals[sl]:
+                out[sl] = self._outcome(sl, False)
+        need_h = [s for s in need_h if s not in out]
+        for sl in need_h:
+            if self.n_evals[sl] >= self.max_evals[sl]:
+                out[sl] = self._outcome(sl, False)
+        return out, [s for s in need_h if s not in out]
+
+    def set_hessians(self, slots, H) -> Dict[int, Outcome]:
+        out: Dict[int, Outcome] = {}
+        for k, sl in enumerate(slots):
+            sl = int(sl)
+            self.H[sl] = H[k]
+            self.n_hess[sl] += 1
+            self.phase[sl] = 1
+            self.halvings[sl] = 0
+            r = self._direction(sl)
+            if r is not None:
+                out[sl] = r
+        return out
+
+    def _direction(self, sl: int) -> Optional[Outcome]:
+        x, g, H, lo, hi = self.x[sl], self.g[sl], self.H[sl], self.lo[sl], self.hi[sl]
+        dg = np.abs(np.diag(H))
+        held = ((x <= lo) & (g > 0)) | ((x >= hi) & (g < 0))
+        flat = (dg == 0.0) & (g == 0.0)
+        free = (lo < hi) & ~held & ~flat
+        idx = np.nonzero(free)[0]
+        if len(idx) == 0:
+            self.edm[sl] = 0.0
+            return self._outcome(sl, True)
+        Hf, gf, Df = H[np.ix_(idx, idx)], g[idx], np.diag(np.where(dg[idx] > 0, dg[idx], 1.0))
+        # convergence is judged on the undamped Hessian, whatever the current damping
+        try:
+            L0 = np.linalg.cholesky(Hf)
+        except np.linalg.LinAlgError:
+            L0 = None
+        if L0 is not None:
+            dn0 = -np.linalg.solve(L0.T, np.linalg.solve(L0, gf))
+            self.edm[sl] = -0.5 * float(gf @ dn0)
+            if self.edm[sl] < self.edm_tol:
+                inv = np.linalg.solve(L0.T, np.linalg.solve(L0, np.eye(len(idx))))
+                err = np.zeros(self.n)
+                err[(lo < hi) & flat] = np.nan
+                err[idx] = np.sqrt(np.diag(inv))
+                return self._outcome(sl, True, err)
+        lam = self.lam[sl]
+        L = L0 if (lam == 0.0 and L0 is not None) else None
+        for _ in range(40):
+            if L is not None:
+                break
+            try:
+                L = np.linalg.cholesky(Hf + lam * Df)
+            except np.linalg.LinAlgError:
+                lam = max(10.0 * lam, 1e-8)
+        if L is None:
+            return self._outcome(sl, False)
+        self.lam[sl] = lam
+        dn = -np.linalg.solve(L.T, np.linalg.solve(L, gf))
+        d = np.zeros(self.n)
+        d[idx] = dn
+        self.d[sl] = d
+        self.t[sl] = 1.0
+        self.xt[sl] = np.clip(x + d, lo, hi)
+        return None
+
+
 # ---------------------------------------------------------------------------
 # service
 # ---------------------------------------------------------------------------
@@ -345,6 +503,7 @@ class FitRecord:
     n_events: int
     admitted: float
     finished: float
+    arrival: float = 0.0
 
 
 @dataclass
@@ -353,6 +512,7 @@ class ServiceStats:
     latencies: List[float] = field(default_factory=list)     # per evaluation, seconds
     batch_sizes: List[int] = field(default_factory=list)
     round_seconds: List[float] = field(default_factory=list)
+    hessian_seconds: List[float] = field(default_factory=list)   # per round with Hessians
     wall: float = 0.0
     admissions_seconds: float = 0.0
 
@@ -370,6 +530,12 @@ class ServiceStats:
             "rounds": len(self.batch_sizes),
             "admission_s": self.admissions_seconds,
             "converged": int(sum(r.outcome.converged for r in self.fits)),
+            "hessians": int(sum(r.outcome.n_hess for r in self.fits)),
+            "evals_per_fit_median": float(np.median([r.outcome.n_evals for r in self.fits])) if self.fits else 0.0,
+            "hessian_round_p50_ms": float(np.percentile(self.hessian_seconds, 50) * 1e3) if self.hessian_seconds else 0.0,
+            # arrival -> result of whole fits (admission queueing included)
+            "fit_latency_p50_s": float(np.percentile([r.finished - r.arrival for r in self.fits], 50)) if self.fits else 0.0,
+            "fit_latency_p99_s": float(np.percentile([r.finished - r.arrival for r in self.fits], 99)) if self.fits else 0.0,
         }
 
 
@@ -377,7 +543,16 @@ class FitService:
     """Runs fits of one model (``plan_text``) with continuous batching on one GPU."""
 
     def __init__(self, plan_text: str, precision="fp64", max_slots: int = 64, hbm_budget: Optional[int] = None,
-                 device: int = 0, clock: Callable[[], float] = time.perf_counter):
+                 device: int = 0, clock: Callable[[], float] = time.perf_counter, method: str = "auto",
+                 edm_tol: float = 1e-5):
+        """``method``: "newton" (exact Hessians, BatchNewton), "lbfgs" (BatchLBFGS), or
+        "auto": Newton unless the model has Custom lineshapes (no Hessian)."""
+        if method == "auto":
+            method = "lbfgs" if any(l.startswith("cexpr") for l in plan_text.splitlines()) else "newton"
+        if method not in ("newton", "lbfgs"):
+            raise ValueError("method must be 'auto', 'newton' or 'lbfgs'")
+        self.method = method
+        self.edm_tol = edm_tol
         self.plan_text = plan_text
         self.precision = _native.precision_code(precision)
         self.max_slots = max_slots
@@ -391,7 +566,10 @@ class FitService:
         self.np = None
 
     def footprint(self, n3) -> int:
-        return _native.footprint_bytes(self.plan_text, n3[0], n3[1], n3[2], self.precision)
+        b = _native.footprint_bytes(self.plan_text, n3[0], n3[1], n3[2], self.precision)
+        if self.method == "newton":
+            b += _native.hessian_workspace_bytes(self.plan_text, n3[0], n3[1], n3[2])
+        return b
 
     def run(self, jobs: List[FitJob]) -> ServiceStats:
         """Serve ``jobs`` until every fit has finished.
@@ -444,7 +622,7 @@ class FitService:
         worker = threading.Thread(target=admit, daemon=True)
         worker.start()
         opt = None
-        P = O = P_host = O_host = None
+        P = O = P_host = O_host = Hd = H_host = None
         active: Dict[int, dict] = {}
         free_slots = list(range(self.max_slots))[::-1]
         admitting = True
@@ -464,7 +642,12 @@ class FitService:
                     job, h, need, nev, t_adm = item
                     if opt is None:
                         self.np = h.n_params
-                        opt = BatchLBFGS(self.max_slots, self.np)
+                        if self.method == "newton":
+                            opt = BatchNewton(self.max_slots, self.np, edm_tol=self.edm_tol)
+                            Hd = torch.zeros((self.max_slots, self.np, self.np), dtype=torch.float64, device=dev)
+                            H_host = torch.zeros((self.max_slots, self.np, self.np), dtype=torch.float64).pin_memory()
+                        else:
+                            opt = BatchLBFGS(self.max_slots, self.np)
                         P = torch.zeros((self.max_slots, self.np), dtype=torch.float64, device=dev)
                         O = torch.zeros((self.max_slots, 1 + self.np), dtype=torch.float64, device=dev)
                         P_host = torch.zeros((self.max_slots, self.np), dtype=torch.float64).pin_memory()
@@ -489,7 +672,20 @@ class FitService:
                 st.round_seconds.append(tdone - tr)
                 st.batch_sizes.append(len(slots))
                 res = O_host.numpy()
-                done = opt.step(slots, res[slots, 0], res[slots, 1:])
+                if self.method == "newton":
+                    done, need_h = opt.step(slots, res[slots, 0], res[slots, 1:])
+                    if need_h:
+                        # exact Hessians where fits moved (P[s] holds the point just evaluated)
+                        th = self.clock()
+                        with torch.cuda.stream(stream):
+                            for s in need_h:
+                                active[s]["h"].hessian(P[s].data_ptr(), Hd[s].data_ptr(), stream.cuda_stream)
+                            H_host.copy_(Hd, non_blocking=True)
+                        stream.synchronize()
+                        st.hessian_seconds.append(self.clock() - th)
+                        done.update(opt.set_hessians(need_h, H_host.numpy()[need_h]))
+                else:
+                    done = opt.step(slots, res[slots, 0], res[slots, 1:])
                 tnext = self.clock()
                 for s in slots:
                     a = active[s]
@@ -497,7 +693,8 @@ class FitService:
                     a["t_req"] = tnext
                 for s, outcome in done.items():
                     a = active.pop(s)
-                    st.fits.append(FitRecord(a["job"].job_id, outcome, a["n_events"], a["admitted"], tnext - t0))
+                    st.fits.append(FitRecord(a["job"].job_id, outcome, a["n_events"], a["admitted"], tnext - t0,
+                                             a["job"].arrival))
                     a["h"].close()
                     with cond:
                         mem["used"] -= a["need"]

@@ -23,6 +23,7 @@
  *   ctpwa_amplitudes   <- analysis::getDataTensor/getPhspTensor-style amplitude
  *                         access (src/Analysis.cu:3640-3700)
  *   ctpwa_footprint_bytes <- DeviceManager::estimate (src/DeviceManager.cu:172-221)
+ *   ctpwa_hessian_workspace_bytes <- the part of that estimate getHessian's buffers take
  */
 #ifndef CTPWA_AMD_H
 #define CTPWA_AMD_H
@@ -99,6 +100,10 @@ int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev, int iters,
 /* device bytes a handle for this plan, these event counts and this precision holds */
 long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg,
                                 int precision);
+
+/* device bytes a handle allocates on its first ctpwa_hessian call (the per-chunk
+ * Jacobian rows and partial sums it keeps until ctpwa_destroy) */
+long long ctpwa_hessian_workspace_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg);
 
 /* number of events of a class, and algorithmic bytes the fused kernel must read per
  * evaluation (SL tensors + invariants + per-event weights) */

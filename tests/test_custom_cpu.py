@@ -102,3 +102,15 @@ def test_custom_functions_finite_differences(oracle_lib):
         fm, gm = oc.nll_grad(pm)
         assert abs((fp - fm) / (2 * h) - g[j]) <= 1e-6 * max(abs(g[j]), 1.0)
         assert np.max(np.abs((gp - gm) / (2 * h) - H[j])) <= 1e-5 * np.max(np.abs(H[j]))
+
+
+def test_fused_kernel_loads_custom_factors():
+    """Custom factors are computed by their own generated kernel (ctpwa_custom); the fused
+    pass only loads their values and derivatives, so the complex-dual code never shares the
+    fused pass's registers."""
+    from ctpwa_amd import _native
+    src = _native.generate_source(plan_of(_custom("exp(-p1*m) * cos(p0*m) / sqrt(p0^2 + m^2)")).to_text(), "fp64")
+    assert "ctpwa_custom(" in src
+    body = src[src.index("void eval_body("):src.index('extern "C" __global__')]
+    assert "cd_" not in body and "cfb[" in body
+    assert "cd_exp(" in src[src.index("ctpwa_custom("):]
