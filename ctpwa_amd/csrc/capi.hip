@@ -178,6 +178,54 @@ static hipModule_t compile_eval(int device, const std::string& src) {
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
+// upload staging, one per device for the process lifetime: two pinned host buffers and
+// two device buffers of STAGE_BYTES (momenta chunks), the mixed form's FP64 batch (grown
+// on demand), a stream of default priority and an event per buffer.  Handle creations on
+// one device take turns on it.
+static const size_t STAGE_BYTES = (size_t)32 << 20;
+struct Staging {
+    std::mutex mu;
+    char* host[2] = {nullptr, nullptr};
+    double* dev[2] = {nullptr, nullptr};
+    double2* sl64 = nullptr;
+    double* inv64 = nullptr;
+    size_t n_sl64 = 0, n_inv64 = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t copied[2] = {nullptr, nullptr};    // the DMA out of host[k] has finished
+    void reserve_mixed(size_t nsl, size_t ninv) {
+        if (nsl > n_sl64) {
+            if (sl64) HIPCHECK(hipFree(sl64));
+            sl64 = nullptr;
+            HIPCHECK(hipMalloc(&sl64, nsl * sizeof(double2)));
+            n_sl64 = nsl;
+        }
+        if (ninv > n_inv64) {
+            if (inv64) HIPCHECK(hipFree(inv64));
+            inv64 = nullptr;
+            HIPCHECK(hipMalloc(&inv64, ninv * sizeof(double)));
+            n_inv64 = ninv;
+        }
+    }
+};
+static std::mutex g_stage_mu;
+static std::map<int, Staging*> g_stage;
+static Staging* staging(int device) {          // called with `device` current
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    auto it = g_stage.find(device);
+    if (it != g_stage.end()) return it->second;
+    Staging* S = new Staging();
+    for (int k = 0; k < 2; ++k) {
+        HIPCHECK(hipHostMalloc((void**)&S->host[k], STAGE_BYTES, hipHostMallocDefault));
+        HIPCHECK(hipMalloc((void**)&S->dev[k], STAGE_BYTES));
+        HIPCHECK(hipEventCreateWithFlags(&S->copied[k], hipEventDisableTiming));
+    }
+    HIPCHECK(hipStreamCreateWithFlags(&S->s, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) HIPCHECK(hipEventRecord(S->copied[k], S->s));
+    g_stage[device] = S;
+    return S;
+}
+
+// ---------------------------------------------------------------------------
 struct ctpwa_handle {
     int device = 0;
     Model_ M;
@@ -461,47 +509,37 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             double* d_cg = h->upload(cgtab);
             int* d_pol = h->upload(P.polmap);
             DInv* d_invs = h->upload(invs);
-            // stage events class by class through a bounded buffer
-            long long maxn = std::max(n_phsp, std::max(n_data, n_bkg));
-            long long stage = std::min<long long>(maxn, 1 << 20);
-            double* d_mom = nullptr;
-            double2* d_sl64 = nullptr;     // mixed: FP64 batch before rounding
-            double* d_inv64 = nullptr;
-            auto release = [&]() {
-                (void)hipFree(d_mom);
-                if (d_sl64) (void)hipFree(d_sl64);
-                if (d_inv64) (void)hipFree(d_inv64);
-            };
-            HIPCHECK(hipMalloc(&d_mom, (size_t)stage * P.nfinal * 4 * sizeof(double)));
-            try {
-                if (h->mixed) {
-                    HIPCHECK(hipMalloc(&d_sl64, (size_t)std::max(1, nu) * P.npol * stage * sizeof(double2)));
-                    HIPCHECK(hipMalloc(&d_inv64, (size_t)std::max(1, ninv) * stage * sizeof(double)));
-                }
-                for (int c = 0; c < 3; ++c) {
-                    for (long long s0 = 0; s0 < ns[c]; s0 += stage) {
-                        long long cnt = std::min(stage, ns[c] - s0);
-                        HIPCHECK(hipMemcpy(d_mom, moms[c] + (size_t)s0 * P.nfinal * 4,
-                                           (size_t)cnt * P.nfinal * 4 * sizeof(double), hipMemcpyHostToDevice));
-                        if (h->mixed) {
-                            HIPCHECK(ctpwa_launch_init(d_mom, cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
-                                                       (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
-                                                       d_sl64, d_inv64, cnt, 0, nullptr));
-                            HIPCHECK(ctpwa_launch_to_mixed(d_sl64, d_inv64, cnt, nu * P.npol, ninv, h->d_imap, h->slf,
-                                                           h->inv, h->invf, h->ntot, h->off[c] + s0, nullptr));
-                        } else {
-                            HIPCHECK(ctpwa_launch_init(d_mom, cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
-                                                       (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
-                                                       h->sl, h->inv, h->ntot, h->off[c] + s0, nullptr));
-                        }
-                        HIPCHECK(hipDeviceSynchronize());
+            // events class by class, in chunks through the device's staging pool: the
+            // host copy of a chunk into pinned memory overlaps the DMA and the init
+            // kernel of the previous one; only the pool's stream is waited for
+            Staging* S = staging(device);
+            std::lock_guard<std::mutex> lk(S->mu);
+            const size_t ev_bytes = (size_t)P.nfinal * 4 * sizeof(double);
+            const long long chunk = std::max<long long>(1, (long long)(STAGE_BYTES / ev_bytes));
+            if (h->mixed) S->reserve_mixed((size_t)std::max(1, nu) * P.npol * chunk, (size_t)std::max(1, ninv) * chunk);
+            int b = 0;
+            for (int c = 0; c < 3; ++c) {
+                for (long long s0 = 0; s0 < ns[c]; s0 += chunk) {
+                    long long cnt = std::min(chunk, ns[c] - s0);
+                    HIPCHECK(hipEventSynchronize(S->copied[b]));      // host[b] free again
+                    std::memcpy(S->host[b], moms[c] + (size_t)s0 * P.nfinal * 4, (size_t)cnt * ev_bytes);
+                    HIPCHECK(hipMemcpyAsync(S->dev[b], S->host[b], (size_t)cnt * ev_bytes, hipMemcpyHostToDevice, S->s));
+                    HIPCHECK(hipEventRecord(S->copied[b], S->s));
+                    if (h->mixed) {
+                        HIPCHECK(ctpwa_launch_init(S->dev[b], cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
+                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                   S->sl64, S->inv64, cnt, 0, S->s));
+                        HIPCHECK(ctpwa_launch_to_mixed(S->sl64, S->inv64, cnt, nu * P.npol, ninv, h->d_imap, h->slf,
+                                                       h->inv, h->invf, h->ntot, h->off[c] + s0, S->s));
+                    } else {
+                        HIPCHECK(ctpwa_launch_init(S->dev[b], cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
+                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                   h->sl, h->inv, h->ntot, h->off[c] + s0, S->s));
                     }
+                    b ^= 1;
                 }
-            } catch (...) {
-                release();
-                throw;
             }
-            release();
+            HIPCHECK(hipStreamSynchronize(S->s));
         }
 
         // model tables for prep / finalize / aux
@@ -682,7 +720,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             for (const Term& t : D.re) if (t.inv >= 0) h->used_inv[t.inv] = 1;
             for (const Term& t : D.im) if (t.inv >= 0) h->used_inv[t.inv] = 1;
         }
-        HIPCHECK(hipDeviceSynchronize());
+        HIPCHECK(hipStreamSynchronize(nullptr));
         (void)hipSetDevice(prev);
         *out = h;
         return 0;
@@ -912,24 +950,32 @@ extern "C" int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int c
 }
 
 extern "C" int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev, int iters, void* stream, double* avg_ms) {
-    if (!h || !params_dev || !avg_ms || iters <= 0) return fail("ctpwa_time_eval_kernel: bad argument");
+    if (!h || !params_dev || !avg_ms || iters <= 0 || iters > 10000) return fail("ctpwa_time_eval_kernel: bad argument");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
-    hipEvent_t a, b;
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    // one event pair around every launch: the average is over the kernels' own durations,
+    // not over the gaps between back-to-back launches
+    std::vector<hipEvent_t> ev(2 * (size_t)iters, nullptr);
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < ev.size() && e == hipSuccess; ++i) e = hipEventCreate(&ev[i]);
+    if (e == hipSuccess) e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
     if (e == hipSuccess) e = launch_custom(h, s);
-    if (e == hipSuccess) e = hipEventRecord(a, s);
-    for (int i = 0; i < iters && e == hipSuccess; ++i) e = launch_eval(h, s);
-    if (e == hipSuccess) e = hipEventRecord(b, s);
-    if (e == hipSuccess) e = hipEventSynchronize(b);
-    float ms = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    for (int i = 0; i < iters && e == hipSuccess; ++i) {
+        e = hipEventRecord(ev[2 * i], s);
+        if (e == hipSuccess) e = launch_eval(h, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[2 * i + 1], s);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(ev.back());
+    double total = 0.0;
+    for (int i = 0; i < iters && e == hipSuccess; ++i) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+        total += ms;
+    }
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
     if (e != hipSuccess) return fail(std::string("ctpwa_time_eval_kernel: ") + hipGetErrorString(e));
-    *avg_ms = ms / iters;
+    *avg_ms = total / iters;
     return 0;
 }
 
@@ -1064,10 +1110,8 @@ extern "C" long long ctpwa_footprint_bytes(const char* plan_text, long long n_ph
         long long nt = n_phsp + n_data + n_bkg;
         long long b = nt * event_bytes(M, nullptr);
         b += (long long)M.table_size() * 8 + (long long)(1 + 2 * M.P.namp + M.P.ntheta) * 1024 * 8;
-        // construction-time staging: momenta, and the FP64 batch of the mixed form
-        long long stage = std::min<long long>(std::max(n_phsp, std::max(n_data, n_bkg)), 1 << 20);
-        b += stage * M.P.nfinal * 32;
-        if (M.mixed) b += stage * ((long long)M.us.size() * M.P.npol * 16 + (long long)M.inv.size() * 8);
+        b += nt * (long long)M.customs.size() * M.ncf() * 8;     // Custom factor rows
+        // (the upload staging is one pool per device for the process, not per handle)
         return b;
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_footprint_bytes: ") + e.what();

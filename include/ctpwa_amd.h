@@ -93,11 +93,14 @@ int ctpwa_intensity(ctpwa_handle* h, const double* params_dev, int cls, double* 
 /* per-event amplitudes of class `cls` (without couplings): device [n_cls][n_pol][n_amp][2] */
 int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
-/* average duration in ms of the fused per-event kernel over `iters` launches at
- * `params_dev` (HIP events on `stream`; the prep kernel runs once before) */
+/* average duration in ms of the fused per-event kernel over `iters` (<= 10000) launches
+ * at `params_dev`: a HIP event pair on `stream` around every launch; the prep kernel (and
+ * the Custom factor kernel) run once before */
 int ctpwa_time_eval_kernel(ctpwa_handle* h, const double* params_dev, int iters, void* stream, double* avg_ms);
 
-/* device bytes a handle for this plan, these event counts and this precision holds */
+/* device bytes a handle for this plan, these event counts and this precision holds (the
+ * upload staging, 2 x 32 MiB pinned host and device buffers plus the mixed form's FP64
+ * chunk, is allocated once per device for the process and shared by all handles) */
 long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg,
                                 int precision);
 
