@@ -36,7 +36,8 @@ extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, in
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
 
-extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, long long, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_zero(double*, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, const double*, int, int, const int2*, int, int,
                                           double*, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, const double*, int, double*, int, hipStream_t);
@@ -865,7 +866,7 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         const Plan& P = M.P;
         HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, M.maxloc, 1, s));
         size_t per_class = (size_t)h->hnG + h->hnUu + h->hNZ + h->hNU;
-        HIPCHECK(hipMemsetAsync(h->htot, 0, 3 * per_class * sizeof(double), s));
+        HIPCHECK(ctpwa_launch_zero(h->htot, (long long)(3 * per_class), s));
         HessArgs a{};
         a.st = h->store(); a.table = h->table;
         a.off_v = (long long)M.off_v(); a.off_rec = (long long)M.off_rec();
@@ -884,7 +885,7 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
             a.wconst = h->wc[c];
             for (long long s0 = 0; s0 < h->n[c]; s0 += h->hchunk) {
                 long long cnt = std::min(h->hchunk, h->n[c] - s0);
-                HIPCHECK(ctpwa_launch_hess_rows(&a, h->off[c] + s0, cnt, s));
+                HIPCHECK(ctpwa_launch_hess_rows(&a, h->off[c] + s0, cnt, M.maxloc, s));
                 for (int e0 = 0; e0 < h->hnG; e0 += 2048) {
                     int ne = std::min(2048, h->hnG - e0);
                     HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, cnt * P.npol, h->hW, 2, P.npol, h->hklG, e0, ne,

@@ -44,20 +44,27 @@ __device__ inline double rec_h(const HessArgs& a, int rec, int i, int k) {
 
 // propagator product of a block and its log-derivatives:
 //   d Psi / d theta_i = -Psi L_i,   d2 Psi = Psi (L_i L_k - M_ik)
-__device__ void block_terms(const HessArgs& a, const DBlock& B, long long e, Cd& psi, Cd* L, Cd* Mm) {
-    int nl = B.nloc;
+// NL bounds the block's local variables (B.nloc <= NL): every array is indexed with
+// unrolled constant indices and lives in registers.
+template <int NL>
+__device__ inline void block_terms(const HessArgs& a, const DBlock& B, long long e, Cd& psi, Cd* L, Cd* Mm) {
+    const int nl = B.nloc;
     psi = {1.0, 0.0};
-    for (int i = 0; i < nl; ++i) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
         L[i] = {0.0, 0.0};
-        for (int k = 0; k < nl; ++k) Mm[i * NLOC + k] = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < NL; ++k) Mm[i * NL + k] = {0.0, 0.0};
     }
     for (int dd = 0; dd < B.nden; ++dd) {
         const DDen& D = a.dens[B.den0 + dd];
         const DDenTerms& T = a.dterms[B.den0 + dd];
-        Cd Dv{0.0, 0.0}, dD[NLOC], d2D[NLOC * NLOC];
-        for (int i = 0; i < nl; ++i) {
+        Cd Dv{0.0, 0.0}, dD[NL], d2D[NL * NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
             dD[i] = {0.0, 0.0};
-            for (int k = 0; k < nl; ++k) d2D[i * NLOC + k] = {0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < NL; ++k) d2D[i * NL + k] = {0.0, 0.0};
         }
         for (int t = 0; t < T.nre + T.nim; ++t) {
             const DTerm& tm = a.terms[T.t0 + t];
@@ -66,22 +73,31 @@ __device__ void block_terms(const HessArgs& a, const DBlock& B, long long e, Cd&
             double c = (tm.coef < 0) ? tm.cval : rec_v(a, D.rec + tm.coef);
             if (im) Dv.i -= c * x; else Dv.r += c * x;
             if (tm.coef < 0) continue;
-            for (int i = 0; i < nl; ++i) {
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+                if (i >= nl) break;
                 double gi = rec_g(a, D.rec + tm.coef, i) * x;
                 if (im) dD[i].i -= gi; else dD[i].r += gi;
-                for (int k = 0; k < nl; ++k) {
+#pragma unroll
+                for (int k = 0; k < NL; ++k) {
+                    if (k >= nl) break;
                     double hik = rec_h(a, D.rec + tm.coef, i, k) * x;
-                    if (im) d2D[i * NLOC + k].i -= hik; else d2D[i * NLOC + k].r += hik;
+                    if (im) d2D[i * NL + k].i -= hik; else d2D[i * NL + k].r += hik;
                 }
             }
         }
         Cd pd = cinv(Dv);
         psi = cm(psi, pd);
-        for (int i = 0; i < nl; ++i) {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            if (i >= nl) break;
             Cd li = cm(pd, dD[i]);
             L[i] = ca(L[i], li);
-            for (int k = 0; k < nl; ++k)
-                Mm[i * NLOC + k] = ca(Mm[i * NLOC + k], cs(cm(pd, d2D[i * NLOC + k]), cm(li, cm(pd, dD[k]))));
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                if (k >= nl) break;
+                Mm[i * NL + k] = ca(Mm[i * NL + k], cs(cm(pd, d2D[i * NL + k]), cm(li, cm(pd, dD[k]))));
+            }
         }
     }
 }
@@ -92,46 +108,59 @@ __device__ inline Cd amp_dF(const HessArgs& a, int rec, double K, Cd psi, const 
     Cd t = cs(Cd{rec_g(a, rec, i), 0.0}, csc(L[i], K));
     return cm(psi, t);
 }
+template <int NL>
 __device__ inline Cd amp_d2F(const HessArgs& a, int rec, double K, Cd psi, const Cd* L, const Cd* Mm, int i, int k) {
     Cd t{rec_h(a, rec, i, k), 0.0};
     t = cs(t, csc(L[k], rec_g(a, rec, i)));
     t = cs(t, csc(L[i], rec_g(a, rec, k)));
-    t = ca(t, csc(cs(cm(L[i], L[k]), Mm[i * NLOC + k]), K));
+    t = ca(t, csc(cs(cm(L[i], L[k]), Mm[i * NL + k]), K));
     return cm(psi, t);
 }
 
+// one event per thread.  NL: block-size class (max local variables, 1/2/4/8); NPT: the
+// number of polarisations when it is 1..4 (S in registers), 0 for any other (S in
+// private memory, runtime bounds)
+template <int NL, int NPT>
 __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, long long n) {
     long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     long long e = e0 + r;
-    const int C = a.C, NP = a.npol, na = a.namp;
+    constexpr int SP = NPT > 0 ? NPT : MAXPOL;
+    const int NP = NPT > 0 ? NPT : a.npol;
+    const int C = a.C, na = a.namp;
     double* Jr = a.J + (size_t)r * NP * 2 * C;
-    Cd S[MAXPOL];
-    for (int p = 0; p < NP; ++p) S[p] = {0.0, 0.0};
+    Cd S[SP];
+#pragma unroll
+    for (int p = 0; p < SP; ++p) S[p] = {0.0, 0.0};
     for (int p = 0; p < NP; ++p)
         for (int c = 0; c < 2 * C; ++c) Jr[(size_t)p * 2 * C + c] = 0.0;
-    Cd psi, L[NLOC], Mm[NLOC * NLOC];
+    Cd psi, L[NL], Mm[NL * NL];
     // pass 1: A columns, S, dS/dtheta columns
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
-        block_terms(a, B, e, psi, L, Mm);
+        block_terms<NL>(a, B, e, psi, L, Mm);
         for (int c = 0; c < B.nsl; ++c) {
             int am = B.site + c;
             const DAmp& A = a.amps[am];
             double K = rec_v(a, A.rec);
             Cd F = amp_F(K, psi);
             Cd v{a.table[a.off_v + 2 * am], a.table[a.off_v + 2 * am + 1]};
-            Cd dF[NLOC];
-            for (int i = 0; i < B.nloc; ++i) dF[i] = cm(v, amp_dF(a, A.rec, K, psi, L, i));
-            for (int p = 0; p < NP; ++p) {
+            Cd dF[NL];
+#pragma unroll
+            for (int i = 0; i < NL; ++i) dF[i] = (i < B.nloc) ? cm(v, amp_dF(a, A.rec, K, psi, L, i)) : Cd{0.0, 0.0};
+#pragma unroll
+            for (int p = 0; p < SP; ++p) {
+                if (p >= NP) break;
                 double2 s = ev_sl(a.st, (long long)A.u * NP + p, e);
                 Cd sp{s.x, s.y};
                 Cd Aap = cm(F, sp);
                 Jr[(size_t)p * 2 * C + am] = Aap.r;
                 Jr[(size_t)p * 2 * C + C + am] = Aap.i;
                 S[p] = ca(S[p], cm(v, Aap));
-                for (int i = 0; i < B.nloc; ++i) {
+#pragma unroll
+                for (int i = 0; i < NL; ++i) {
+                    if (i >= B.nloc) break;
                     int j = B.vars[i];
                     Cd d = cm(dF[i], sp);
                     Jr[(size_t)p * 2 * C + na + j] += d.r;
@@ -141,7 +170,11 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
         }
     }
     double I = 0.0;
-    for (int p = 0; p < NP; ++p) I += S[p].r * S[p].r + S[p].i * S[p].i;
+#pragma unroll
+    for (int p = 0; p < SP; ++p) {
+        if (p >= NP) break;
+        I += S[p].r * S[p].r + S[p].i * S[p].i;
+    }
     double w = (a.cls == 0) ? 1.0 : (a.wts ? a.wts[e] : a.wconst);
     double alpha, beta;
     if (a.cls == 0) { alpha = 1.0; beta = 0.0; }
@@ -157,9 +190,10 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
-        block_terms(a, B, e, psi, L, Mm);
-        double Rb[NLOC * NLOC];
-        for (int q = 0; q < NLOC * NLOC; ++q) Rb[q] = 0.0;
+        block_terms<NL>(a, B, e, psi, L, Mm);
+        double Rb[NL * NL];
+#pragma unroll
+        for (int q = 0; q < NL * NL; ++q) Rb[q] = 0.0;
         for (int c = 0; c < B.nsl; ++c) {
             int am = B.site + c;
             const DAmp& A = a.amps[am];
@@ -168,7 +202,9 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             Cd v{a.table[a.off_v + 2 * am], a.table[a.off_v + 2 * am + 1]};
             // T_u = sum_p conj(sl) S ; conj(T_u) = <S, sl>
             Cd T{0.0, 0.0};
-            for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int p = 0; p < SP; ++p) {
+                if (p >= NP) break;
                 double2 s = ev_sl(a.st, (long long)A.u * NP + p, e);
                 T = ca(T, cm(Cd{s.x, -s.y}, S[p]));
             }
@@ -176,21 +212,34 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             Cd g = cm(ccj(F), T);                     // dI/dv_a* = <A_a, S>
             Ur[am] = 2.0 * g.r;
             Ur[na + am] = 2.0 * g.i;
-            for (int i = 0; i < B.nloc; ++i) {
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+                if (i >= B.nloc) break;
                 Cd dF = amp_dF(a, A.rec, K, psi, L, i);
                 Cd q = cm(dF, cT);                    // <S, dA_a/dtheta_i>
                 Zr[zq++] = q.r;
                 Zr[zq++] = q.i;
                 Cd vq = cm(v, q);                     // contributes to dI/dtheta
                 Ur[2 * na + B.vars[i]] += 2.0 * vq.r;
-                for (int k = i; k < B.nloc; ++k) {
-                    Cd d2 = cm(v, cm(amp_d2F(a, A.rec, K, psi, L, Mm, i, k), cT));
-                    Rb[i * NLOC + k] += d2.r;
+#pragma unroll
+                for (int k = 0; k < NL; ++k) {
+                    if (k < i) continue;
+                    if (k >= B.nloc) break;
+                    Cd d2 = cm(v, cm(amp_d2F<NL>(a, A.rec, K, psi, L, Mm, i, k), cT));
+                    Rb[i * NL + k] += d2.r;
                 }
             }
         }
-        for (int i = 0; i < B.nloc; ++i)
-            for (int k = i; k < B.nloc; ++k) Zr[zq++] = Rb[i * NLOC + k];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            if (i >= B.nloc) break;
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                if (k < i) continue;
+                if (k >= B.nloc) break;
+                Zr[zq++] = Rb[i * NL + k];
+            }
+        }
     }
     Zr[zq++] = I;
 }
@@ -244,9 +293,16 @@ k_wcolsum(const double* __restrict__ X, int ncol, long long nrows, const double*
     long long per = (nrows + nwg - 1) / nwg;
     long long r0 = (long long)wg * per, r1 = r0 + per < nrows ? r0 + per : nrows;
     for (int c = threadIdx.x; c < ncol; c += GR_THREADS) {
-        double s = 0.0;
-        for (long long r = r0; r < r1; ++r) s += (Wt ? Wt[r * wstride] : 1.0) * X[r * ncol + c];
-        part[(long long)wg * ncol + c] = s;
+        // eight interleaved partial sums (independent loads in flight), combined in a
+        // fixed order: the result does not depend on timing
+        double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        long long r = r0;
+        for (; r + 8 <= r1; r += 8) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += (Wt ? Wt[(r + k) * wstride] : 1.0) * X[(r + k) * ncol + c];
+        }
+        for (int k = 0; r < r1; ++r, ++k) s[k] += (Wt ? Wt[r * wstride] : 1.0) * X[r * ncol + c];
+        part[(long long)wg * ncol + c] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
     }
 }
 
@@ -254,9 +310,14 @@ k_wcolsum(const double* __restrict__ X, int ncol, long long nrows, const double*
 __global__ void k_accum_parts(const double* __restrict__ part, int nwg, int ncol, double* __restrict__ tot) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncol) return;
-    double s = 0.0;
-    for (int g = 0; g < nwg; ++g) s += part[(long long)g * ncol + c];
-    tot[c] += s;
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int g = 0;
+    for (; g + 8 <= nwg; g += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += part[(long long)(g + k) * ncol + c];
+    }
+    for (int k = 0; g < nwg; ++g, ++k) s[k] += part[(long long)g * ncol + c];
+    tot[c] += ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 
@@ -382,12 +443,39 @@ __global__ void __launch_bounds__(256) k_hess_assemble(HessAsm h) {
 
 using namespace ctpwa;
 
+template <int NL>
+static void launch_rows_np(const HessArgs* a, long long e0, long long n, dim3 g, dim3 b, hipStream_t s) {
+    switch (a->npol) {
+    case 1: hipLaunchKernelGGL((k_hess_rows<NL, 1>), g, b, 0, s, *a, e0, n); break;
+    case 2: hipLaunchKernelGGL((k_hess_rows<NL, 2>), g, b, 0, s, *a, e0, n); break;
+    case 3: hipLaunchKernelGGL((k_hess_rows<NL, 3>), g, b, 0, s, *a, e0, n); break;
+    case 4: hipLaunchKernelGGL((k_hess_rows<NL, 4>), g, b, 0, s, *a, e0, n); break;
+    default: hipLaunchKernelGGL((k_hess_rows<NL, 0>), g, b, 0, s, *a, e0, n); break;
+    }
+}
+
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_hess_rows(const HessArgs* a, long long e0,
-                                                                                   long long n, hipStream_t s) {
+                                                                                   long long n, int maxloc, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     int bs = 128;
-    long long nb = (n + bs - 1) / bs;
-    hipLaunchKernelGGL(k_hess_rows, dim3((unsigned)nb), dim3(bs), 0, s, *a, e0, n);
+    dim3 g((unsigned)((n + bs - 1) / bs)), b(bs);
+    if (maxloc <= 1) launch_rows_np<1>(a, e0, n, g, b, s);
+    else if (maxloc <= 2) launch_rows_np<2>(a, e0, n, g, b, s);
+    else if (maxloc <= 4) launch_rows_np<4>(a, e0, n, g, b, s);
+    else launch_rows_np<8>(a, e0, n, g, b, s);
+    return hipGetLastError();
+}
+
+__global__ void k_zero(double* __restrict__ p, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+// zero fill by a kernel (not hipMemsetAsync: the call may be captured in a graph)
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_zero(double* p, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    unsigned nb = (unsigned)std::min<long long>(1024, (n + 255) / 256);
+    hipLaunchKernelGGL(k_zero, dim3(nb), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 
