@@ -38,9 +38,10 @@ extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long lo
 
 extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, long long, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_zero(double*, long long, hipStream_t);
-extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, const double*, int, int, const int2*, int, int,
-                                          double*, int, hipStream_t);
-extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, const double*, int, double*, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, long long, const double*, long long,
+                                          const int2*, int, int, double*, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, long long, const double*, double*, int,
+                                            hipStream_t);
 extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_hess_assemble(const ctpwa::HessAsm*, hipStream_t);
 
@@ -872,7 +873,8 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         a.off_v = (long long)M.off_v(); a.off_rec = (long long)M.off_rec();
         a.amps = h->aux.amps; a.blocks = h->aux.blocks; a.dens = h->aux.dens; a.dterms = h->aux.dterms; a.terms = h->aux.terms;
         a.namp = P.namp; a.nblock = (int)P.blocks.size(); a.npol = P.npol; a.ntheta = P.ntheta;
-        a.J = h->hJ; a.U = h->hU; a.Z = h->hZ; a.W = h->hW; a.C = h->hC; a.NU = h->hNU; a.NZ = h->hNZ;
+        a.J = h->hJ; a.U = h->hU; a.Z = h->hZ; a.W = h->hW; a.ld = h->hchunk;
+        a.C = h->hC; a.NU = h->hNU; a.NZ = h->hNZ;
         HessAsm as{};
         for (int c = 0; c < 3; ++c) {
             double* base = h->htot + c * per_class;
@@ -888,21 +890,21 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
                 HIPCHECK(ctpwa_launch_hess_rows(&a, h->off[c] + s0, cnt, M.maxloc, s));
                 for (int e0 = 0; e0 < h->hnG; e0 += 2048) {
                     int ne = std::min(2048, h->hnG - e0);
-                    HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, cnt * P.npol, h->hW, 2, P.npol, h->hklG, e0, ne,
-                                                h->hpart, h->hnwg, s));
+                    HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, h->hchunk * P.npol, cnt * P.npol, h->hW, cnt,
+                                                h->hklG, e0, ne, h->hpart, h->hnwg, s));
                     HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.G[c] + e0, s));
                 }
                 if (c > 0) {
                     for (int e0 = 0; e0 < h->hnUu; e0 += 2048) {
                         int ne = std::min(2048, h->hnUu - e0);
-                        HIPCHECK(ctpwa_launch_wgram(h->hU, h->hNU, cnt, h->hW + 1, 2, 1, h->hklU, e0, ne, h->hpart,
-                                                    h->hnwg, s));
+                        HIPCHECK(ctpwa_launch_wgram(h->hU, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hklU,
+                                                    e0, ne, h->hpart, h->hnwg, s));
                         HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.Uu[c] + e0, s));
                     }
                 }
-                HIPCHECK(ctpwa_launch_wcolsum(h->hZ, h->hNZ, cnt, h->hW, 2, h->hpart, h->hnwg, s));
+                HIPCHECK(ctpwa_launch_wcolsum(h->hZ, h->hNZ, h->hchunk, cnt, h->hW, h->hpart, h->hnwg, s));
                 HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, h->hNZ, (double*)as.Zs[c], s));
-                HIPCHECK(ctpwa_launch_wcolsum(h->hU, h->hNU, cnt, nullptr, 0, h->hpart, h->hnwg, s));
+                HIPCHECK(ctpwa_launch_wcolsum(h->hU, h->hNU, h->hchunk, cnt, nullptr, h->hpart, h->hnwg, s));
                 HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, h->hNU, (double*)as.gs[c], s));
             }
         }

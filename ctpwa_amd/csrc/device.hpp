@@ -163,11 +163,13 @@ struct HessArgs {
     const double* wts;       // per-event (global index) or null
     double wconst;
     int cls;
-    // outputs (chunk-local rows)
-    double* J;               // [rows = n_ev * npol][2C]
-    double* U;               // [n_ev][NU = 2 namp + ntheta]
-    double* Z;               // [n_ev][NZ]
-    double* W;               // [n_ev][2] alpha, beta
+    // outputs, column-major over the chunk's rows (a wave's store of one column is one
+    // contiguous run): row R = p * n_ev + r (polarisation p of chunk event r) for J
+    double* J;               // [2C][ld * npol]
+    double* U;               // [NU = 2 namp + ntheta][ld]
+    double* Z;               // [NZ][ld]
+    double* W;               // [2][ld] alpha, beta
+    long long ld;            // row capacity of a chunk
     int C, NU, NZ;
 };
 

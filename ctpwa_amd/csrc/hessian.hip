@@ -128,12 +128,17 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     constexpr int SP = NPT > 0 ? NPT : MAXPOL;
     const int NP = NPT > 0 ? NPT : a.npol;
     const int C = a.C, na = a.namp;
-    double* Jr = a.J + (size_t)r * NP * 2 * C;
+    const long long ldJ = a.ld * NP;
+    // column c, polarisation p of this event: Jc[c * ldJ + p * n]
+    double* Jc = a.J + r;
     Cd S[SP];
 #pragma unroll
     for (int p = 0; p < SP; ++p) S[p] = {0.0, 0.0};
     for (int p = 0; p < NP; ++p)
-        for (int c = 0; c < 2 * C; ++c) Jr[(size_t)p * 2 * C + c] = 0.0;
+        for (int c = na; c < C; ++c) {
+            Jc[c * ldJ + p * n] = 0.0;
+            Jc[(C + c) * ldJ + p * n] = 0.0;
+        }
     Cd psi, L[NL], Mm[NL * NL];
     // pass 1: A columns, S, dS/dtheta columns
     for (int b = 0; b < a.nblock; ++b) {
@@ -155,16 +160,16 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
                 double2 s = ev_sl(a.st, (long long)A.u * NP + p, e);
                 Cd sp{s.x, s.y};
                 Cd Aap = cm(F, sp);
-                Jr[(size_t)p * 2 * C + am] = Aap.r;
-                Jr[(size_t)p * 2 * C + C + am] = Aap.i;
+                Jc[am * ldJ + p * n] = Aap.r;
+                Jc[(C + am) * ldJ + p * n] = Aap.i;
                 S[p] = ca(S[p], cm(v, Aap));
 #pragma unroll
                 for (int i = 0; i < NL; ++i) {
                     if (i >= B.nloc) break;
                     int j = B.vars[i];
                     Cd d = cm(dF[i], sp);
-                    Jr[(size_t)p * 2 * C + na + j] += d.r;
-                    Jr[(size_t)p * 2 * C + C + na + j] += d.i;
+                    Jc[(na + j) * ldJ + p * n] += d.r;
+                    Jc[(C + na + j) * ldJ + p * n] += d.i;
                 }
             }
         }
@@ -180,12 +185,13 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     if (a.cls == 0) { alpha = 1.0; beta = 0.0; }
     else if (I == 0.0) { alpha = 0.0; beta = 0.0; }
     else { alpha = w / I; beta = -w / (I * I); }
-    a.W[2 * r] = alpha;
-    a.W[2 * r + 1] = beta;
-    // pass 2: u = dI/dx and the second-part scalars
-    double* Ur = a.U + (size_t)r * a.NU;
-    double* Zr = a.Z + (size_t)r * a.NZ;
-    for (int j = 0; j < a.ntheta; ++j) Ur[2 * na + j] = 0.0;
+    a.W[r] = alpha;
+    a.W[a.ld + r] = beta;
+    // pass 2: u = dI/dx and the second-part scalars (column k of this event: X[k * ld])
+    double* Ur = a.U + r;
+    double* Zr = a.Z + r;
+    const long long ld = a.ld;
+    for (int j = 0; j < a.ntheta; ++j) Ur[(2 * na + j) * ld] = 0.0;
     int zq = 0;
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
@@ -210,17 +216,17 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             }
             Cd cT = ccj(T);
             Cd g = cm(ccj(F), T);                     // dI/dv_a* = <A_a, S>
-            Ur[am] = 2.0 * g.r;
-            Ur[na + am] = 2.0 * g.i;
+            Ur[am * ld] = 2.0 * g.r;
+            Ur[(na + am) * ld] = 2.0 * g.i;
 #pragma unroll
             for (int i = 0; i < NL; ++i) {
                 if (i >= B.nloc) break;
                 Cd dF = amp_dF(a, A.rec, K, psi, L, i);
                 Cd q = cm(dF, cT);                    // <S, dA_a/dtheta_i>
-                Zr[zq++] = q.r;
-                Zr[zq++] = q.i;
+                Zr[(zq++) * ld] = q.r;
+                Zr[(zq++) * ld] = q.i;
                 Cd vq = cm(v, q);                     // contributes to dI/dtheta
-                Ur[2 * na + B.vars[i]] += 2.0 * vq.r;
+                Ur[(2 * na + B.vars[i]) * ld] += 2.0 * vq.r;
 #pragma unroll
                 for (int k = 0; k < NL; ++k) {
                     if (k < i) continue;
@@ -237,22 +243,24 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             for (int k = 0; k < NL; ++k) {
                 if (k < i) continue;
                 if (k >= B.nloc) break;
-                Zr[zq++] = Rb[i * NL + k];
+                Zr[(zq++) * ld] = Rb[i * NL + k];
             }
         }
     }
-    Zr[zq++] = I;
+    Zr[(zq++) * ld] = I;
 }
 
-// weighted Gram of a row-major matrix: part[wg][entry] = sum_rows w(row/rep) X[row][k] X[row][l]
-// for the entries [ent0, ent0 + nent) of the upper-triangle list kl[].
+// weighted Gram of a column-major matrix X[col][row] (column stride ldx, rows
+// [0, nrows)): part[wg][entry] = sum_rows w(row) X[k][row] X[l][row] for the entries
+// [ent0, ent0 + nent) of the upper-triangle list kl[]; the weight of row R is
+// Wt[R mod nev] (rows p * nev + r share event r's weight)
 constexpr int GR_THREADS = 256;
 constexpr int GR_EPT = 8;
 constexpr int GR_TILE = 16;
 
 __global__ void __launch_bounds__(GR_THREADS)
-k_wgram(const double* __restrict__ X, int ncol, long long nrows, const double* __restrict__ Wt, int wstride,
-        int rep, const int2* __restrict__ kl, int ent0, int nent, double* __restrict__ part) {
+k_wgram(const double* __restrict__ X, int ncol, long long ldx, long long nrows, const double* __restrict__ Wt,
+        long long nev, const int2* __restrict__ kl, int ent0, int nent, double* __restrict__ part) {
     extern __shared__ double tile[];        // [GR_TILE][ncol] + [GR_TILE] weights
     double* wt = tile + GR_TILE * ncol;
     int nwg = gridDim.x, wg = blockIdx.x;
@@ -268,9 +276,12 @@ k_wgram(const double* __restrict__ X, int ncol, long long nrows, const double* _
     for (long long t0 = r0; t0 < r1; t0 += GR_TILE) {
         int nt = (int)((r1 - t0) < GR_TILE ? (r1 - t0) : GR_TILE);
         __syncthreads();
-        for (int q = threadIdx.x; q < nt * ncol; q += GR_THREADS)
-            tile[q] = X[(t0 + q / ncol) * (long long)ncol + q % ncol];
-        for (int q = threadIdx.x; q < nt; q += GR_THREADS) wt[q] = Wt[((t0 + q) / rep) * wstride];
+        // consecutive threads: consecutive rows of one column
+        for (int q = threadIdx.x; q < nt * ncol; q += GR_THREADS) {
+            int col = q / nt, rr = q - col * nt;
+            tile[rr * ncol + col] = X[(long long)col * ldx + t0 + rr];
+        }
+        for (int q = threadIdx.x; q < nt; q += GR_THREADS) wt[q] = Wt[(t0 + q) % nev];
         __syncthreads();
         for (int q = 0; q < GR_EPT; ++q) {
             if (ids[q].x < 0) continue;
@@ -285,24 +296,28 @@ k_wgram(const double* __restrict__ X, int ncol, long long nrows, const double* _
     }
 }
 
-// weighted column sums: part[wg][c] = sum_rows w(row) X[row][c]  (w == null: weight 1)
+// weighted column sums of a column-major X[col][row]: part[wg][c] = sum_rows w(row) X[c][row]
+// (w == null: weight 1).  The threads of a workgroup stride over its rows (coalesced) and
+// the per-thread sums are combined by a fixed LDS tree.
 __global__ void __launch_bounds__(GR_THREADS)
-k_wcolsum(const double* __restrict__ X, int ncol, long long nrows, const double* __restrict__ Wt, int wstride,
+k_wcolsum(const double* __restrict__ X, int ncol, long long ldx, long long nrows, const double* __restrict__ Wt,
           double* __restrict__ part) {
+    __shared__ double sh[GR_THREADS];
     int nwg = gridDim.x, wg = blockIdx.x;
     long long per = (nrows + nwg - 1) / nwg;
     long long r0 = (long long)wg * per, r1 = r0 + per < nrows ? r0 + per : nrows;
-    for (int c = threadIdx.x; c < ncol; c += GR_THREADS) {
-        // eight interleaved partial sums (independent loads in flight), combined in a
-        // fixed order: the result does not depend on timing
-        double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        long long r = r0;
-        for (; r + 8 <= r1; r += 8) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) s[k] += (Wt ? Wt[(r + k) * wstride] : 1.0) * X[(r + k) * ncol + c];
+    for (int c = 0; c < ncol; ++c) {
+        const double* x = X + (long long)c * ldx;
+        double s = 0.0;
+        for (long long r = r0 + threadIdx.x; r < r1; r += GR_THREADS) s += (Wt ? Wt[r] : 1.0) * x[r];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = GR_THREADS / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
         }
-        for (int k = 0; r < r1; ++r, ++k) s[k] += (Wt ? Wt[r * wstride] : 1.0) * X[r * ncol + c];
-        part[(long long)wg * ncol + c] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+        if (threadIdx.x == 0) part[(long long)wg * ncol + c] = sh[0];
+        __syncthreads();
     }
 }
 
@@ -479,19 +494,20 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_zero(do
     return hipGetLastError();
 }
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wgram(const double* X, int ncol, long long nrows,
-                                                                               const double* Wt, int wstride, int rep,
-                                                                               const int2* kl, int ent0, int nent,
-                                                                               double* part, int nwg, hipStream_t s) {
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wgram(const double* X, int ncol, long long ldx,
+                                                                               long long nrows, const double* Wt,
+                                                                               long long nev, const int2* kl, int ent0,
+                                                                               int nent, double* part, int nwg,
+                                                                               hipStream_t s) {
     size_t sh = (size_t)(GR_TILE * ncol + GR_TILE) * sizeof(double);
-    hipLaunchKernelGGL(k_wgram, dim3(nwg), dim3(GR_THREADS), sh, s, X, ncol, nrows, Wt, wstride, rep, kl, ent0, nent, part);
+    hipLaunchKernelGGL(k_wgram, dim3(nwg), dim3(GR_THREADS), sh, s, X, ncol, ldx, nrows, Wt, nev, kl, ent0, nent, part);
     return hipGetLastError();
 }
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wcolsum(const double* X, int ncol, long long nrows,
-                                                                                 const double* Wt, int wstride,
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wcolsum(const double* X, int ncol, long long ldx,
+                                                                                 long long nrows, const double* Wt,
                                                                                  double* part, int nwg, hipStream_t s) {
-    hipLaunchKernelGGL(k_wcolsum, dim3(nwg), dim3(GR_THREADS), 0, s, X, ncol, nrows, Wt, wstride, part);
+    hipLaunchKernelGGL(k_wcolsum, dim3(nwg), dim3(GR_THREADS), 0, s, X, ncol, ldx, nrows, Wt, part);
     return hipGetLastError();
 }
 
