@@ -43,6 +43,10 @@ extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, long lon
 extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, long long, const double*, double*, int,
                                             hipStream_t);
 extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
+extern "C" int ctpwa_mfma_gram_ok(int);
+extern "C" long long ctpwa_mfma_gram_part_doubles(int, int);
+extern "C" hipError_t ctpwa_launch_wgram_mfma(const double*, int, long long, long long, const double*, long long,
+                                               const int2*, int, double*, int, double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_hess_assemble(const ctpwa::HessAsm*, hipStream_t);
 
 using namespace ctpwa;
@@ -260,6 +264,7 @@ struct ctpwa_handle {
     long long hchunk = 0;
     int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0, hnwg = 0;
     double *hJ = nullptr, *hU = nullptr, *hZ = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
+    double* hmpart = nullptr;              // per-wave tile partials of the matrix-core Gram
     int2 *hklG = nullptr, *hklU = nullptr;
     std::vector<int> used_inv;
     // storage precision (0 FP64, 1 mixed) and the FP32 stores of the mixed form
@@ -812,6 +817,13 @@ static HessDims hess_dims(const Model_& M) {
 }
 static const long long HESS_CHUNK = 1 << 19;     // events per Jacobian pass
 static const int HESS_NWG = 512;                 // split-K workgroups of the Gram
+static const int HESS_MFMA_NWG = 512;            // workgroups of the matrix-core Gram
+static long long hess_mfma_part_doubles(const HessDims& d) {
+    long long n = 0;
+    if (ctpwa_mfma_gram_ok(2 * d.C)) n = std::max(n, ctpwa_mfma_gram_part_doubles(2 * d.C, HESS_MFMA_NWG));
+    if (ctpwa_mfma_gram_ok(d.NU)) n = std::max(n, ctpwa_mfma_gram_part_doubles(d.NU, HESS_MFMA_NWG));
+    return n;
+}
 static long long hess_workspace_bytes(const Model_& M, long long maxn) {
     HessDims d = hess_dims(M);
     long long chunk = std::min<long long>(maxn, HESS_CHUNK);
@@ -819,7 +831,8 @@ static long long hess_workspace_bytes(const Model_& M, long long maxn) {
     long long maxcol = std::max<long long>(2048, std::max(d.NU, d.NZ));
     long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + d.NZ + 2) + HESS_NWG * maxcol + 3 * (nG + nUu + d.NZ + d.NU) +
                   (long long)d.NU * d.NU;
-    return w * 8 + (nG + nUu) * 8 + 9 * 256;     // + the index lists and allocation slack
+    w += hess_mfma_part_doubles(d);
+    return w * 8 + (nG + nUu) * 8 + 10 * 256;    // + the index lists and allocation slack
 }
 
 static void hess_setup(ctpwa_handle* h) {
@@ -853,6 +866,7 @@ static void hess_setup(ctpwa_handle* h) {
     // totals: per class G, Uu, Z, g
     h->htot = h->alloc<double>((size_t)3 * (h->hnG + h->hnUu + h->hNZ + h->hNU));
     h->hHext = h->alloc<double>((size_t)h->hNU * h->hNU);
+    h->hmpart = h->alloc<double>((size_t)std::max<long long>(1, hess_mfma_part_doubles(hd)));
     h->hess_ready = true;
 }
 
@@ -888,13 +902,21 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
             for (long long s0 = 0; s0 < h->n[c]; s0 += h->hchunk) {
                 long long cnt = std::min(h->hchunk, h->n[c] - s0);
                 HIPCHECK(ctpwa_launch_hess_rows(&a, h->off[c] + s0, cnt, M.maxloc, s));
-                for (int e0 = 0; e0 < h->hnG; e0 += 2048) {
-                    int ne = std::min(2048, h->hnG - e0);
-                    HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, h->hchunk * P.npol, cnt * P.npol, h->hW, cnt,
-                                                h->hklG, e0, ne, h->hpart, h->hnwg, s));
-                    HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.G[c] + e0, s));
+                if (ctpwa_mfma_gram_ok(2 * h->hC)) {
+                    HIPCHECK(ctpwa_launch_wgram_mfma(h->hJ, 2 * h->hC, h->hchunk * P.npol, cnt * P.npol, h->hW, cnt,
+                                                     h->hklG, h->hnG, h->hmpart, HESS_MFMA_NWG, (double*)as.G[c], s));
+                } else {
+                    for (int e0 = 0; e0 < h->hnG; e0 += 2048) {
+                        int ne = std::min(2048, h->hnG - e0);
+                        HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, h->hchunk * P.npol, cnt * P.npol, h->hW, cnt,
+                                                    h->hklG, e0, ne, h->hpart, h->hnwg, s));
+                        HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.G[c] + e0, s));
+                    }
                 }
-                if (c > 0) {
+                if (c > 0 && ctpwa_mfma_gram_ok(h->hNU)) {
+                    HIPCHECK(ctpwa_launch_wgram_mfma(h->hU, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hklU,
+                                                     h->hnUu, h->hmpart, HESS_MFMA_NWG, (double*)as.Uu[c], s));
+                } else if (c > 0) {
                     for (int e0 = 0; e0 < h->hnUu; e0 += 2048) {
                         int ne = std::min(2048, h->hnUu - e0);
                         HIPCHECK(ctpwa_launch_wgram(h->hU, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hklU,

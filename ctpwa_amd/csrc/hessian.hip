@@ -22,6 +22,8 @@
 // and maps it to the fit parameters (v_a = r_a p_f(a)).
 //
 // Events are processed in chunks; every sum is a fixed-order reduction.
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace ctpwa {
@@ -296,27 +298,192 @@ k_wgram(const double* __restrict__ X, int ncol, long long ldx, long long nrows, 
     }
 }
 
+// The same weighted Gram on the matrix cores, for matrices of at most MG_MAXCOL
+// columns: v_mfma_f64_16x16x4 accumulates the 16x16 tiles (I <= J) of the padded Gram,
+// D[i][j] += sum_k (w_k X[k][16 I + i]) X[k][16 J + j] over 4 rows k per instruction.
+// Every wave owns a contiguous range of rows, stages 32-row slabs of all columns in its
+// own LDS region (column stride padded by one double against bank conflicts) and keeps
+// its tiles in registers; it writes one partial per tile, and k_accum_mfma sums the
+// partials in a fixed order.  Lane layout of the instruction (16x16x4, f64; measured on
+// the device): A[i][k] in lane 16 k + i, B[k][j] in lane 16 k + j, D[4 r + q][j] in
+// component r of lane 16 q + j.
+constexpr int MG_WAVES = 4;                  // waves per workgroup
+constexpr int MG_ROWS = 32;                  // rows per LDS slab
+constexpr int MG_MAXCOL = 64;                // padded columns (4 tiles of 16)
+constexpr int MG_MAXPAIR = 10;               // tile pairs I <= J of a 4 x 4 tiling
+typedef double mg_acc_t __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(64 * MG_WAVES)
+k_wgram_mfma(const double* __restrict__ X, int ncol, long long ldx, long long nrows, const double* __restrict__ Wt,
+             long long nev, double* __restrict__ part) {
+    extern __shared__ double smem[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int T = (ncol + 15) >> 4, ncp = T * 16, stride = MG_ROWS + 1;
+    const int npair = T * (T + 1) / 2;
+    double* xs = smem + (size_t)wave * (ncp * stride + MG_ROWS);
+    double* ws = xs + ncp * stride;
+    // tile pairs in row-major upper-triangle order: LDS offsets of their A and B columns
+    int aoff[MG_MAXPAIR], boff[MG_MAXPAIR];
+    {
+        int q = 0;
+        for (int I = 0; I < 4; ++I)
+            for (int J = I; J < 4; ++J) {
+                if (q < MG_MAXPAIR) {
+                    aoff[q] = (I * 16 + (lane & 15)) * stride;
+                    boff[q] = (J * 16 + (lane & 15)) * stride;
+                }
+                if (I < T && J < T) ++q;
+            }
+    }
+    // this wave's rows: the grid's waves split [0, nrows) into ranges of whole slabs
+    const long long nw = (long long)gridDim.x * MG_WAVES;
+    const long long gw = (long long)blockIdx.x * MG_WAVES + wave;
+    const long long per = ((nrows + nw - 1) / nw + MG_ROWS - 1) / MG_ROWS * MG_ROWS;
+    const long long r0 = gw * per < nrows ? gw * per : nrows;
+    const long long r1 = r0 + per < nrows ? r0 + per : nrows;
+    mg_acc_t acc[MG_MAXPAIR];
+#pragma unroll
+    for (int q = 0; q < MG_MAXPAIR; ++q) acc[q] = (mg_acc_t){0.0, 0.0, 0.0, 0.0};
+    // software pipeline: the next slab is loaded into registers while the current one is
+    // multiplied from LDS.  Lane l always carries row (l & 31) of columns 2 it + (l >> 5).
+    const int rr = lane & 31, ch = lane >> 5, nit = ncp >> 1;
+    double pre[MG_MAXCOL / 2];
+    double wpre = 0.0;
+    auto load = [&](long long t0) {
+        const int nt = (int)((r1 - t0) < MG_ROWS ? (r1 - t0) : MG_ROWS);
+#pragma unroll
+        for (int it = 0; it < MG_MAXCOL / 2; ++it) {
+            const int c = 2 * it + ch;
+            pre[it] = (it < nit && c < ncol && rr < nt) ? X[(long long)c * ldx + t0 + rr] : 0.0;
+        }
+        wpre = (ch == 0 && rr < nt) ? Wt[(t0 + rr) % nev] : 0.0;
+    };
+    if (r0 < r1) load(r0);
+    for (long long t0 = r0; t0 < r1; t0 += MG_ROWS) {
+#pragma unroll
+        for (int it = 0; it < MG_MAXCOL / 2; ++it)
+            if (it < nit) xs[(2 * it + ch) * stride + rr] = pre[it];
+        if (ch == 0) ws[rr] = wpre;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (t0 + MG_ROWS < r1) load(t0 + MG_ROWS);
+#pragma unroll
+        for (int k0 = 0; k0 < MG_ROWS; k0 += 4) {
+            const int kr = k0 + (lane >> 4);
+            const double w = ws[kr];
+#pragma unroll
+            for (int q = 0; q < MG_MAXPAIR; ++q) {
+                if (q < npair) {
+                    const double av = w * xs[aoff[q] + kr];
+                    const double bv = xs[boff[q] + kr];
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[q], 0, 0, 0);
+                }
+            }
+        }
+        // the slab is consumed before the next one overwrites it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    // the workgroup's partial: waves 1..3 leave their tiles in LDS (the slab regions are
+    // free once every wave has finished), wave 0 adds them in wave order and writes
+    // part[(blockIdx.x * npair + q) * 256 + 16 i + j]
+    __syncthreads();
+    if (wave > 0) {
+        double* o = smem + (size_t)(wave - 1) * npair * 256;
+#pragma unroll
+        for (int q = 0; q < MG_MAXPAIR; ++q)
+            if (q < npair) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) o[q * 256 + (4 * r + (lane >> 4)) * 16 + (lane & 15)] = acc[q][r];
+            }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < MG_MAXPAIR; ++q)
+            if (q < npair) {
+                double* o = part + ((size_t)blockIdx.x * npair + q) * 256;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = q * 256 + (4 * r + (lane >> 4)) * 16 + (lane & 15);
+                    double v = acc[q][r];
+                    for (int w = 1; w < MG_WAVES; ++w) v += smem[(size_t)(w - 1) * npair * 256 + e];
+                    o[e - q * 256] = v;
+                }
+            }
+    }
+}
+
+// tot[idx(a, b)] += sum over the nwg workgroup partials of k_wgram_mfma, for every
+// entry a <= b < ncol of the Gram (packed upper triangle, row-major).  Each workgroup
+// takes 64 consecutive tile elements (one per lane, coalesced reads); its four waves add
+// the partials g = w, w + 4, ... with eight interleaved sums combined by a fixed tree,
+// and wave 0 adds the waves' sums in wave order: a fixed order throughout.
+__global__ void __launch_bounds__(256) k_accum_mfma(const double* __restrict__ part, int nwg, int ncol,
+                                                    double* __restrict__ tot) {
+    __shared__ double sh[4][64];
+    const int T = (ncol + 15) >> 4, npair = T * (T + 1) / 2, E = npair * 256;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (e < E) {
+        int g = wave;
+        for (; g + 28 < nwg; g += 32) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s[k] += part[(size_t)(g + 4 * k) * E + e];
+        }
+        for (; g < nwg; g += 4) s[0] += part[(size_t)g * E + e];
+    }
+    sh[wave][lane] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    __syncthreads();
+    if (wave == 0 && e < E) {
+        const double v = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+        // element e of tile pair q = (I, J) (row-major upper-triangle order)
+        int q = e >> 8, I = 0;
+        while (q >= T - I) { q -= T - I; ++I; }
+        const int J = I + q;
+        const int a = I * 16 + ((e >> 4) & 15), b = J * 16 + (e & 15);
+        if (a < ncol && b < ncol && a <= b) tot[(long long)a * ncol - (long long)a * (a - 1) / 2 + (b - a)] += v;
+    }
+}
+
 // weighted column sums of a column-major X[col][row]: part[wg][c] = sum_rows w(row) X[c][row]
-// (w == null: weight 1).  The threads of a workgroup stride over its rows (coalesced) and
-// the per-thread sums are combined by a fixed LDS tree.
+// (w == null: weight 1).  The threads of a workgroup stride over its rows (coalesced),
+// eight columns per pass; per-thread sums are combined by wave shuffles and then the
+// waves' sums in wave order: a fixed order
 __global__ void __launch_bounds__(GR_THREADS)
 k_wcolsum(const double* __restrict__ X, int ncol, long long ldx, long long nrows, const double* __restrict__ Wt,
           double* __restrict__ part) {
-    __shared__ double sh[GR_THREADS];
+    constexpr int CB = 8, NWV = GR_THREADS / 64;
+    __shared__ double sh[NWV][CB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int nwg = gridDim.x, wg = blockIdx.x;
     long long per = (nrows + nwg - 1) / nwg;
     long long r0 = (long long)wg * per, r1 = r0 + per < nrows ? r0 + per : nrows;
-    for (int c = 0; c < ncol; ++c) {
-        const double* x = X + (long long)c * ldx;
-        double s = 0.0;
-        for (long long r = r0 + threadIdx.x; r < r1; r += GR_THREADS) s += (Wt ? Wt[r] : 1.0) * x[r];
-        sh[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = GR_THREADS / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-            __syncthreads();
+    for (int c0 = 0; c0 < ncol; c0 += CB) {
+        double s[CB];
+#pragma unroll
+        for (int k = 0; k < CB; ++k) s[k] = 0.0;
+        for (long long r = r0 + threadIdx.x; r < r1; r += GR_THREADS) {
+            const double w = Wt ? Wt[r] : 1.0;
+#pragma unroll
+            for (int k = 0; k < CB; ++k)
+                if (c0 + k < ncol) s[k] += w * X[(long long)(c0 + k) * ldx + r];
         }
-        if (threadIdx.x == 0) part[(long long)wg * ncol + c] = sh[0];
+#pragma unroll
+        for (int k = 0; k < CB; ++k)
+            for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < CB; ++k) sh[wave][k] = s[k];
+        __syncthreads();
+        if (threadIdx.x < CB && c0 + threadIdx.x < ncol) {
+            double v = sh[0][threadIdx.x];
+            for (int w = 1; w < NWV; ++w) v += sh[w][threadIdx.x];
+            part[(long long)wg * ncol + c0 + threadIdx.x] = v;
+        }
         __syncthreads();
     }
 }
@@ -501,6 +668,31 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wgram(c
                                                                                hipStream_t s) {
     size_t sh = (size_t)(GR_TILE * ncol + GR_TILE) * sizeof(double);
     hipLaunchKernelGGL(k_wgram, dim3(nwg), dim3(GR_THREADS), sh, s, X, ncol, ldx, nrows, Wt, nev, kl, ent0, nent, part);
+    return hipGetLastError();
+}
+
+// weighted Gram on the matrix cores (ncol <= MG_MAXCOL): nwg workgroups of MG_WAVES waves,
+// partials [nwg * MG_WAVES][pairs][256] in part, summed into tot (packed upper triangle)
+extern "C" __attribute__((visibility("hidden"))) int ctpwa_mfma_gram_ok(int ncol) { return ncol <= MG_MAXCOL; }
+extern "C" __attribute__((visibility("hidden"))) long long ctpwa_mfma_gram_part_doubles(int ncol, int nwg) {
+    const int T = (ncol + 15) / 16;
+    return (long long)nwg * (T * (T + 1) / 2) * 256;
+}
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wgram_mfma(const double* X, int ncol, long long ldx,
+                                                                                    long long nrows, const double* Wt,
+                                                                                    long long nev, const int2* kl, int nent,
+                                                                                    double* part, int nwg, double* tot,
+                                                                                    hipStream_t s) {
+    if (ncol > MG_MAXCOL || ncol <= 0) return hipErrorInvalidValue;
+    const int ncp = (ncol + 15) / 16 * 16, T = ncp / 16;
+    // slab regions of the waves, or the tiles of waves 1..3 when the workgroup combines them
+    size_t sh = std::max((size_t)MG_WAVES * (ncp * (MG_ROWS + 1) + MG_ROWS), (size_t)(MG_WAVES - 1) * (T * (T + 1) / 2) * 256) *
+                sizeof(double);
+    hipLaunchKernelGGL(k_wgram_mfma, dim3(nwg), dim3(64 * MG_WAVES), sh, s, X, ncol, ldx, nrows, Wt, nev, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int E = (T * (T + 1) / 2) * 256;
+    hipLaunchKernelGGL(k_accum_mfma, dim3((E + 63) / 64), dim3(256), 0, s, part, nwg, ncol, tot);
     return hipGetLastError();
 }
 
