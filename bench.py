@@ -137,7 +137,19 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
             "algorithmic_bytes": lay["bytes_per_eval"], "algorithmic_bytes_def": bdef,
             "per_amplitude_bytes_formula": survey_bytes}
     log("[bench] %s fused kernel %.4f ms, %.1f GB/s of %d B" % (prec, kern_ms, achieved, lay["bytes_per_eval"]))
-    return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof}
+    hess_ms = None
+    if args.hessian_reps > 0:
+        # getHessian on the same workload (not part of `value`): one warm-up, then the mean
+        ph = pts[0].detach()
+        ana.getHessian(ph)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.hessian_reps):
+            ana.getHessian(ph)
+        torch.cuda.synchronize()
+        hess_ms = (time.perf_counter() - t) / args.hessian_reps * 1e3
+        log("[bench] %s getHessian %.3f ms" % (prec, hess_ms))
+    return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof, "hessian_ms": hess_ms}
 
 
 def run_serving(args, samples, plan, spec, W, precisions):
@@ -189,6 +201,8 @@ def main():
     ap.add_argument("--serving-method", choices=["auto", "newton", "lbfgs"], default="auto",
                     help="minimiser of the toy fits (auto: Newton with the GPU Hessian)")
     ap.add_argument("--serving-slots", type=int, default=64, help="fits evaluated together at most")
+    ap.add_argument("--hessian-reps", type=int, default=10,
+                    help="getHessian calls timed per form on the same workload (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
                          "known size that calibrates the HBM byte counters of a --pmc profile)")
@@ -278,6 +292,10 @@ def main():
     }
     if compare is not None:
         out["fp64"] = compare
+    if args.hessian_reps > 0:
+        out["hessian"] = {"call": "analysis.getHessian on the same workload, mean of %d after one warm-up"
+                                  % args.hessian_reps,
+                          "ms": {p: round(results[p]["hessian_ms"], 4) for p in precisions}}
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)
