@@ -13,6 +13,9 @@ pybind11 module (src/main.cu:90-131) and ``class analysis`` (src/Analysis.cu:126
 * ``getNVector() / getNFreeTheta() / getNParams() / getParamNames() /
   getFreeResParams() / getFitMode() / setFitMode() / getNPolarizations() /
   getAmplitudeNames() / isValid() / reCalcAmp(theta)``.
+* ``getFitFractions(vector, hessian)`` and ``writeResult(params, filename,
+  is_saved_weight, waves)``: post-fit outputs from the amplitude Gram
+  (ctpwa_amd/results.py); ``writeResult`` writes a NumPy ``.npz`` instead of a ROOT file.
 
 All device work runs on the current torch stream; nothing synchronises the host
 except reading results.
@@ -114,6 +117,11 @@ class analysis:
             device = torch.cuda.current_device() if torch.cuda.is_available() else 0
         self.device = torch.device("cuda", device)
         self.n_events = n
+        self._samples = s
+        self._weights = w
+        self._truth = None
+        self._theta = np.asarray(self.plan.theta_init() if hasattr(self.plan, "theta_init") else
+                                 [t.init for t in self.plan.theta], dtype=np.float64)
         self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
                                  w["phsp"], w["data"], w["bkg"], device=device, precision=precision)
         self._valid = True
@@ -129,6 +137,7 @@ class analysis:
     def getHessian(self, params: torch.Tensor) -> torch.Tensor:
         self._check_params(params)
         p = params.detach().contiguous()
+        self._remember_theta(p)
         n = self.getNParams()
         H = torch.empty((n, n), dtype=torch.float64, device=self.device)
         self._h.hessian(p.data_ptr(), H.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
@@ -174,7 +183,76 @@ class analysis:
         if t.numel() != self.plan.n_theta:
             raise RuntimeError("theta must have %d entries" % self.plan.n_theta)
 
+    def getFitFractions(self, vector: torch.Tensor, hessian: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Fit fractions FF_b = integral |P_b|^2 / sum_j integral |P_j|^2 over the
+        ``phsp_truth`` sample, with errors sqrt(diag(J C J^T)) (src/Analysis.cu:3527-3638).
+
+        ``vector``: the free couplings as a complex128 tensor [n_free] on the device (the
+        resonance parameters are those of the last getNLL/getHessian call, else their
+        initial values), or the whole float64 parameter vector.  ``hessian``: optional
+        full Hessian (its coupling block gives C); computed at the point when omitted.
+        Returns [n_partials, 2] float64 on the CPU: fraction, error."""
+        from . import results as R
+        params = self._params_from_vector(vector)
+        Q = self._truth_gram(params)
+        pn = params.detach().cpu().numpy()
+        ff = R.fit_fractions(Q, self.plan, pn)
+        if hessian is None:
+            hessian = self.getHessian(params)
+        J = R.fit_fraction_jacobian(Q, self.plan, pn)
+        err = R.fit_fraction_errors(J, hessian.detach().cpu().numpy(), self.plan.n_free)
+        return torch.tensor(np.stack([ff, err], axis=1), dtype=torch.float64)
+
+    def writeResult(self, params: torch.Tensor, filename: str, is_saved_weight: int = 0, waves=()) -> str:
+        """Fit result over the phase-space sample as a NumPy .npz (the reference writes ROOT,
+        src/Analysis.cu:1355-1700): ``legends`` (one label per partial wave),
+        ``interference`` (symmetric: sum |P_b|^2 / total on the diagonal,
+        sum 2 Re(conj(P_b) P_b') / total off it), ``phsp_integral``, ``data_integral``;
+        with ``is_saved_weight=1`` also per-event ``totalweight`` and ``weight_<k>`` (k-th
+        partial wave), normalised to the data yield, and the final-state momenta
+        ``<particle>_{px,py,pz,E}``.  Returns the file written."""
+        from . import results as R
+        self._check_params(params)
+        if len(waves):
+            raise RuntimeError("writeResult: wave subsets are not built")
+        p = params.detach().contiguous()
+        Q = self._gram(self._h, p, 0)
+        X, total = R.interference_matrix(Q, self.plan, p.cpu().numpy())
+        nd = float(np.sum(self._weights["data"])) if self._weights["data"] is not None else float(self.n_events["data"])
+        di = R.data_integral(nd, self._weights["bkg"], self.n_events["bkg"])
+        out = {"legends": np.array(R.partial_names(self.plan)), "interference": X,
+               "phsp_integral": np.float64(total), "data_integral": np.float64(di)}
+        if is_saved_weight == 1:
+            norm = di / (total if total > 0 else 1e-30)
+            w = self.getPartialIntensities(p, 0).cpu().numpy() * norm
+            out["totalweight"] = w[-1]
+            for k in range(w.shape[0] - 1):
+                out["weight_%d" % k] = w[k]
+            mom = self._samples["phsp"]
+            for i, name in enumerate(self.plan.final_order):       # events hold (E, px, py, pz)
+                for c, comp in ((1, "px"), (2, "py"), (3, "pz"), (0, "E")):
+                    out["%s_%s" % (name, comp)] = np.ascontiguousarray(mom[:, i, c])
+        path = filename if filename.endswith(".npz") else os.path.splitext(filename)[0] + ".npz"
+        np.savez(path, **out)
+        return path
+
     # ---- extensions ----------------------------------------------------------
+    def getPartialIntensities(self, params: torch.Tensor, cls: int = 0) -> torch.Tensor:
+        """[n_partials + 1, N] float64 on the device: per event sum_pol |P_b|^2 for every
+        partial wave b, then the total intensity."""
+        self._check_params(params)
+        n = self.n_events[("phsp", "data", "bkg")[cls]]
+        out = torch.empty((len(self.plan.blocks) + 1, n), dtype=torch.float64, device=self.device)
+        if n:
+            self._h.partial_intensities(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
+                                        torch.cuda.current_stream(self.device).cuda_stream)
+        return out
+
+    def getAmplitudeGram(self, params: torch.Tensor, cls: int = 0) -> np.ndarray:
+        """complex [n_amp, n_amp] sum over a sample of conj(A_a) A_a' (no couplings)."""
+        self._check_params(params)
+        return self._gram(self._h, params.detach().contiguous(), cls)
+
     def getIntensity(self, params: torch.Tensor, cls: int = 1) -> torch.Tensor:
         """Per-event intensity of a sample (0 phsp, 1 data, 2 bkg)."""
         self._check_params(params)
@@ -214,8 +292,45 @@ class analysis:
         if params.dim() != 1 or params.numel() != self.getNParams():
             raise RuntimeError("params must be 1-D with %d entries" % self.getNParams())
 
+    def _remember_theta(self, p: torch.Tensor):
+        self._theta = p[2 * self.plan.n_free:].detach().cpu().numpy().copy()
+
+    def _params_from_vector(self, vector: torch.Tensor) -> torch.Tensor:
+        if not isinstance(vector, torch.Tensor) or not vector.is_cuda or vector.device != self.device:
+            raise RuntimeError("vector must be a tensor on %s" % self.device)
+        if vector.dtype == torch.float64 and vector.dim() == 1 and vector.numel() == self.getNParams():
+            return vector.detach().contiguous()
+        if vector.dtype != torch.complex128 or vector.numel() != self.plan.n_free:
+            raise RuntimeError("vector must be complex128 with %d entries (or the float64 parameters)"
+                               % self.plan.n_free)
+        v = vector.detach().reshape(-1)
+        th = torch.tensor(self._theta, dtype=torch.float64, device=self.device)
+        return torch.cat([v.real.to(torch.float64), v.imag.to(torch.float64), th]).contiguous()
+
+    def _gram(self, handle, p: torch.Tensor, cls: int) -> np.ndarray:
+        na = self.plan.n_amp
+        out = torch.empty((na, na, 2), dtype=torch.float64, device=self.device)
+        handle.amplitude_gram(p.data_ptr(), cls, out.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
+        o = out.cpu().numpy()
+        return o[..., 0] + 1j * o[..., 1]
+
+    def _truth_gram(self, p: torch.Tensor) -> np.ndarray:
+        """Amplitude Gram over the phsp_truth sample (config ``Data: phsp_truth`` or
+        ``samples["phsp_truth"]``); its handle is built on first use."""
+        if self._truth is None:
+            truth = self._samples.get("phsp_truth")
+            df = self.cfg.data_files
+            if truth is None and "phsp_truth" in df:
+                truth = load_sample(df["phsp_truth"], self.cfg.base_dir, len(self.plan.final_order))
+            if truth is None:
+                raise RuntimeError("No phsp_truth in config")
+            self._truth = _native.Handle(self.plan.to_text(), truth, truth[:1], None, device=self.device.index,
+                                         precision=self.precision)
+        return self._gram(self._truth, p, 0)
+
     def _nll_grad_tensor(self, params: torch.Tensor) -> torch.Tensor:
         p = params.detach().contiguous()
+        self._remember_theta(p)
         out = torch.empty(1 + self.getNParams(), dtype=torch.float64, device=self.device)
         self._h.nll_grad(p.data_ptr(), out.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
         return out

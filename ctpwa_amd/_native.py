@@ -22,6 +22,7 @@ EXPORTS = [
     "ctpwa_footprint_bytes", "ctpwa_hessian_workspace_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
     "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
+    "ctpwa_amplitude_gram", "ctpwa_partial_intensities",
 ]
 
 _lib = None
@@ -47,7 +48,7 @@ def lib():
     L.ctpwa_param_info.argtypes = [_vp, _dp, _dp, _dp]
     for f in ("ctpwa_nll_grad", "ctpwa_hessian"):
         getattr(L, f).argtypes = [_vp, _vp, _vp, _vp]
-    for f in ("ctpwa_intensity", "ctpwa_amplitudes"):
+    for f in ("ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_amplitude_gram", "ctpwa_partial_intensities"):
         getattr(L, f).argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     L.ctpwa_time_eval_kernel.argtypes = [_vp, _vp, ctypes.c_int, _vp, _dp]
     L.ctpwa_footprint_bytes.restype = _ll
@@ -198,6 +199,16 @@ class Handle:
     def amplitudes(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
         check(lib().ctpwa_amplitudes(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
                                      ctypes.c_void_p(stream)), "ctpwa_amplitudes")
+
+    def amplitude_gram(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
+        """complex [n_amp][n_amp] sum over the class of conj(A_l) A_k (no couplings)"""
+        check(lib().ctpwa_amplitude_gram(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
+                                         ctypes.c_void_p(stream)), "ctpwa_amplitude_gram")
+
+    def partial_intensities(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
+        """[n_block + 1][n_cls]: per-event partial-wave intensities, then the total"""
+        check(lib().ctpwa_partial_intensities(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
+                                              ctypes.c_void_p(stream)), "ctpwa_partial_intensities")
 
     def time_eval_kernel(self, params_ptr: int, iters: int, stream: int) -> float:
         ms = ctypes.c_double()

@@ -44,6 +44,9 @@ extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, long l
                                             hipStream_t);
 extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
 extern "C" int ctpwa_mfma_gram_ok(int);
+extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs*, int, long long, long long, double*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_gram_to_complex(const double*, int, double*, hipStream_t);
 extern "C" long long ctpwa_mfma_gram_part_doubles(int, int);
 extern "C" hipError_t ctpwa_launch_wgram_mfma(const double*, int, long long, long long, const double*, long long,
                                                const int2*, int, double*, int, double*, hipStream_t);
@@ -265,6 +268,12 @@ struct ctpwa_handle {
     int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0, hnwg = 0;
     double *hJ = nullptr, *hU = nullptr, *hZ = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
     double* hmpart = nullptr;              // per-wave tile partials of the matrix-core Gram
+    // amplitude-Gram workspace (allocated on the first ctpwa_amplitude_gram call)
+    bool agram_ready = false;
+    long long gchunk = 0;
+    int gnG = 0;
+    double *gX = nullptr, *gW = nullptr, *gpart = nullptr, *gtot = nullptr;
+    int2* gkl = nullptr;
     int2 *hklG = nullptr, *hklU = nullptr;
     std::vector<int> used_inv;
     // storage precision (0 FP64, 1 mixed) and the FP32 stores of the mixed form
@@ -868,6 +877,71 @@ static void hess_setup(ctpwa_handle* h) {
     h->hHext = h->alloc<double>((size_t)h->hNU * h->hNU);
     h->hmpart = h->alloc<double>((size_t)std::max<long long>(1, hess_mfma_part_doubles(hd)));
     h->hess_ready = true;
+}
+
+// amplitude Gram of one class: Q[l][k] = sum_e sum_p conj(A_l) A_k (amplitudes without
+// couplings), the integrals behind fit fractions and the interference matrix
+static void agram_setup(ctpwa_handle* h) {
+    const Plan& P = h->M.P;
+    const int n2 = 2 * P.namp;
+    long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
+    h->gchunk = std::max<long long>(1, std::min<long long>(maxn, 1 << 18));
+    h->gX = h->alloc<double>((size_t)h->gchunk * P.npol * n2);
+    h->gW = h->upload(std::vector<double>((size_t)h->gchunk, 1.0));
+    auto kl = upper_triangle(n2);
+    h->gnG = (int)kl.size();
+    h->gkl = h->upload(kl);
+    if (ctpwa_mfma_gram_ok(n2)) h->gpart = h->alloc<double>((size_t)ctpwa_mfma_gram_part_doubles(n2, HESS_MFMA_NWG));
+    else h->gpart = h->alloc<double>((size_t)HESS_NWG * 2048);
+    h->gtot = h->alloc<double>((size_t)h->gnG);
+    h->agram_ready = true;
+}
+
+extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
+    if (!h || !params_dev || !out_dev) return fail("ctpwa_amplitude_gram: NULL argument");
+    if (cls < 0 || cls > 2) return fail("ctpwa_amplitude_gram: class must be 0 (phsp), 1 (data) or 2 (bkg)");
+    if (!h->M.customs.empty()) return fail("ctpwa_amplitude_gram: models with Custom lineshapes are not built");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    try {
+        if (!h->agram_ready) agram_setup(h);
+        const Plan& P = h->M.P;
+        const int n2 = 2 * P.namp;
+        const long long ldx = h->gchunk * P.npol;
+        HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s));
+        HIPCHECK(ctpwa_launch_zero(h->gtot, h->gnG, s));
+        for (long long s0 = 0; s0 < h->n[cls]; s0 += h->gchunk) {
+            long long cnt = std::min(h->gchunk, h->n[cls] - s0);
+            HIPCHECK(ctpwa_launch_amp_rows(&h->aux, h->off[cls] + s0, cnt, h->gX, ldx, s));
+            if (ctpwa_mfma_gram_ok(n2)) {
+                HIPCHECK(ctpwa_launch_wgram_mfma(h->gX, n2, ldx, cnt * P.npol, h->gW, cnt, h->gkl, h->gnG, h->gpart,
+                                                 HESS_MFMA_NWG, h->gtot, s));
+            } else {
+                for (int e0 = 0; e0 < h->gnG; e0 += 2048) {
+                    int ne = std::min(2048, h->gnG - e0);
+                    HIPCHECK(ctpwa_launch_wgram(h->gX, n2, ldx, cnt * P.npol, h->gW, cnt, h->gkl, e0, ne, h->gpart, HESS_NWG, s));
+                    HIPCHECK(ctpwa_launch_accum(h->gpart, HESS_NWG, ne, h->gtot + e0, s));
+                }
+            }
+        }
+        HIPCHECK(ctpwa_launch_gram_to_complex(h->gtot, P.namp, out_dev, s));
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(std::string("ctpwa_amplitude_gram: ") + e.what());
+    }
+}
+
+extern "C" int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
+    if (!h || !params_dev || !out_dev) return fail("ctpwa_partial_intensities: NULL argument");
+    if (cls < 0 || cls > 2) return fail("ctpwa_partial_intensities: class must be 0 (phsp), 1 (data) or 2 (bkg)");
+    if (!h->M.customs.empty()) return fail("ctpwa_partial_intensities: models with Custom lineshapes are not built");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess)
+        e = ctpwa_launch_partial_int(&h->aux, (int)h->M.P.blocks.size(), h->off[cls], h->n[cls], out_dev, s);
+    if (e != hipSuccess) return fail(std::string("ctpwa_partial_intensities: ") + hipGetErrorString(e));
+    return 0;
 }
 
 extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream) {

@@ -538,6 +538,16 @@ __device__ double d2I_sum(const HessAsm& h, int c, int x, int y) {
     return v;
 }
 
+// complex Gram Q[l][k] = sum conj(A_l) A_k (n x n, interleaved) from the packed real Gram G
+// of the 2n columns [Re A | Im A]
+__global__ void k_gram_to_complex(const double* __restrict__ G, int n, double* __restrict__ Q) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n * n) return;
+    const int l = q / n, k = q - l * n, n2 = 2 * n;
+    Q[2 * q] = gram_at(G, n2, l, k) + gram_at(G, n2, n + l, n + k);
+    Q[2 * q + 1] = gram_at(G, n2, l, n + k) - gram_at(G, n2, n + l, k);
+}
+
 __global__ void __launch_bounds__(256) k_hess_assemble(HessAsm h) {
     const int n = h.namp, NU = h.NU, P = h.ntheta;
     // phase-space mean
@@ -706,6 +716,12 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wcolsum
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_accum(const double* part, int nwg, int ncol,
                                                                                double* tot, hipStream_t s) {
     hipLaunchKernelGGL(k_accum_parts, dim3((ncol + 255) / 256), dim3(256), 0, s, part, nwg, ncol, tot);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_gram_to_complex(const double* G, int n, double* Q,
+                                                                                        hipStream_t s) {
+    hipLaunchKernelGGL(k_gram_to_complex, dim3((n * n + 255) / 256), dim3(256), 0, s, G, n, Q);
     return hipGetLastError();
 }
 

@@ -483,6 +483,62 @@ __global__ void k_amplitudes(AuxArgs a, long long e0, long long n, double* __res
     }
 }
 
+// amplitude rows for Grams of the amplitudes (fit fractions, interference): column-major
+// X[col][R], R = p * n + i (polarisation p of event e0 + i), columns Re A_a (a < namp)
+// then Im A_a; A_a = K_a Psi_b sl_u,p without couplings
+__global__ void k_amp_rows(AuxArgs a, long long e0, long long n, double* __restrict__ X, long long ldx) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long e = e0 + i;
+    for (int am = 0; am < a.namp; ++am) {
+        const DAmp& A = a.amps[am];
+        const DBlock& B = a.blocks[A.block];
+        double2 psi = block_psi(a, B, e);
+        double K = a.table[a.off_rec + (long long)A.rec * REC];
+        for (int p = 0; p < a.npol; ++p) {
+            double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
+            X[(long long)am * ldx + p * n + i] = K * (psi.x * s.x - psi.y * s.y);
+            X[(long long)(a.namp + am) * ldx + p * n + i] = K * (psi.x * s.y + psi.y * s.x);
+        }
+    }
+}
+
+// per-event partial-wave intensities: out[b][i] = sum_p |P_b(e, p)|^2 for the nblock
+// partial waves (blocks) and out[nblock][i] = I(e) = sum_p |sum_b P_b|^2, with
+// P_b = Psi_b sum_{a in b} c_a sl_u(a),p and c_a = v_a K_a (src/ComputeResults.cu:57-190)
+__global__ void k_partial_int(AuxArgs a, int nblock, long long e0, long long n, double* __restrict__ out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long e = e0 + i;
+    double Sr[MAXPOL], Si[MAXPOL];
+    for (int p = 0; p < a.npol; ++p) { Sr[p] = 0.0; Si[p] = 0.0; }
+    for (int b = 0; b < nblock; ++b) {
+        const DBlock& B = a.blocks[b];
+        double w = 0.0;
+        if (B.nsl > 0) {
+            double2 psi = block_psi(a, B, e);
+            for (int p = 0; p < a.npol; ++p) {
+                double yr = 0.0, yi = 0.0;
+                for (int c = 0; c < B.nsl; ++c) {
+                    const int am = B.site + c;
+                    const double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
+                    double2 s = ev_sl(a.st, (long long)a.amps[am].u * a.npol + p, e);
+                    yr += cr * s.x - ci * s.y;
+                    yi += cr * s.y + ci * s.x;
+                }
+                const double pr = psi.x * yr - psi.y * yi, pi = psi.x * yi + psi.y * yr;
+                w += pr * pr + pi * pi;
+                Sr[p] += pr;
+                Si[p] += pi;
+            }
+        }
+        out[(long long)b * n + i] = w;
+    }
+    double I = 0.0;
+    for (int p = 0; p < a.npol; ++p) I += Sr[p] * Sr[p] + Si[p] * Si[p];
+    out[(long long)nblock * n + i] = I;
+}
+
 __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __restrict__ out) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -565,6 +621,23 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finaliz
     if (e != hipSuccess) return e;
     size_t shmem = (size_t)f->nacc * sizeof(double);
     hipLaunchKernelGGL(ctpwa::k_assemble, dim3(1), dim3(ctpwa::ASM_THREADS), shmem, stream, *f);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs* a, long long e0,
+                                                                                  long long n, double* X, long long ldx,
+                                                                                  hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctpwa::k_amp_rows, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, e0, n, X, ldx);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs* a, int nblock,
+                                                                                     long long e0, long long n, double* out,
+                                                                                     hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctpwa::k_partial_int, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, nblock, e0, n,
+                       out);
     return hipGetLastError();
 }
 

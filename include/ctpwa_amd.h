@@ -24,6 +24,8 @@
  *                         access (src/Analysis.cu:3640-3700)
  *   ctpwa_footprint_bytes <- DeviceManager::estimate (src/DeviceManager.cu:172-221)
  *   ctpwa_hessian_workspace_bytes <- the part of that estimate getHessian's buffers take
+ *   ctpwa_amplitude_gram <- the integrals of getFitFractions / writeResult's interference
+ *   ctpwa_partial_intensities <- writeResult's per-event total and partial weights
  */
 #ifndef CTPWA_AMD_H
 #define CTPWA_AMD_H
@@ -92,6 +94,20 @@ int ctpwa_intensity(ctpwa_handle* h, const double* params_dev, int cls, double* 
 
 /* per-event amplitudes of class `cls` (without couplings): device [n_cls][n_pol][n_amp][2] */
 int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
+
+/* amplitude Gram of class `cls`: device [n_amp][n_amp][2] complex
+ * Q[l][k] = sum_events sum_pol conj(A_l) A_k of the amplitudes without couplings (the
+ * partial-wave and interference integrals of computeResults and getFitFractions,
+ * src/ComputeResults.cu:57-190, src/ComputeBF.cu, src/Analysis.cu:3382-3420,3527-3638);
+ * fails for models with Custom lineshapes */
+int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
+
+/* per-event partial-wave intensities of class `cls`: device [n_block + 1][n_cls];
+ * row b < n_block: sum_pol |P_b|^2 with P_b the coherent sum of partial wave (block) b's
+ * amplitudes times couplings, row n_block: the total intensity (writeResult's per-event
+ * weights before normalisation, src/ComputeResults.cu:22-190, src/Analysis.cu:1460-1700);
+ * fails for models with Custom lineshapes */
+int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
 /* average duration in ms of the fused per-event kernel over `iters` (<= 10000) launches
  * at `params_dev`: a HIP event pair on `stream` around every launch; the prep kernel (and
