@@ -388,6 +388,7 @@ static void check_q0_masses(const Model_& M) {
                 double cfg = side ? n.md2 : n.md1;
                 int ref = side ? n.d2ref : n.d1ref;
                 if (cfg > 0) continue;
+                if (ref >= 0 && cfg == 0.0) continue;    // massless final-state particle: q0 mass 0
                 bool ok = false;
                 if (ref < 0)
                     for (const PRes& r : B.res) if (r.node == -ref - 1 && !r.params.empty()) ok = true;

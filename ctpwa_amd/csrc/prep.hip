@@ -176,12 +176,13 @@ __device__ inline DJ<NV, H> res_param(const DRes& R, int p, const DBlock& B, con
 template <int NV, bool H>
 __device__ inline DJ<NV, H> q0_dmass(double cfg, int ref, const DBlock& B, const DRes* res, const double* theta) {
     if (cfg > 0) return jc<NV, H>(cfg);
-    if (ref < 0) {
-        int kk = -ref - 1;
-        for (int r = 0; r < B.nres; ++r) {
-            const DRes& R = res[B.res0 + r];
-            if (R.node == kk && R.np > 0) return res_param<NV, H>(R, 0, B, theta);
-        }
+    // a final-state particle configured with mass 0: the reference falls back to the
+    // event mass here, which is 0 for a massless particle (up to rounding)
+    if (ref >= 0) return jc<NV, H>(0.0);
+    int kk = -ref - 1;
+    for (int r = 0; r < B.nres; ++r) {
+        const DRes& R = res[B.res0 + r];
+        if (R.node == kk && R.np > 0) return res_param<NV, H>(R, 0, B, theta);
     }
     return jc<NV, H>(1.0);   // unreachable: the host rejects event-dependent q0
 }

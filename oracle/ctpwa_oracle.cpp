@@ -11,9 +11,11 @@
 // the same quantities.
 //
 // Parity pins (see tests/): CG against sympy, Y_lm against scipy, Wigner d against the
-// matrix exponential of J_y, lineshapes against closed forms, gradient and Hessian
-// against central finite differences of this file's own NLL, and likelihood
-// invariances (global phase, single-amplitude independence of v).
+// matrix exponential of J_y, lineshapes against closed forms, the massive spin transform
+// against the Thomas-Wigner rotation composed from explicit Lorentz boosts, the massless
+// one against its m -> 0 limit, the node tensor against the canonical amplitude in a
+// moving frame, gradient and Hessian against central finite differences of this file's
+// own NLL, and likelihood invariances (global phase, single-amplitude independence of v).
 
 #include <algorithm>
 #include <cmath>
@@ -1280,6 +1282,17 @@ void orc_node_tensor(const double* p1, int dj1, const double* p2, int dj2, int d
     std::vector<cd> amp(dj * dj1 * dj2);
     pwa_amp(amp.data(), LV(p1[0], p1[1], p1[2], p1[3]), dj1, LV(p2[0], p2[1], p2[2], p2[3]), dj2, dj, dS, L);
     for (size_t i = 0; i < amp.size(); ++i) { out[2 * i] = amp[i].real(); out[2 * i + 1] = amp[i].imag(); }
+}
+
+void orc_massive_trans(const double* P, const double* p, int dim, double* out) {
+    std::vector<cd> T(dim * dim);
+    massive_trans(T.data(), LV(P[0], P[1], P[2], P[3]), LV(p[0], p[1], p[2], p[3]), dim);
+    for (int i = 0; i < dim * dim; ++i) { out[2 * i] = T[i].real(); out[2 * i + 1] = T[i].imag(); }
+}
+void orc_massless_trans(const double* P, const double* p, int dim, double* out) {
+    std::vector<cd> T(dim * dim);
+    massless_trans(T.data(), LV(P[0], P[1], P[2], P[3]), LV(p[0], p[1], p[2], p[3]), dim);
+    for (int i = 0; i < dim * dim; ++i) { out[2 * i] = T[i].real(); out[2 * i + 1] = T[i].imag(); }
 }
 
 }  // extern "C"

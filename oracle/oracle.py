@@ -61,6 +61,8 @@ def lib():
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_flatte.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.orc_gs.argtypes = [ctypes.c_double] * 6 + [_dp]
+        L.orc_massive_trans.argtypes = [_dp, _dp, ctypes.c_int, _dp]
+        L.orc_massless_trans.argtypes = [_dp, _dp, ctypes.c_int, _dp]
         L.orc_node_tensor.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _dp]
         _lib = L
@@ -192,3 +194,23 @@ def node_tensor(p1, dj1, p2, dj2, dj, dS, L):
     out = np.zeros((dj, dj1, dj2, 2))
     lib().orc_node_tensor(_ptr(p1), dj1, _ptr(p2), dj2, dj, dS, L, _ptr(out))
     return out[..., 0] + 1j * out[..., 1]
+
+
+def _trans(fn, P, p, dim):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    out = np.zeros((dim, dim, 2))
+    fn(_ptr(P), _ptr(p), dim, _ptr(out))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def massive_trans(P, p, dim):
+    """Spin-transform matrix of a massive daughter with four-momentum p of a mother with
+    four-momentum P (both (E, px, py, pz) in one frame); rows and columns ordered
+    m = j, j-1, ..., -j."""
+    return _trans(lib().orc_massive_trans, P, p, dim)
+
+
+def massless_trans(P, p, dim):
+    """The diagonal phase matrix of a massless daughter (before the helicity rotation)."""
+    return _trans(lib().orc_massless_trans, P, p, dim)
