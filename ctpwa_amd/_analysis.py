@@ -156,7 +156,7 @@ class analysis:
         return list(self.plan.param_names)
 
     def getAmplitudeNames(self):
-        return list(self.plan.amp_names)
+        return [self.plan.amp_names[a] for a in self.plan.ref_amplitudes()]
 
     def getNPolarizations(self) -> int:
         return self.plan.npol
@@ -242,7 +242,7 @@ class analysis:
         partial wave b, then the total intensity."""
         self._check_params(params)
         n = self.n_events[("phsp", "data", "bkg")[cls]]
-        out = torch.empty((len(self.plan.blocks) + 1, n), dtype=torch.float64, device=self.device)
+        out = torch.empty((len(self.plan.waves) + 1, n), dtype=torch.float64, device=self.device)
         if n:
             self._h.partial_intensities(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
                                         torch.cuda.current_stream(self.device).cuda_stream)

@@ -45,7 +45,8 @@ extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, long l
 extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
 extern "C" int ctpwa_mfma_gram_ok(int);
 extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, hipStream_t);
-extern "C" hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs*, int, long long, long long, double*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
+                                               double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_gram_to_complex(const double*, int, double*, hipStream_t);
 extern "C" long long ctpwa_mfma_gram_part_doubles(int, int);
 extern "C" hipError_t ctpwa_launch_wgram_mfma(const double*, int, long long, long long, const double*, long long,
@@ -261,6 +262,8 @@ struct ctpwa_handle {
     hipFunction_t fn_aux = nullptr;        // amplitudes / intensities (models with Custom lineshapes)
     hipFunction_t fn_custom = nullptr;     // Custom factors and derivatives per event (models with Custom lineshapes)
     double* cfb = nullptr;                 // their per-event rows, [ncustom * ncf][ntot]
+    int* wave_ptr = nullptr;               // partial wave w = blocks wave_blk[wave_ptr[w] .. wave_ptr[w+1])
+    int* wave_blk = nullptr;
     std::string src;
     // Hessian workspace (allocated on the first ctpwa_hessian call)
     bool hess_ready = false;
@@ -722,6 +725,16 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->fin.totals = h->alloc<double>((size_t)3 * h->nacc);
         h->aux = AuxArgs{h->store(), h->table, (long long)M.off_c(), (long long)M.off_rec(),
                          d_amps, d_blocks, d_dens, d_dterms, d_terms, P.namp, P.npol};
+        {
+            std::vector<int> wptr(P.nwave + 1, 0), wblk;
+            for (int w = 0; w < P.nwave; ++w) {
+                for (size_t b = 0; b < P.blocks.size(); ++b)
+                    if (P.bwave[b] == w) wblk.push_back((int)b);
+                wptr[w + 1] = (int)wblk.size();
+            }
+            h->wave_ptr = h->upload(wptr);
+            h->wave_blk = h->upload(wblk);
+        }
 
         h->src = generate_eval_kernel(M);
         h->mod = compile_eval(device, h->src);
@@ -940,7 +953,8 @@ extern "C" int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_d
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
     if (e == hipSuccess)
-        e = ctpwa_launch_partial_int(&h->aux, (int)h->M.P.blocks.size(), h->off[cls], h->n[cls], out_dev, s);
+        e = ctpwa_launch_partial_int(&h->aux, h->wave_ptr, h->wave_blk, h->M.P.nwave, h->off[cls], h->n[cls],
+                                     out_dev, s);
     if (e != hipSuccess) return fail(std::string("ctpwa_partial_intensities: ") + hipGetErrorString(e));
     return 0;
 }

@@ -61,6 +61,10 @@ struct Plan {
     std::vector<double> aratio;
     std::vector<int> g_slot;
     std::vector<double> g_mu, g_sig;
+    // partial wave of every block (a block and its identical-particle permuted copies
+    // form one wave); waves are numbered 0 .. nwave-1
+    std::vector<int> bwave;
+    int nwave = 0;
 };
 
 inline Plan parse_plan(const std::string& text) {
@@ -111,6 +115,11 @@ inline Plan parse_plan(const std::string& text) {
         else if (key == "namp") ls >> P.namp >> P.nfree;
         else if (key == "amap") { P.amap.resize(P.namp); for (auto& x : P.amap) ls >> x; }
         else if (key == "aratio") { P.aratio.resize(P.namp); for (auto& x : P.aratio) ls >> x; }
+        else if (key == "bwave") {
+            P.bwave.resize(P.blocks.size());
+            for (auto& x : P.bwave)
+                if (!(ls >> x)) throw std::runtime_error("plan: bwave is truncated");
+        }
         else if (key == "ngauss") {}
         else if (key == "gauss") { int s; double mu, sg; ls >> s >> mu >> sg; P.g_slot.push_back(s); P.g_mu.push_back(mu); P.g_sig.push_back(sg); }
         else if (key == "end") { ended = true; break; }
@@ -120,6 +129,17 @@ inline Plan parse_plan(const std::string& text) {
     if (P.nfinal <= 0 || P.nfinal > 16) throw std::runtime_error("plan: nfinal must be in [1, 16]");
     if (P.npol <= 0) throw std::runtime_error("plan: npol must be positive");
     if ((int)P.th_init.size() != P.ntheta) throw std::runtime_error("plan: theta count mismatch");
+    if (P.bwave.empty())
+        for (size_t b = 0; b < P.blocks.size(); ++b) P.bwave.push_back((int)b);
+    if (P.bwave.size() != P.blocks.size()) throw std::runtime_error("plan: bwave count mismatch");
+    for (int w : P.bwave) P.nwave = std::max(P.nwave, w + 1);
+    std::vector<int> seen(P.nwave, 0);
+    for (int w : P.bwave) {
+        if (w < 0) throw std::runtime_error("plan: negative partial-wave index");
+        seen[w] = 1;
+    }
+    for (int x : seen)
+        if (!x) throw std::runtime_error("plan: partial-wave indices are not contiguous");
     return P;
 }
 

@@ -507,16 +507,21 @@ __global__ void k_amp_rows(AuxArgs a, long long e0, long long n, double* __restr
 // per-event partial-wave intensities: out[b][i] = sum_p |P_b(e, p)|^2 for the nblock
 // partial waves (blocks) and out[nblock][i] = I(e) = sum_p |sum_b P_b|^2, with
 // P_b = Psi_b sum_{a in b} c_a sl_u(a),p and c_a = v_a K_a (src/ComputeResults.cu:57-190)
-__global__ void k_partial_int(AuxArgs a, int nblock, long long e0, long long n, double* __restrict__ out) {
+// partial wave w = blocks wblk[wptr[w] .. wptr[w+1]) (a block and its identical-particle
+// permuted copies): out[w][i] = sum_pol |sum over the wave's blocks P_b|^2,
+// out[nwave][i] = the total intensity
+__global__ void k_partial_int(AuxArgs a, const int* __restrict__ wptr, const int* __restrict__ wblk, int nwave,
+                              long long e0, long long n, double* __restrict__ out) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     long long e = e0 + i;
-    double Sr[MAXPOL], Si[MAXPOL];
+    double Sr[MAXPOL], Si[MAXPOL], Wr[MAXPOL], Wi[MAXPOL];
     for (int p = 0; p < a.npol; ++p) { Sr[p] = 0.0; Si[p] = 0.0; }
-    for (int b = 0; b < nblock; ++b) {
-        const DBlock& B = a.blocks[b];
-        double w = 0.0;
-        if (B.nsl > 0) {
+    for (int w = 0; w < nwave; ++w) {
+        for (int p = 0; p < a.npol; ++p) { Wr[p] = 0.0; Wi[p] = 0.0; }
+        for (int k = wptr[w]; k < wptr[w + 1]; ++k) {
+            const DBlock& B = a.blocks[wblk[k]];
+            if (B.nsl <= 0) continue;
             double2 psi = block_psi(a, B, e);
             for (int p = 0; p < a.npol; ++p) {
                 double yr = 0.0, yi = 0.0;
@@ -527,17 +532,21 @@ __global__ void k_partial_int(AuxArgs a, int nblock, long long e0, long long n, 
                     yr += cr * s.x - ci * s.y;
                     yi += cr * s.y + ci * s.x;
                 }
-                const double pr = psi.x * yr - psi.y * yi, pi = psi.x * yi + psi.y * yr;
-                w += pr * pr + pi * pi;
-                Sr[p] += pr;
-                Si[p] += pi;
+                Wr[p] += psi.x * yr - psi.y * yi;
+                Wi[p] += psi.x * yi + psi.y * yr;
             }
         }
-        out[(long long)b * n + i] = w;
+        double v = 0.0;
+        for (int p = 0; p < a.npol; ++p) {
+            v += Wr[p] * Wr[p] + Wi[p] * Wi[p];
+            Sr[p] += Wr[p];
+            Si[p] += Wi[p];
+        }
+        out[(long long)w * n + i] = v;
     }
     double I = 0.0;
     for (int p = 0; p < a.npol; ++p) I += Sr[p] * Sr[p] + Si[p] * Si[p];
-    out[(long long)nblock * n + i] = I;
+    out[(long long)nwave * n + i] = I;
 }
 
 __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __restrict__ out) {
@@ -633,12 +642,13 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_amp_row
     return hipGetLastError();
 }
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs* a, int nblock,
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs* a, const int* wptr,
+                                                                                     const int* wblk, int nwave,
                                                                                      long long e0, long long n, double* out,
                                                                                      hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ctpwa::k_partial_int, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, nblock, e0, n,
-                       out);
+    hipLaunchKernelGGL(ctpwa::k_partial_int, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, wptr, wblk,
+                       nwave, e0, n, out);
     return hipGetLastError();
 }
 

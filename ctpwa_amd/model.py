@@ -22,6 +22,7 @@ line-oriented text document, see :meth:`Plan.to_text`.
 from __future__ import annotations
 
 import itertools
+import warnings
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
@@ -136,6 +137,8 @@ class Block:
     nsl: int
     res: List[ResInst]
     chain_key: str
+    wave: int = -1           # partial wave: the block itself, or the block a permuted copy sums into
+    sigma: int = 0           # identical-particle permutation (0: the identity)
 
 
 @dataclass
@@ -169,6 +172,20 @@ class Plan:
     @property
     def n_theta(self) -> int:
         return len(self.theta)
+
+    @property
+    def waves(self) -> List[int]:
+        """Blocks that are partial waves (the identity permutation), in block order."""
+        return [b for b, blk in enumerate(self.blocks) if blk.wave == b]
+
+    def ref_amplitudes(self) -> List[int]:
+        """Amplitudes of the identity permutation (the reference's amplitude list)."""
+        return [a for blk in self.blocks if blk.sigma == 0 for a in range(blk.site, blk.site + blk.nsl)]
+
+    def wave_of(self) -> List[int]:
+        """Partial-wave position (index into ``waves``) of every block."""
+        pos = {b: i for i, b in enumerate(self.waves)}
+        return [pos[blk.wave] for blk in self.blocks]
 
     def n_params(self) -> int:
         return 2 * self.n_free + self.n_theta
@@ -207,6 +224,8 @@ class Plan:
         out.append("namp %d %d" % (self.n_amp, self.n_free))
         out.append("amap " + " ".join(str(x) for x in self.amp_free))
         out.append("aratio " + " ".join(repr(float(x)) for x in self.amp_ratio))
+        if any(blk.wave != b for b, blk in enumerate(self.blocks)):
+            out.append("bwave " + " ".join(str(x) for x in self.wave_of()))
         out.append("ngauss %d" % len(self.gauss))
         for (s, mu, sig) in self.gauss:
             out.append("gauss %d %r %r" % (s, mu, sig))
@@ -355,6 +374,147 @@ def _apply_var_equal(cfg: ConfigParser, blocks, theta) -> None:
     theta[:] = [theta[j] for j in keep]
 
 
+def _chain_identical_groups(cfg: ConfigParser, chain) -> List[Tuple[List[str], bool]]:
+    """Identical groups with at least two members among the chain's particles, members
+    in Particles order, groups in the order of their names (src/Analysis.cu:4553-4580)."""
+    in_chain = set()
+    for st in chain.decay_steps:
+        in_chain.add(st.mother)
+        in_chain.update(st.daughters)
+    members: Dict[str, List[str]] = {}
+    boson: Dict[str, bool] = {}
+    for p in cfg.particles:
+        if p.identical_group and p.name in in_chain:
+            members.setdefault(p.identical_group, []).append(p.name)
+            boson[p.identical_group] = not p.is_fermion()
+    return [(members[g], boson[g]) for g in sorted(members) if len(members[g]) >= 2]
+
+
+def _identical_permutations(steps, root: str, groups, final_order: List[str]):
+    """The permuted topologies of one chain (src/AmpGen.cu:194-403).
+
+    For every element g != 1 of a group's permutation group (all of S_n, stabiliser
+    swaps of sibling members included), the subtree of member g[i] takes the place of
+    member i's, path by path.  The sign is the parity of g for fermions, +1 for bosons.
+    Returns [(read, sign)] with ``read[f]`` the final-state index whose momentum the
+    permuted topology reads where the chain names final particle f.
+
+    Several groups combine the way the reference combines them: each element is a
+    complete particle -> slot map and the earlier groups' map is laid over the later
+    group's (src/AmpGen.cu:366-385), so only the first group's permutations survive,
+    each repeated once per non-identity element of every later group, with the
+    product of the signs."""
+    maps: List[Dict[str, str]] = []
+    signs: List[float] = []
+    names = set()
+    for st in steps:
+        names.add(st.mother)
+        names.update(st.daughters)
+    for members, is_boson in groups:
+        subs = [m for m in members if m != root and m in names]
+        if len(subs) < 2:
+            continue
+        paths: List[Dict[Tuple[int, ...], str]] = []
+        name_path: List[Dict[str, Tuple[int, ...]]] = []
+        for m in subs:
+            pth = {(): m}
+            npth = {m: ()}
+            changed = True
+            while changed:
+                changed = False
+                for st in steps:
+                    if st.mother not in npth:
+                        continue
+                    for side, d in ((1, st.daughters[0]), (2, st.daughters[1])):
+                        q = npth[st.mother] + (side,)
+                        if q not in pth:
+                            pth[q] = d
+                            npth[d] = q
+                            changed = True
+            paths.append(pth)
+            name_path.append(npth)
+        if any(set(pth) != set(paths[0]) for pth in paths):
+            warnings.warn("identical group members have different decay structures (%s); "
+                          "no symmetrisation" % ", ".join(subs))
+            break
+        if any(i != j and subs[j] in name_path[i] for i in range(len(subs)) for j in range(len(subs))):
+            warnings.warn("identical group members nested inside each other (%s); "
+                          "no symmetrisation" % ", ".join(subs))
+            break
+        n = len(subs)
+        gmaps: List[Dict[str, str]] = []
+        gsigns: List[float] = []
+        for g in itertools.permutations(range(n)):
+            if list(g) == list(range(n)):
+                continue
+            inv = sum(1 for i in range(n) for j in range(i + 1, n) if g[i] > g[j])
+            sg = 1.0 if (is_boson or inv % 2 == 0) else -1.0
+            # placement: the momentum of particle Y goes to the slot of particle X
+            m = {nm: nm for nm in names}
+            for i in range(n):
+                for q, nm_i in paths[i].items():
+                    m[paths[g[i]][q]] = nm_i
+            gmaps.append(m)
+            gsigns.append(sg)
+        if not maps:
+            maps, signs = gmaps, gsigns
+        else:
+            merged, msigns = [], []
+            for a in range(len(maps)):
+                for b in range(len(gmaps)):
+                    mm = dict(gmaps[b])
+                    mm.update(maps[a])
+                    merged.append(mm)
+                    msigns.append(signs[a] * gsigns[b])
+            maps, signs = merged, msigns
+    out = []
+    for m, sg in zip(maps, signs):
+        read = list(range(len(final_order)))
+        for y, x in m.items():
+            if x in final_order and y in final_order:
+                read[final_order.index(x)] = final_order.index(y)
+        out.append((read, sg))
+    return out
+
+
+def _symmetrise(topologies: List[Topology], blocks: List[Block], perms: Dict[int, list],
+                amp_names: List[str], amp_free: List[int], amp_ratio: List[float]) -> int:
+    """Append one permuted copy of every topology with identical particles and of each of
+    its blocks: the copy reads the permuted momenta, shares the block's resonance
+    parameters, and its amplitudes take the original's coupling times the permutation's
+    sign, so that sum_a v_a A_a carries A(x) + sum_sigma sgn(sigma) A(x_sigma)
+    (src/AmpGen.cu:1440-1460).  Returns the new amplitude count."""
+    import copy
+    n_amp = len(amp_names)
+    n_orig = len(blocks)
+    for b in range(n_orig):
+        blocks[b].wave = b
+    copies: Dict[Tuple[int, int], int] = {}
+    for b in range(n_orig):
+        blk = blocks[b]
+        for k, (read, sg) in enumerate(perms.get(blk.topo, []), start=1):
+            key = (blk.topo, k)
+            if key not in copies:
+                t = topologies[blk.topo]
+                nodes = []
+                for nd in t.nodes:
+                    c = copy.copy(nd)
+                    c.d1_ref = read[nd.d1_ref] if nd.d1_ref >= 0 else nd.d1_ref
+                    c.d2_ref = read[nd.d2_ref] if nd.d2_ref >= 0 else nd.d2_ref
+                    nodes.append(c)
+                copies[key] = len(topologies)
+                topologies.append(Topology(t.chain, list(t.comb), nodes, list(t.sl_combos)))
+            blocks.append(Block(copies[key], n_amp, blk.nsl, copy.deepcopy(blk.res), blk.chain_key,
+                                wave=b, sigma=k))
+            for c in range(blk.nsl):
+                a = blk.site + c
+                amp_names.append("%s#perm%d" % (amp_names[a], k))
+                amp_free.append(amp_free[a])
+                amp_ratio.append(amp_ratio[a] * sg)
+            n_amp += blk.nsl
+    return n_amp
+
+
 def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
     if fit_mode != 1:
         raise ConfigError("only fit mode 1 (vspace) is built; chain x step mode is not")
@@ -366,9 +526,6 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
     for nm in final_order:
         if nm not in pmap:
             raise ConfigError("Data.order particle '%s' is not in Particles" % nm)
-    for p in particles:
-        if p.identical_group:
-            raise ConfigError("identical-particle symmetrisation is not built")
     npol, polmap = _polarization(particles)
 
     topologies: List[Topology] = []
@@ -380,6 +537,7 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
     amp_chain_key: List[str] = []
     gls = 0
     initial: Optional[str] = None
+    perms: Dict[int, list] = {}          # topology -> identical-particle permutations
 
     for chain in cfg.decay_chains:
         # intermediate_resonance_map (src/Info.cu:60-120)
@@ -417,7 +575,13 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
                 (s0, p0), (s1, p1), (s2, p2) = qn(st.mother), qn(st.daughters[0]), qn(st.daughters[1])
                 if 0 in (s0, s1, s2):
                     raise ConfigError("unresolved quantum numbers in chain %s" % chain.name)
-                sl = com_sl((s0, s1, s2), (p0, p1, p2), st.p_break, False, True,
+                # identical daughters: (-1)^(L+S) = +1 (bosons) or -1 (fermions)
+                # (src/Info.cu:150-162)
+                g1 = pmap[st.daughters[0]].identical_group if st.daughters[0] in pmap else ""
+                g2 = pmap[st.daughters[1]].identical_group if st.daughters[1] in pmap else ""
+                ident = bool(g1) and g1 == g2
+                boson = not pmap[st.daughters[0]].is_fermion() if ident else True
+                sl = com_sl((s0, s1, s2), (p0, p1, p2), st.p_break, ident, boson,
                             cfg.global_maxL, st.sl_filter)
                 masses = tuple(pmap[nm].mass if nm in pmap else -1.0
                                for nm in (st.mother, st.daughters[0], st.daughters[1]))
@@ -444,6 +608,10 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
             sl_combos = [tuple(c) for c in itertools.product(*[nd.sl_list for nd in nodes])]
             topo_idx = len(topologies)
             topologies.append(Topology(chain.name, comb, nodes, sl_combos))
+            groups = _chain_identical_groups(cfg, chain)
+            if groups:
+                perms[topo_idx] = _identical_permutations(chain.decay_steps, nodes[0].mother,
+                                                          groups, final_order)
 
             # resonance combinations (last intermediate fastest)
             res_lists = [ir_map[key] for key in comb]
@@ -544,6 +712,7 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
         if abs(amp_ratio[a] - 1.0) < 1e-10:
             free_names[amp_free[a]] = amp_names[a]
     param_names = [free_names.get(i, "") for i in range(n_free)] + [t.name for t in theta]
+    n_amp = _symmetrise(topologies, blocks, perms, amp_names, amp_free, amp_ratio)
 
     gauss = []
     for nm, sig in cfg.gauss_constr.items():

@@ -9,7 +9,8 @@ object, the complex Gram of the amplitudes over a sample,
     Q[a][a'] = sum_events sum_pol conj(A_a) A_a'        (ctpwa_amplitude_gram)
 
 with a partial wave b (one chain with one resonance set: the reference's nSLvectors_
-groups, src/Info.cu:240-300) the coherent sum P_b = sum_{a in b} v_a A_a.  Then
+groups, src/Info.cu:240-300, together with its identical-particle permuted copies) the
+coherent sum P_b = sum_{a in b} v_a A_a.  Then
 
     M[b][b'] = sum conj(P_b) P_b' = sum_{a in b, a' in b'} conj(v_a) v_a' Q[a][a']
 
@@ -38,10 +39,12 @@ def coupling_vector(plan, params: np.ndarray) -> np.ndarray:
 
 
 def partial_weights_matrix(plan, v: np.ndarray) -> np.ndarray:
-    """W[b][a] = v_a for the amplitudes a of partial wave (block) b, else 0."""
-    W = np.zeros((len(plan.blocks), plan.n_amp), dtype=np.complex128)
+    """W[w][a] = v_a for the amplitudes a of partial wave w (a block and its
+    identical-particle permuted copies), else 0."""
+    wave = plan.wave_of()
+    W = np.zeros((len(plan.waves), plan.n_amp), dtype=np.complex128)
     for b, B in enumerate(plan.blocks):
-        W[b, B.site:B.site + B.nsl] = v[B.site:B.site + B.nsl]
+        W[wave[b], B.site:B.site + B.nsl] = v[B.site:B.site + B.nsl]
     return W
 
 
@@ -65,13 +68,14 @@ def fit_fraction_jacobian(Q: np.ndarray, plan, params: np.ndarray) -> np.ndarray
     QW = Q @ W.T                                 # [a][b] = (Q W_b)_a
     d = np.real(np.einsum("ba,ab->b", np.conj(W), QW))
     S = d.sum() if d.sum() > 0 else 1e-30
-    nb = len(plan.blocks)
-    D = np.zeros((nb, 2 * n))                    # dM[b][b] / dx
+    wave = plan.wave_of()
+    D = np.zeros((len(plan.waves), 2 * n))       # dM[w][w] / dx
     for b, B in enumerate(plan.blocks):
+        w = wave[b]
         for a in range(B.site, B.site + B.nsl):
             f, r = plan.amp_free[a], plan.amp_ratio[a]
-            D[b, f] += 2.0 * r * np.real(QW[a, b])          # dv_a / dRe p_f = r_a
-            D[b, n + f] += 2.0 * r * np.imag(QW[a, b])      # dv_a / dIm p_f = i r_a
+            D[w, f] += 2.0 * r * np.real(QW[a, w])          # dv_a / dRe p_f = r_a
+            D[w, n + f] += 2.0 * r * np.imag(QW[a, w])      # dv_a / dIm p_f = i r_a
     return (D * S - np.outer(d, D.sum(axis=0))) / (S * S)
 
 
@@ -105,7 +109,8 @@ def partial_names(plan) -> List[str]:
     the chain's resonances for a partial wave without allowed waves."""
     import re
     out = []
-    for b, B in enumerate(plan.blocks):
+    for b in plan.waves:
+        B = plan.blocks[b]
         if B.nsl > 0:
             out.append(re.sub(r"_SL\(\d+,\d+\)", "", plan.amp_names[B.site]))
         else:

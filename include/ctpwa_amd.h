@@ -102,11 +102,14 @@ int ctpwa_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double*
  * fails for models with Custom lineshapes */
 int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
-/* per-event partial-wave intensities of class `cls`: device [n_block + 1][n_cls];
- * row b < n_block: sum_pol |P_b|^2 with P_b the coherent sum of partial wave (block) b's
- * amplitudes times couplings, row n_block: the total intensity (writeResult's per-event
- * weights before normalisation, src/ComputeResults.cu:22-190, src/Analysis.cu:1460-1700);
- * fails for models with Custom lineshapes */
+/* per-event partial-wave intensities of class `cls`: device [n_wave + 1][n_cls];
+ * row w < n_wave: sum_pol |P_w|^2 with P_w the coherent sum of partial wave w's
+ * amplitudes times couplings, row n_wave: the total intensity (writeResult's per-event
+ * weights before normalisation, src/ComputeResults.cu:22-190, src/Analysis.cu:1460-1700).
+ * A partial wave is one resonance set of one chain (a plan block) together with its
+ * identical-particle permuted copies; n_wave is the number of distinct values of the
+ * plan's `bwave` record, or the block count when the plan has none.
+ * Fails for models with Custom lineshapes */
 int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
 /* average duration in ms of the fused per-event kernel over `iters` (<= 10000) launches
