@@ -316,9 +316,12 @@ class ConfigParser:
                 r.free = [int(x) for x in props["free"]]
             if "free_range" in props:
                 r.free_range = [[float(a) for a in rg] for rg in props["free_range"]]
-            for k in ("file", "bins", "extrapolate", "method", "expr", "expression"):
+            for k in ("file", "bins", "extrapolate", "expr", "expression"):
                 if k in props:
                     r.options[k] = str(props[k])
+            if "range" in props:
+                # the reference joins the values with std::to_string ("%f")
+                r.options["range"] = ",".join("%f" % float(x) for x in props["range"])
             pn = props.get("paramNames", props.get("params"))
             if pn:
                 r.options["params"] = ",".join(str(x) for x in pn)

@@ -25,7 +25,8 @@ std::string generate_eval_kernel(const Model_& M);
 
 extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctpwa::DTopo*, int, const ctpwa::DNode*,
                                          const ctpwa::DKin*, int, const ctpwa::DU*, int, const double*, const int*, int,
-                                         const ctpwa::DInv*, int, double2*, double*, long long, long long, hipStream_t);
+                                         const ctpwa::DInv*, int, const double*, double2*, double*, long long, long long,
+                                         hipStream_t);
 extern "C" hipError_t ctpwa_launch_to_mixed(const double2*, const double*, long long, int, int, const int*, float2*,
                                              double*, float*, long long, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
@@ -393,8 +394,10 @@ static void check_q0_masses(const Model_& M) {
                 if (cfg > 0) continue;
                 if (ref >= 0 && cfg == 0.0) continue;    // massless final-state particle: q0 mass 0
                 bool ok = false;
+                // an occupying resonance gives its mass parameter, or 1 when it has none
+                // (src/AmpGen.cu:1504-1527)
                 if (ref < 0)
-                    for (const PRes& r : B.res) if (r.node == -ref - 1 && !r.params.empty()) ok = true;
+                    for (const PRes& r : B.res) if (r.node == -ref - 1) ok = true;
                 if (!ok) throw std::runtime_error("a daughter without a configured mass and without a resonance mass "
                                                   "parameter makes q0 event-dependent; not built");
             }
@@ -529,6 +532,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             double* d_cg = h->upload(cgtab);
             int* d_pol = h->upload(P.polmap);
             DInv* d_invs = h->upload(invs);
+            const double* d_tabs = h->upload(M.tabs);
             // events class by class, in chunks through the device's staging pool: the
             // host copy of a chunk into pinned memory overlaps the DMA and the init
             // kernel of the previous one; only the pool's stream is waited for
@@ -547,13 +551,13 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
                     HIPCHECK(hipEventRecord(S->copied[b], S->s));
                     if (h->mixed) {
                         HIPCHECK(ctpwa_launch_init(S->dev[b], cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
-                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv, d_tabs,
                                                    S->sl64, S->inv64, cnt, 0, S->s));
                         HIPCHECK(ctpwa_launch_to_mixed(S->sl64, S->inv64, cnt, nu * P.npol, ninv, h->d_imap, h->slf,
                                                        h->inv, h->invf, h->ntot, h->off[c] + s0, S->s));
                     } else {
                         HIPCHECK(ctpwa_launch_init(S->dev[b], cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
-                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv,
+                                                   (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv, d_tabs,
                                                    h->sl, h->inv, h->ntot, h->off[c] + s0, S->s));
                     }
                     b ^= 1;
@@ -605,7 +609,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         }
         for (const DenSpec& D : M.dens) {
             dens.push_back(DDen{D.block, D.res, D.coef_base, D.ncoef});
-            DDenTerms dt{(int)D.re.size(), (int)D.im.size(), (int)terms.size(), 0};
+            DDenTerms dt{(int)D.re.size(), (int)D.im.size(), (int)terms.size(), D.stat ? 1 : 0};
             for (const Term& t : D.re) terms.push_back(DTerm{t.inv, t.coef, t.cval});
             for (const Term& t : D.im) terms.push_back(DTerm{t.inv, t.coef, t.cval});
             dterms.push_back(dt);

@@ -429,6 +429,10 @@ std::string generate_eval_kernel(const Model_& M) {
             o << "    double dr" << d << " = 0.0, di" << d << " = 0.0;\n";
             for (const Term& t : Dn.re) o << "    dr" << d << " += " << term_expr((int)d, t) << ";\n";
             for (const Term& t : Dn.im) o << "    di" << d << " += " << term_expr((int)d, t) << ";\n";
+            if (Dn.stat) {   // table lineshape: the factor itself
+                o << "    const double pr" << d << " = dr" << d << ", pi" << d << " = di" << d << ";\n";
+                continue;
+            }
             o << "    const double q" << d << " = frcp(dr" << d << " * dr" << d << " + di" << d << " * di" << d << ");\n";
             o << "    const double pr" << d << " = dr" << d << " * q" << d << ", pi" << d << " = di" << d << " * q" << d << ";\n";
         }

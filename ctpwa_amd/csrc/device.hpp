@@ -34,7 +34,9 @@ struct DDen { int block, res, rec, ncoef; };
 struct DAmp { int block, slc, freeidx, rec, u, pad; double ratio; };
 struct DTerm { int inv, coef; double cval; };   // inv < 0: 1; coef < 0: constant cval
 struct DThTerm { int block, rec; };               // theta-gradient accumulator -> coefficient record
-struct DDenTerms { int nre, nim, t0, pad; };      // terms [t0, t0+nre) real, then nim imaginary
+// terms [t0, t0+nre) real, then nim imaginary; stat 0: the factor is 1 / (sum re - i sum im),
+// stat 1 (table lineshapes): the factor is sum re + i sum im itself
+struct DDenTerms { int nre, nim, t0, stat; };
 
 // Blatt-Weisskopf polynomial N_L(z), Bf(L,q,q0,d) = sqrt(N_L(q0 d)/N_L(q d))
 // (include/ResModel.cuh:184-211); N is 1 for L = 0 and for L > 6.

@@ -88,7 +88,8 @@ __device__ inline void block_terms(const HessArgs& a, const DBlock& B, long long
                 }
             }
         }
-        Cd pd = cinv(Dv);
+        // a table lineshape is the factor sum re + i sum im itself (no theta dependence)
+        Cd pd = T.stat ? Cd{Dv.r, -Dv.i} : cinv(Dv);
         psi = cm(psi, pd);
 #pragma unroll
         for (int i = 0; i < NL; ++i) {

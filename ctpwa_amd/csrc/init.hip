@@ -252,13 +252,67 @@ __device__ void node_tensor(Cx* amp, const LV& p1, const LV& p2, const DNode& n,
     }
 }
 
+// Hist shape table t = [m_min, m_max, n, v_0..v_{n-1}] at mass m: linear between the bin
+// values on x = (m - m_min) / (m_max - m_min) * n clamped to [0, n - 2], the first/last
+// value outside the range, 0 for a negative or NaN mass (include/ResModel.cuh:67-85)
+__device__ double hist_value(const double* t, double m) {
+    if (!(m >= 0.0)) return 0.0;
+    const double lo = t[0], hi = t[1];
+    const int n = (int)t[2];
+    const double* v = t + 3;
+    if (n < 2 || hi <= lo) return 0.0;
+    if (m <= lo) return v[0];
+    if (m >= hi) return v[n - 1];
+    double x = (m - lo) / (hi - lo) * n;
+    if (x < 0.0) x = 0.0;
+    if (x >= (double)(n - 1)) x = (double)(n - 2);
+    const int i = (int)x;
+    const double f = x - i;
+    return v[i] * (1.0 - f) + v[i + 1] * f;
+}
+
+// Interp table (linear): t = [1, N, x_min, dx, re..., im...] for equally spaced points or
+// [-2, N, x..., re..., im...] otherwise; the segment index is clamped to [0, N - 2] and
+// the fraction is not, so the end segments extrapolate linearly
+// (include/ResModel.cuh:226-290)
+__device__ void interp_value(const double* t, double x, double& fr, double& fi) {
+    const int hdr = (int)t[0];
+    const int N = (int)t[1];
+    const double *re, *im;
+    int i;
+    double frac;
+    if (hdr >= 0) {
+        const double xmin = t[2], dx = t[3];
+        re = t + 4;
+        im = t + 4 + N;
+        const double pos = (x - xmin) / dx;
+        i = (int)pos;
+        if (i < 0) { i = 0; frac = 0.0; }
+        else if (i >= N - 1) { i = N - 2; frac = 1.0; }
+        else frac = pos - (double)i;
+    } else {
+        const double* xv = t + 2;
+        re = t + 2 + N;
+        im = t + 2 + 2 * N;
+        int lo = 0, hi = N - 1;
+        while (lo < hi - 1) {
+            const int mid = (lo + hi) / 2;
+            if (x < xv[mid]) hi = mid; else lo = mid;
+        }
+        i = lo;
+        frac = (x - xv[i]) / (xv[i + 1] - xv[i]);
+    }
+    fr = re[i] + (re[i + 1] - re[i]) * frac;
+    fi = im[i] + (im[i + 1] - im[i]) * frac;
+}
+
 __global__ void __launch_bounds__(128)
 k_init_events(const double* __restrict__ mom, long long N, int nfinal,
               const DTopo* __restrict__ topos, int ntopo, const DNode* __restrict__ nodes,
               const DKin* __restrict__ kins, int nkin,
               const DU* __restrict__ us, int nu, const double* __restrict__ cgtab,
               const int* __restrict__ polmap, int npol,
-              const DInv* __restrict__ invs, int ninv,
+              const DInv* __restrict__ invs, int ninv, const double* __restrict__ tabs,
               double2* __restrict__ sl, double* __restrict__ inv, long long ld, long long e_off) {
     long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= N) return;
@@ -382,6 +436,17 @@ k_init_events(const double* __restrict__ mom, long long N, int nfinal,
         case 7: val = q * q * q / mm; break;
         case 8: val = mm; break;                                           // m (Custom)
         case 9: val = q; break;                                            // q (Custom)
+        case 10:
+        case 11: {                                                         // table lineshapes
+            const double* t = tabs + I.L;
+            if ((int)I.d == 6) val = (I.kind == 10) ? hist_value(t, mm) : 0.0;
+            else {
+                double fr, fi;
+                interp_value(t, mm, fr, fi);
+                val = (I.kind == 10) ? fr : fi;
+            }
+            break;
+        }
         default: val = 1.0;
         }
         inv[(size_t)i * ld + eo] = val;
@@ -425,12 +490,12 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_init(co
                                          const ctpwa::DTopo* topos, int ntopo, const ctpwa::DNode* nodes,
                                          const ctpwa::DKin* kins, int nkin, const ctpwa::DU* us, int nu,
                                          const double* cgtab, const int* polmap, int npol,
-                                         const ctpwa::DInv* invs, int ninv, double2* sl, double* inv,
-                                         long long ld, long long e_off, hipStream_t stream) {
+                                         const ctpwa::DInv* invs, int ninv, const double* tabs, double2* sl,
+                                         double* inv, long long ld, long long e_off, hipStream_t stream) {
     if (N <= 0) return hipSuccess;
     int bs = 128;
     long long nb = (N + bs - 1) / bs;
     hipLaunchKernelGGL(ctpwa::k_init_events, dim3((unsigned)nb), dim3(bs), 0, stream, mom, N, nfinal, topos, ntopo,
-                       nodes, kins, nkin, us, nu, cgtab, polmap, npol, invs, ninv, sl, inv, ld, e_off);
+                       nodes, kins, nkin, us, nu, cgtab, polmap, npol, invs, ninv, tabs, sl, inv, ld, e_off);
     return hipGetLastError();
 }

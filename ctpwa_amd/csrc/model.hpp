@@ -35,9 +35,10 @@
 
 namespace ctpwa {
 
-enum Model { M_BWR = 0, M_BW = 1, M_ONE = 2, M_FLATTE = 3, M_GS = 4, M_CUSTOM = 5 };
+enum Model { M_BWR = 0, M_BW = 1, M_ONE = 2, M_FLATTE = 3, M_GS = 4, M_CUSTOM = 5, M_HIST = 6, M_INTERP = 7 };
 enum InvKind { INV_ONE = 0, INV_S = 1, INV_E = 2, INV_RHO_RE = 3, INV_RHO_IM = 4,
-               INV_GS_Q2H = 5, INV_GS_Q2 = 6, INV_GS_Q3M = 7, INV_M = 8, INV_Q = 9 };
+               INV_GS_Q2H = 5, INV_GS_Q2 = 6, INV_GS_Q3M = 7, INV_M = 8, INV_Q = 9,
+               INV_TAB_RE = 10, INV_TAB_IM = 11 };   // table lineshape value at m (L: table offset, d: model)
 
 struct PNode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md1, md2; };
 struct PTopo { std::vector<PNode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
@@ -47,6 +48,9 @@ struct PRes {
     std::vector<int> slots;
     std::vector<double> ch;
     std::vector<std::string> expr;   // Custom: expression tokens in prefix order
+    // Hist: [m_min, m_max, n, values...]; Interp: [method, N, x_min, dx, re..., im...]
+    // (equal spacing) or [-(method+1), N, x..., re..., im...]
+    std::vector<double> tab;
 };
 struct PBlock { int topo, site, nsl; std::vector<PRes> res; };
 
@@ -110,6 +114,13 @@ inline Plan parse_plan(const std::string& text) {
             std::vector<std::string> toks(n);
             for (auto& t : toks) if (!(ls >> t)) throw std::runtime_error("cexpr is truncated");
             P.blocks.back().res.back().expr = toks;
+        } else if (key == "rtab") {
+            if (P.blocks.empty() || P.blocks.back().res.empty()) throw std::runtime_error("rtab before res");
+            long long n; ls >> n;
+            if (n <= 0 || n > 10000000) throw std::runtime_error("bad rtab length");
+            std::vector<double> t((size_t)n);
+            for (auto& x : t) if (!(ls >> x)) throw std::runtime_error("rtab is truncated");
+            P.blocks.back().res.back().tab = t;
         } else if (key == "ntheta") ls >> P.ntheta;
         else if (key == "theta") { int j; double a, lo, hi; ls >> j >> a >> lo >> hi; P.th_init.push_back(a); P.th_lo.push_back(lo); P.th_hi.push_back(hi); }
         else if (key == "namp") ls >> P.namp >> P.nfree;
@@ -161,7 +172,10 @@ struct USpec { int topo, slc; uint32_t mask; };   // mask bit k: fold node k's b
 struct Term { int inv; int coef; double cval; };   // inv < 0: constant 1; coef < 0: constant cval
 struct DenSpec {
     int block, res, node;
-    std::vector<Term> re, im;   // D = sum re - i sum im
+    // D = sum re - i sum im; a static factor (table lineshapes) is instead the factor
+    // F = sum re + i sum im itself, without theta dependence
+    std::vector<Term> re, im;
+    bool stat = false;
     int ncoef = 0;              // theta-dependent coefficients (slots in the table)
     int coef_base = 0;          // index of the first coefficient record
 };
@@ -198,6 +212,7 @@ struct Model_ {
     std::vector<CustomSpec> customs;
     std::vector<std::vector<int>> block_customs;  // [block] -> custom indices
     std::vector<CTerm> cterms;                    // Custom theta-gradient accumulators
+    std::vector<double> tabs;                     // table lineshapes, concatenated
     int maxloc = 0;
     // code-generation options of the fused kernel
     int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
@@ -214,7 +229,8 @@ struct Model_ {
         for (size_t i = 0; i < inv.size(); ++i) {
             // squared masses, and the m and q of Custom expressions (arbitrary formulas
             // that may cancel like m0^2 - m^2), stay FP64
-            if (!mixed || inv[i].kind == INV_S || inv[i].kind == INV_M || inv[i].kind == INV_Q) m[i] = nd++;
+            if (!mixed || inv[i].kind == INV_S || inv[i].kind == INV_M || inv[i].kind == INV_Q ||
+                inv[i].kind == INV_TAB_RE || inv[i].kind == INV_TAB_IM) m[i] = nd++;
             else m[i] = -(nf++) - 1;
         }
         return m;
@@ -305,6 +321,7 @@ inline Model_ derive(const Plan& P) {
         return i;
     };
     std::map<std::tuple<int, int, uint32_t>, int> u_idx;
+    std::vector<std::pair<int, size_t>> tab_starts;   // offset and length of each table in M.tabs
     int rec = 0;
     M.block_dens.resize(P.blocks.size());
     M.block_customs.resize(P.blocks.size());
@@ -334,7 +351,7 @@ inline Model_ derive(const Plan& P) {
             if (k == 0 || res_at[k] < 0) fold = T.nodes[k].has_bf;
             else {
                 int mdl = B.res[res_at[k]].model;
-                fold = T.nodes[k].has_bf && mdl != M_BW && mdl != M_CUSTOM;
+                fold = T.nodes[k].has_bf && mdl != M_BW && mdl != M_CUSTOM && mdl != M_HIST && mdl != M_INTERP;
             }
             if (fold) mask |= 1u << k;
         }
@@ -388,6 +405,43 @@ inline Model_ derive(const Plan& P) {
                     D.im.push_back({rr, 1 + i, 0.0});
                 }
                 D.ncoef = 1 + nch;
+                break;
+            }
+            case M_HIST:
+            case M_INTERP: {
+                // value of the table at the pair mass, no vertex barrier
+                // (include/ResModel.cuh:67-85,153-157,226-290)
+                // the device reads the table through its header: check every length here
+                const std::vector<double>& tb = R.tab;
+                if (R.model == M_HIST) {
+                    if (tb.size() < 5 || tb[2] != std::floor(tb[2]) || tb[2] < 2 || tb.size() != 3 + (size_t)tb[2] ||
+                        !(tb[1] > tb[0]))
+                        throw std::runtime_error("Hist table must be [m_min, m_max, n >= 2, n values] with m_max > m_min");
+                } else {
+                    if (tb.size() < 2 || tb[1] != std::floor(tb[1]) || tb[1] < 2)
+                        throw std::runtime_error("Interp table needs at least two points");
+                    size_t N = (size_t)tb[1];
+                    bool uniform = tb[0] >= 0;
+                    if ((uniform && (tb[0] != 1.0 || tb.size() != 4 + 2 * N || !(tb[3] != 0.0))) ||
+                        (!uniform && (tb[0] != -2.0 || tb.size() != 2 + 3 * N)))
+                        throw std::runtime_error("Interp table must be [1, N, x_min, dx, re, im] or [-2, N, x, re, im]");
+                }
+                int off = -1;
+                for (const auto& ts : tab_starts)
+                    if (ts.second == R.tab.size() && std::equal(R.tab.begin(), R.tab.end(), M.tabs.begin() + ts.first)) {
+                        off = ts.first;
+                        break;
+                    }
+                if (off < 0) {
+                    off = (int)M.tabs.size();
+                    tab_starts.push_back({off, R.tab.size()});
+                    M.tabs.insert(M.tabs.end(), R.tab.begin(), R.tab.end());
+                }
+                int tre = add_inv({kin, INV_TAB_RE, off, (double)R.model, 0.0, 0.0});
+                D.re = {{tre, -1, 1.0}};
+                if (R.model == M_INTERP) D.im = {{add_inv({kin, INV_TAB_IM, off, (double)R.model, 0.0, 0.0}), -1, 1.0}};
+                D.ncoef = 0;
+                D.stat = true;
                 break;
             }
             case M_GS: {

@@ -295,7 +295,8 @@ __device__ __forceinline__ void prep_body(const double* __restrict__ params, dou
             continue;
         }
         const DRes& R = a.res[B.res0 + r];
-        if (R.model == 1 || R.model == 5) continue;        // BW and Custom carry no automatic barrier
+        // BW, Custom and the table lineshapes (Hist, Interp) carry no automatic barrier
+        if (R.model == 1 || R.model == 5 || R.model == 6 || R.model == 7) continue;
         J m0 = (R.np > 0) ? res_param<NV, H>(R, 0, B, theta) : jc<NV, H>(1.0);
         J q0 = jq0ad(m0, m1, m2);
         if (n.has_bf && L >= 1 && L <= 6) K = jmul(K, jsqrt(jbfpoly(L, q0, n.bf_d)));
@@ -456,8 +457,12 @@ __device__ inline double2 block_psi(const AuxArgs& a, const DBlock& B, long long
             double x = (T.inv < 0) ? 1.0 : ev_inv(a.st, T.inv, e);
             if (t < DT.nre) dr += c * x; else di += c * x;
         }
-        double q = 1.0 / (dr * dr + di * di);
-        double xr = dr * q, xi = di * q;                  // 1 / (dr - i di)
+        double xr = dr, xi = di;                          // table lineshape: the factor itself
+        if (!DT.stat) {
+            const double q = 1.0 / (dr * dr + di * di);
+            xr = dr * q;                                  // 1 / (dr - i di)
+            xi = di * q;
+        }
         double nr = pr * xr - pi * xi, ni = pr * xi + pi * xr;
         pr = nr; pi = ni;
     }

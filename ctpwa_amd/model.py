@@ -22,6 +22,7 @@ line-oriented text document, see :meth:`Plan.to_text`.
 from __future__ import annotations
 
 import itertools
+import os
 import warnings
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
@@ -29,7 +30,7 @@ from typing import Dict, List, Optional, Tuple
 from .config import ConfigError, ConfigParser, Particle, ResonanceConfig
 
 # lineshape model codes shared with the native library
-MODEL_CODES = {"BWR": 0, "BW": 1, "ONE": 2, "Flatte": 3, "GS": 4, "Custom": 5}
+MODEL_CODES = {"BWR": 0, "BW": 1, "ONE": 2, "Flatte": 3, "GS": 4, "Custom": 5, "Hist": 6, "Interp": 7}
 # ordered parameter names per model (src/Resonance.cu:14-75)
 _MODEL_PARAM_SPECS = {
     "BWR": [("mass", False), ("width", False), ("r", True)],
@@ -38,6 +39,8 @@ _MODEL_PARAM_SPECS = {
     "Flatte": [("mass", False)],
     "GS": [("mass", False), ("width", False)],
     "Custom": [],
+    "Hist": [("mass", True)],
+    "Interp": [],
 }
 _EXTRA_PREFIX = {"Flatte": "g"}
 
@@ -128,6 +131,7 @@ class ResInst:
     node: int                # node of the topology whose mother is ``tag``
     slots: List[int] = field(default_factory=list)   # theta slot per param or -1
     expr: List[str] = field(default_factory=list)    # Custom: serialized expression
+    table: List[float] = field(default_factory=list)  # Hist / Interp: the shape table
 
 
 @dataclass
@@ -218,6 +222,8 @@ class Plan:
                 out.append(" ".join(toks))
                 if r.model == "Custom":
                     out.append("cexpr %d %s" % (len(r.expr), " ".join(r.expr)))
+                if r.table:
+                    out.append("rtab %d %s" % (len(r.table), " ".join(repr(float(x)) for x in r.table)))
         out.append("ntheta %d" % len(self.theta))
         for j, th in enumerate(self.theta):
             out.append("theta %d %r %r %r" % (j, th.init, th.lower, th.upper))
@@ -372,6 +378,75 @@ def _apply_var_equal(cfg: ConfigParser, blocks, theta) -> None:
         for inst in b.res:
             inst.slots = [(-1 if s < 0 else new_index[remap.get(s, s)]) for s in inst.slots]
     theta[:] = [theta[j] for j in keep]
+
+
+def _open_table(cfg: ConfigParser, name: str, path: str):
+    """A lineshape table file: as given (relative to the working directory, as the
+    reference opens it), else relative to the config file's directory."""
+    for cand in (path, os.path.join(cfg.base_dir, path)):
+        if os.path.isfile(cand):
+            return open(cand)
+    raise ConfigError("%s: cannot open table file %s" % (name, path))
+
+
+def _table(cfg: ConfigParser, rconf: ResonanceConfig, model: str) -> List[float]:
+    """Shape table of a Hist or Interp lineshape (src/Resonance.cu:76-188).
+
+    Hist: ``file`` (whitespace-separated values), ``bins`` (their number, >= 2) and
+    ``range`` [lo, hi] -> [lo, hi, n, values...].  Interp: ``file`` with rows x Re Im
+    -> [1, N, x_0, dx, re..., im...] when consecutive x differ by dx within 1e-10,
+    else [-2, N, x..., re..., im...]; the interpolation is linear (the reference's
+    configuration reader passes no method)."""
+    opt = rconf.options
+    if model == "Hist":
+        if not opt.get("file") or not opt.get("bins") or not opt.get("range"):
+            raise ConfigError("Hist %s requires file, bins and range" % rconf.name)
+        try:
+            n = int(opt["bins"])
+        except ValueError:
+            raise ConfigError("Hist %s: bins must be an integer" % rconf.name)
+        if n < 2:
+            raise ConfigError("Hist %s needs bins >= 2" % rconf.name)
+        rng = []
+        for tok in opt["range"].split(","):
+            try:
+                rng.append(float(tok))
+            except ValueError:
+                pass
+        if len(rng) != 2 or not rng[1] > rng[0]:
+            raise ConfigError("Hist %s needs range lo, hi with hi > lo" % rconf.name)
+        with _open_table(cfg, rconf.name, opt["file"]) as f:
+            vals = []
+            for tok in f.read().split():
+                try:
+                    vals.append(float(tok))
+                except ValueError:
+                    break
+        if len(vals) != n:
+            raise ConfigError("Hist %s: file %s has %d values, expected %d" % (rconf.name, opt["file"], len(vals), n))
+        return [rng[0], rng[1], float(n)] + vals
+    if not opt.get("file"):
+        raise ConfigError("Interp %s requires a file" % rconf.name)
+    with _open_table(cfg, rconf.name, opt["file"]) as f:
+        toks = f.read().split()
+    x, re_, im_ = [], [], []
+    for k in range(0, len(toks) - 2, 3):
+        try:
+            a, b, c = float(toks[k]), float(toks[k + 1]), float(toks[k + 2])
+        except ValueError:
+            break
+        x.append(a)
+        re_.append(b)
+        im_.append(c)
+    if len(x) < 2:
+        raise ConfigError("Interp %s: file %s needs at least two rows x Re Im" % (rconf.name, opt["file"]))
+    dx = x[1] - x[0]
+    uniform = all(abs(x[i] - x[i - 1] - dx) <= 1e-10 for i in range(1, len(x)))
+    if uniform:
+        if dx == 0.0:
+            raise ConfigError("Interp %s: repeated x values" % rconf.name)
+        return [1.0, float(len(x)), x[0], dx] + re_ + im_
+    return [-2.0, float(len(x))] + x + re_ + im_
 
 
 def _chain_identical_groups(cfg: ConfigParser, chain) -> List[Tuple[List[str], bool]]:
@@ -643,6 +718,8 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
                         raise ConfigError("Flatte %s needs one channel per coupling" % rconf.name)
                     chans = [(float(c[0]), float(c[1])) for c in rconf.channels]
                     inst = ResInst(rconf.name, key[0], model, vals, names, chans, node_k, [-1] * len(vals))
+                    if model in ("Hist", "Interp"):
+                        inst.table = _table(cfg, rconf, model)
                     if model == "Custom":
                         from . import customexpr
                         text = rconf.options.get("expression", rconf.options.get("expr"))

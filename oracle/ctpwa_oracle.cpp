@@ -475,7 +475,7 @@ static CJ gs_jet(double m, const Jet& m0, const Jet& g0, double q, const Jet& q0
 struct ONode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md1, md2; };
 struct OTopo { std::vector<ONode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
 struct ORes { int node, model; std::vector<double> params; std::vector<int> slots; std::vector<double> ch;
-              std::vector<std::string> expr; };
+              std::vector<std::string> expr; std::vector<double> tab; };
 struct OBlock { int topo, site, nsl; std::vector<ORes> res; };
 struct OPlan {
     int nfinal = 0, npol = 0;
@@ -528,6 +528,11 @@ static OPlan parse_plan(const char* text) {
             std::vector<std::string> toks(n);
             for (auto& t : toks) ls >> t;
             P.blocks.back().res.back().expr = toks;
+        } else if (key == "rtab") {
+            long long n; ls >> n;
+            std::vector<double> t((size_t)n);
+            for (auto& x : t) ls >> x;
+            P.blocks.back().res.back().tab = t;
         } else if (key == "theta") { int j; double a, lo, hi; ls >> j >> a >> lo >> hi; P.th_init.push_back(a); }
         else if (key == "ntheta") ls >> P.ntheta;
         else if (key == "namp") ls >> P.namp >> P.nfree;
@@ -786,6 +791,55 @@ struct CExpr {
     }
 };
 
+// Hist lineshape (include/ResModel.cuh:67-85, 153-157): table [m_min, m_max, n, v...];
+// for m_min < m < m_max the value interpolates linearly between v[i] and v[i+1] at
+// position x = n (m - m_min) / (m_max - m_min), with x held to [0, n - 2]; v[0] at and
+// below m_min, v[n-1] at and above m_max, 0 for a negative or NaN mass.
+static double hist_lookup(const std::vector<double>& t, double m) {
+    if (!(m >= 0.0)) return 0.0;
+    int n = (int)t[2];
+    if (n < 2 || !(t[1] > t[0])) return 0.0;
+    if (m <= t[0]) return t[3];
+    if (m >= t[1]) return t[3 + n - 1];
+    double x = std::min(std::max(n * (m - t[0]) / (t[1] - t[0]), 0.0), (double)(n - 2));
+    int i = (int)x;
+    double w = x - i;
+    return (1.0 - w) * t[3 + i] + w * t[3 + i + 1];
+}
+
+// Interp lineshape, linear (include/ResModel.cuh:226-290, src/Resonance.cu:76-128):
+// equal spacing [1, N, x0, dx, re, im]: segment i = trunc((x - x0) / dx), below 0 ->
+// segment 0 at its start, at or above N-1 -> segment N-2 at its end, else the fraction
+// within it; unequal spacing [-2, N, xs, re, im]: the segment by bisection, with the
+// fraction unclamped (linear extrapolation past the ends).
+static cd interp_lookup(const std::vector<double>& t, double x) {
+    int N = (int)t[1];
+    const double *re, *im;
+    int i;
+    double w;
+    if (t[0] >= 0) {
+        re = &t[4];
+        im = &t[4 + N];
+        double pos = (x - t[2]) / t[3];
+        i = (int)pos;                       // truncation toward zero
+        if (i < 0) { i = 0; w = 0.0; }
+        else if (i >= N - 1) { i = N - 2; w = 1.0; }
+        else w = pos - i;
+    } else {
+        const double* xs = &t[2];
+        re = &t[2 + N];
+        im = &t[2 + 2 * N];
+        int a = 0, b = N - 1;
+        while (b - a > 1) {
+            int c = (a + b) / 2;
+            if (x < xs[c]) b = c; else a = c;
+        }
+        i = a;
+        w = (x - xs[i]) / (xs[i + 1] - xs[i]);
+    }
+    return cd(re[i] + w * (re[i + 1] - re[i]), im[i] + w * (im[i + 1] - im[i]));
+}
+
 static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin,
                        const std::vector<int>& vars, const std::vector<double>& theta) {
     const OTopo& T = P.topos[B.topo];
@@ -798,13 +852,15 @@ static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin
         const ONode& n = T.nodes[k];
         int L = T.slc[c][k].second;
         double mm = kin[4 * k], qq = kin[4 * k + 1], md1 = kin[4 * k + 2], md2 = kin[4 * k + 3];
-        // q0 daughter masses: config mass, else the block resonance's mass parameter, else event mass
+        // q0 daughter masses: config mass, else the mass parameter of the block resonance
+        // occupying the daughter (1 when it has no parameters, src/AmpGen.cu:1504-1527),
+        // else the event mass
         auto dmass = [&](double cfg, int ref, double evm) -> Jet {
             if (cfg > 0) return Jet(cfg);
             if (ref < 0) {
                 int kk = -ref - 1;
-                if (res_at[kk] >= 0 && !B.res[res_at[kk]].params.empty())
-                    return res_param(B.res[res_at[kk]], 0, vars, theta);
+                if (res_at[kk] >= 0)
+                    return B.res[res_at[kk]].params.empty() ? Jet(1.0) : res_param(B.res[res_at[kk]], 0, vars, theta);
             }
             return Jet(evm);
         };
@@ -859,6 +915,15 @@ static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin
             CExpr ce{R.expr, 0, R, vars, theta, mm, qq};
             f = ce.eval();
             if (ce.i != R.expr.size()) throw std::runtime_error("custom expression: trailing tokens");
+            break;
+        }
+        case 6: {   // Hist: the table value, real, no vertex barrier
+            f = CJ{Jet(hist_lookup(R.tab, mm)), Jet(0.0)};
+            break;
+        }
+        case 7: {   // Interp: the table value, no vertex barrier
+            cd z = interp_lookup(R.tab, mm);
+            f = CJ{Jet(z.real()), Jet(z.imag())};
             break;
         }
         default: throw std::runtime_error("model not supported by the oracle");
@@ -1277,6 +1342,11 @@ void orc_gs(double m, double m0, double g0, double q, double q0, double mpi, dou
     g_nv = 0;
     CJ z = gs_jet(m, Jet(m0), Jet(g0), q, Jet(q0), mpi);
     out[0] = z.re.v; out[1] = z.im.v;
+}
+double orc_hist(const double* tab, int n, double m) { return hist_lookup(std::vector<double>(tab, tab + n), m); }
+void orc_interp(const double* tab, int n, double x, double* out) {
+    cd z = interp_lookup(std::vector<double>(tab, tab + n), x);
+    out[0] = z.real(); out[1] = z.imag();
 }
 void orc_node_tensor(const double* p1, int dj1, const double* p2, int dj2, int dj, int dS, int L, double* out) {
     std::vector<cd> amp(dj * dj1 * dj2);

@@ -61,6 +61,9 @@ def lib():
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.orc_flatte.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         L.orc_gs.argtypes = [ctypes.c_double] * 6 + [_dp]
+        L.orc_hist.restype = ctypes.c_double
+        L.orc_hist.argtypes = [_dp, ctypes.c_int, ctypes.c_double]
+        L.orc_interp.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp]
         L.orc_massive_trans.argtypes = [_dp, _dp, ctypes.c_int, _dp]
         L.orc_massless_trans.argtypes = [_dp, _dp, ctypes.c_int, _dp]
         L.orc_node_tensor.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
@@ -214,3 +217,17 @@ def massive_trans(P, p, dim):
 def massless_trans(P, p, dim):
     """The diagonal phase matrix of a massless daughter (before the helicity rotation)."""
     return _trans(lib().orc_massless_trans, P, p, dim)
+
+
+def hist(table, m):
+    """Hist lineshape value at mass m for a table [m_min, m_max, n, values...]."""
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    return lib().orc_hist(_ptr(t), len(t), float(m))
+
+
+def interp(table, x):
+    """Interp lineshape (linear) at x for a table [1, N, x0, dx, re, im] or [-2, N, xs, re, im]."""
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    out = np.zeros(2)
+    lib().orc_interp(_ptr(t), len(t), float(x), _ptr(out))
+    return complex(out[0], out[1])
