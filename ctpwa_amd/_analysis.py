@@ -13,6 +13,9 @@ pybind11 module (src/main.cu:90-131) and ``class analysis`` (src/Analysis.cu:126
 * ``getNVector() / getNFreeTheta() / getNParams() / getParamNames() /
   getFreeResParams() / getFitMode() / setFitMode() / getNPolarizations() /
   getAmplitudeNames() / isValid() / reCalcAmp(theta)``.
+* ``getPhspTensor() / getDataTensor() / getBkgTensor() / getBkgWeightsTensor() /
+  getSLVectors() / getConstraintsIndex() / getConstraintsValues()``; ``getSLAmpsTensor``
+  and ``saveSLAmps`` raise (the SL tensors are stored with barrier factors folded in).
 * ``getFitFractions(vector, hessian)`` and ``writeResult(params, filename,
   is_saved_weight, waves)``: post-fit outputs from the amplitude Gram
   (ctpwa_amd/results.py); ``writeResult`` writes a NumPy ``.npz`` instead of a ROOT file.
@@ -170,18 +173,66 @@ class analysis:
         return self._fit_mode
 
     def setFitMode(self, mode: int) -> None:
-        if mode != 1:
-            raise RuntimeError("only fit mode 1 (vspace) is built")
-        self._fit_mode = mode
+        """0 (chain x step) or 1 (vspace).  The reference builds its parameter mapping in
+        vspace at construction (src/Analysis.cu:4382-4478) and the mode read afterwards
+        changes no count or layout of that mapping (src/Analysis.cu:1309-1342), so both
+        values keep the vspace layout here."""
+        if mode not in (0, 1):
+            raise RuntimeError("fit mode must be 0 (chain x step) or 1 (vspace)")
+        self._fit_mode = int(mode)
 
     def reCalcAmp(self, theta) -> None:
-        """The reference recomputes its stored amplitudes at ``theta``
-        (src/AmpGen.cu:2344-2726).  Here amplitudes are evaluated from the theta part of
-        the parameter vector inside every call, so there is nothing to recompute; the
-        argument is validated only."""
-        t = torch.as_tensor(theta, dtype=torch.float64)
+        """Recompute the stored amplitudes at ``theta`` (src/AmpGen.cu:2344-2726).  Every
+        likelihood call evaluates the amplitudes from the theta part of its parameter
+        vector; this sets the theta that the amplitude getters (getDataTensor, ...) and
+        getFitFractions use until the next getNLL / getHessian."""
+        t = torch.as_tensor(theta, dtype=torch.float64).detach().cpu().reshape(-1)
         if t.numel() != self.plan.n_theta:
             raise RuntimeError("theta must have %d entries" % self.plan.n_theta)
+        self._theta = t.numpy().copy()
+
+    def getSLVectors(self) -> torch.Tensor:
+        """Number of SL amplitudes of every partial wave, int32 on the device
+        (src/Analysis.cu:1346-1351, src/Info.cu:288)."""
+        return torch.tensor([self.plan.blocks[b].nsl for b in self.plan.waves], dtype=torch.int32,
+                            device=self.device)
+
+    def getConstraintsIndex(self):
+        """Amplitude index groups of the legacy constraint mode.  The reference always
+        builds its coupling matrix when the model has amplitudes (src/Info.cu:322-336),
+        which leaves these empty (src/Analysis.cu:4352-4378)."""
+        return []
+
+    def getConstraintsValues(self):
+        return []
+
+    def getPhspTensor(self) -> torch.Tensor:
+        """Amplitudes of the phase-space sample at the current theta without couplings,
+        flattened [event][pol][amp] complex128 on the device (src/Analysis.cu:3641-3676);
+        identical-particle permutations are summed into their amplitude."""
+        return self._stored_amplitudes(0)
+
+    def getDataTensor(self) -> torch.Tensor:
+        return self._stored_amplitudes(1)
+
+    def getBkgTensor(self) -> torch.Tensor:
+        if self.n_events["bkg"] == 0:
+            return torch.empty(0, dtype=torch.complex128, device=self.device)
+        return self._stored_amplitudes(2)
+
+    def getBkgWeightsTensor(self) -> torch.Tensor:
+        """Background weights as float32 on the device (empty without weights)."""
+        w = self._weights["bkg"]
+        if self.n_events["bkg"] == 0 or w is None:
+            return torch.empty(0, dtype=torch.float32, device=self.device)
+        return torch.tensor(np.asarray(w, dtype=np.float32), device=self.device)
+
+    def getSLAmpsTensor(self):
+        raise RuntimeError("getSLAmpsTensor is not built: the per-event SL tensors are stored with the "
+                           "event parts of the barrier factors folded in, not as the bare angular tensors")
+
+    def saveSLAmps(self, filename: str):
+        raise RuntimeError("saveSLAmps is not built (see getSLAmpsTensor)")
 
     def getFitFractions(self, vector: torch.Tensor, hessian: Optional[torch.Tensor] = None) -> torch.Tensor:
         """Fit fractions FF_b = integral |P_b|^2 / sum_j integral |P_j|^2 over the
@@ -205,29 +256,55 @@ class analysis:
 
     def writeResult(self, params: torch.Tensor, filename: str, is_saved_weight: int = 0, waves=()) -> str:
         """Fit result over the phase-space sample as a NumPy .npz (the reference writes ROOT,
-        src/Analysis.cu:1355-1700): ``legends`` (one label per partial wave),
-        ``interference`` (symmetric: sum |P_b|^2 / total on the diagonal,
-        sum 2 Re(conj(P_b) P_b') / total off it), ``phsp_integral``, ``data_integral``;
-        with ``is_saved_weight=1`` also per-event ``totalweight`` and ``weight_<k>`` (k-th
-        partial wave), normalised to the data yield, and the final-state momenta
-        ``<particle>_{px,py,pz,E}``.  Returns the file written."""
+        src/Analysis.cu:1355-1700, src/ComputeResults.cu:49-185):
+
+        * ``legends`` (one label per partial wave);
+        * ``interference`` (symmetric: sum |P_b|^2 / total on the diagonal,
+          sum 2 Re(conj(P_b) P_b') / total off it);
+        * ``phsp_integral`` and ``data_integral``.
+
+        ``waves``, a subset of partial-wave indices, replaces the per-event total by
+        sum_pol |sum_{w in waves} P_w|^2 and the phase-space integral by its sum, as the
+        reference does; the interference matrix keeps the full total.  With
+        ``is_saved_weight=1`` the file also holds, normalised to the data yield by
+        data_integral / phsp_integral:
+        * the per-event ``totalweight`` and ``weight_<k>`` (k-th partial wave);
+        * ``interf_<i>_<j>`` (i < j), sum_pol 2 Re(P_i conj P_j), so that without a
+          subset totalweight = sum_k weight_k + sum_{i<j} interf_i_j;
+        * the final-state momenta ``<particle>_{px,py,pz,E}``.
+        Returns the file written."""
         from . import results as R
         self._check_params(params)
-        if len(waves):
-            raise RuntimeError("writeResult: wave subsets are not built")
+        nw = len(self.plan.waves)
+        sel = [int(w) for w in waves]
+        for w in sel:
+            if not 0 <= w < nw:
+                raise RuntimeError("waves index out of range: %d (partials = %d)" % (w, nw))
         p = params.detach().contiguous()
+        self._remember_theta(p)
         Q = self._gram(self._h, p, 0)
         X, total = R.interference_matrix(Q, self.plan, p.cpu().numpy())
         nd = float(np.sum(self._weights["data"])) if self._weights["data"] is not None else float(self.n_events["data"])
         di = R.data_integral(nd, self._weights["bkg"], self.n_events["bkg"])
+        integral = total
+        P = None
+        if sel or is_saved_weight == 1:
+            P = self.getPartialAmplitudes(p, 0)                       # [nw, N, nPol]
+            S = P[sel].sum(dim=0) if sel else P.sum(dim=0)
+            ev_total = (S.real ** 2 + S.imag ** 2).sum(dim=1)
+            if sel:
+                integral = float(ev_total.sum().item())
         out = {"legends": np.array(R.partial_names(self.plan)), "interference": X,
-               "phsp_integral": np.float64(total), "data_integral": np.float64(di)}
+               "phsp_integral": np.float64(integral), "data_integral": np.float64(di)}
         if is_saved_weight == 1:
-            norm = di / (total if total > 0 else 1e-30)
-            w = self.getPartialIntensities(p, 0).cpu().numpy() * norm
-            out["totalweight"] = w[-1]
-            for k in range(w.shape[0] - 1):
-                out["weight_%d" % k] = w[k]
+            norm = di / (integral if integral > 0 else 1e-30)
+            out["totalweight"] = (ev_total * norm).cpu().numpy()
+            for k in range(nw):
+                out["weight_%d" % k] = ((P[k].real ** 2 + P[k].imag ** 2).sum(dim=1) * norm).cpu().numpy()
+            for a in range(nw):
+                for b in range(a + 1, nw):
+                    t = 2.0 * (P[a].real * P[b].real + P[a].imag * P[b].imag).sum(dim=1) * norm
+                    out["interf_%d_%d" % (a, b)] = t.cpu().numpy()
             mom = self._samples["phsp"]
             for i, name in enumerate(self.plan.final_order):       # events hold (E, px, py, pz)
                 for c, comp in ((1, "px"), (2, "py"), (3, "pz"), (0, "E")):
@@ -247,6 +324,17 @@ class analysis:
             self._h.partial_intensities(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
                                         torch.cuda.current_stream(self.device).cuda_stream)
         return out
+
+    def getPartialAmplitudes(self, params: torch.Tensor, cls: int = 0) -> torch.Tensor:
+        """[n_partials, N, nPol] complex128 on the device: per event and polarisation the
+        coherent amplitude P_b of every partial wave (couplings applied)."""
+        self._check_params(params)
+        n = self.n_events[("phsp", "data", "bkg")[cls]]
+        out = torch.empty((len(self.plan.waves), n, self.plan.npol, 2), dtype=torch.float64, device=self.device)
+        if n:
+            self._h.partial_amplitudes(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
+                                       torch.cuda.current_stream(self.device).cuda_stream)
+        return torch.view_as_complex(out)
 
     def getAmplitudeGram(self, params: torch.Tensor, cls: int = 0) -> np.ndarray:
         """complex [n_amp, n_amp] sum over a sample of conj(A_a) A_a' (no couplings)."""
@@ -291,6 +379,24 @@ class analysis:
             raise RuntimeError("params must be on %s" % self.device)
         if params.dim() != 1 or params.numel() != self.getNParams():
             raise RuntimeError("params must be 1-D with %d entries" % self.getNParams())
+
+    def _stored_amplitudes(self, cls: int) -> torch.Tensor:
+        plan = self.plan
+        p = torch.cat([torch.zeros(2 * plan.n_free, dtype=torch.float64),
+                       torch.tensor(self._theta, dtype=torch.float64)]).to(self.device)
+        A = self.getAmplitudes(p, cls)
+        ref = plan.ref_amplitudes()
+        if len(ref) != plan.n_amp:
+            out = A[:, :, ref].clone()
+            for blk in plan.blocks:
+                if blk.sigma == 0:
+                    continue
+                orig = plan.blocks[blk.wave]
+                for c in range(blk.nsl):
+                    a, a0 = blk.site + c, orig.site + c
+                    out[:, :, ref.index(a0)] += (plan.amp_ratio[a] / plan.amp_ratio[a0]) * A[:, :, a]
+            A = out
+        return A.reshape(-1)
 
     def _remember_theta(self, p: torch.Tensor):
         self._theta = p[2 * self.plan.n_free:].detach().cpu().numpy().copy()

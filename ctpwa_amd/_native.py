@@ -22,7 +22,7 @@ EXPORTS = [
     "ctpwa_footprint_bytes", "ctpwa_hessian_workspace_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
     "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
-    "ctpwa_amplitude_gram", "ctpwa_partial_intensities",
+    "ctpwa_amplitude_gram", "ctpwa_partial_intensities", "ctpwa_partial_amplitudes",
 ]
 
 _lib = None
@@ -48,7 +48,8 @@ def lib():
     L.ctpwa_param_info.argtypes = [_vp, _dp, _dp, _dp]
     for f in ("ctpwa_nll_grad", "ctpwa_hessian"):
         getattr(L, f).argtypes = [_vp, _vp, _vp, _vp]
-    for f in ("ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_amplitude_gram", "ctpwa_partial_intensities"):
+    for f in ("ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_amplitude_gram", "ctpwa_partial_intensities",
+              "ctpwa_partial_amplitudes"):
         getattr(L, f).argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
     L.ctpwa_time_eval_kernel.argtypes = [_vp, _vp, ctypes.c_int, _vp, _dp]
     L.ctpwa_footprint_bytes.restype = _ll
@@ -206,9 +207,14 @@ class Handle:
                                          ctypes.c_void_p(stream)), "ctpwa_amplitude_gram")
 
     def partial_intensities(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
-        """[n_block + 1][n_cls]: per-event partial-wave intensities, then the total"""
+        """[n_wave + 1][n_cls]: per-event partial-wave intensities, then the total"""
         check(lib().ctpwa_partial_intensities(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
                                               ctypes.c_void_p(stream)), "ctpwa_partial_intensities")
+
+    def partial_amplitudes(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
+        """[n_wave][n_cls][n_pol][2]: per-event coherent partial-wave amplitudes"""
+        check(lib().ctpwa_partial_amplitudes(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
+                                              ctypes.c_void_p(stream)), "ctpwa_partial_amplitudes")
 
     def time_eval_kernel(self, params_ptr: int, iters: int, stream: int) -> float:
         ms = ctypes.c_double()

@@ -48,6 +48,8 @@ extern "C" int ctpwa_mfma_gram_ok(int);
 extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
                                                double*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_partial_amp(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
+                                               double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_gram_to_complex(const double*, int, double*, hipStream_t);
 extern "C" long long ctpwa_mfma_gram_part_doubles(int, int);
 extern "C" hipError_t ctpwa_launch_wgram_mfma(const double*, int, long long, long long, const double*, long long,
@@ -960,6 +962,20 @@ extern "C" int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_d
         e = ctpwa_launch_partial_int(&h->aux, h->wave_ptr, h->wave_blk, h->M.P.nwave, h->off[cls], h->n[cls],
                                      out_dev, s);
     if (e != hipSuccess) return fail(std::string("ctpwa_partial_intensities: ") + hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int ctpwa_partial_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
+    if (!h || !params_dev || !out_dev) return fail("ctpwa_partial_amplitudes: NULL argument");
+    if (cls < 0 || cls > 2) return fail("ctpwa_partial_amplitudes: class must be 0 (phsp), 1 (data) or 2 (bkg)");
+    if (!h->M.customs.empty()) return fail("ctpwa_partial_amplitudes: models with Custom lineshapes are not built");
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess)
+        e = ctpwa_launch_partial_amp(&h->aux, h->wave_ptr, h->wave_blk, h->M.P.nwave, h->off[cls], h->n[cls],
+                                     out_dev, s);
+    if (e != hipSuccess) return fail(std::string("ctpwa_partial_amplitudes: ") + hipGetErrorString(e));
     return 0;
 }
 

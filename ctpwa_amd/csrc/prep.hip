@@ -554,6 +554,36 @@ __global__ void k_partial_int(AuxArgs a, const int* __restrict__ wptr, const int
     out[(long long)nwave * n + i] = I;
 }
 
+// coherent partial-wave amplitudes: out[w][i][p] = sum over the wave's blocks of P_b
+// (couplings applied), interleaved complex, events [e0, e0 + n)
+__global__ void k_partial_amp(AuxArgs a, const int* __restrict__ wptr, const int* __restrict__ wblk, int nwave,
+                              long long e0, long long n, double2* __restrict__ out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    long long e = e0 + i;
+    for (int w = 0; w < nwave; ++w) {
+        for (int p = 0; p < a.npol; ++p) {
+            double wr = 0.0, wi = 0.0;
+            for (int k = wptr[w]; k < wptr[w + 1]; ++k) {
+                const DBlock& B = a.blocks[wblk[k]];
+                if (B.nsl <= 0) continue;
+                double2 psi = block_psi(a, B, e);
+                double yr = 0.0, yi = 0.0;
+                for (int c = 0; c < B.nsl; ++c) {
+                    const int am = B.site + c;
+                    const double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
+                    double2 s = ev_sl(a.st, (long long)a.amps[am].u * a.npol + p, e);
+                    yr += cr * s.x - ci * s.y;
+                    yi += cr * s.y + ci * s.x;
+                }
+                wr += psi.x * yr - psi.y * yi;
+                wi += psi.x * yi + psi.y * yr;
+            }
+            out[((long long)w * n + i) * a.npol + p] = make_double2(wr, wi);
+        }
+    }
+}
+
 __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __restrict__ out) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -654,6 +684,16 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_partial
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(ctpwa::k_partial_int, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, wptr, wblk,
                        nwave, e0, n, out);
+    return hipGetLastError();
+}
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_partial_amp(const ctpwa::AuxArgs* a, const int* wptr,
+                                                                                     const int* wblk, int nwave,
+                                                                                     long long e0, long long n, double* out,
+                                                                                     hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctpwa::k_partial_amp, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, wptr, wblk,
+                       nwave, e0, n, (double2*)out);
     return hipGetLastError();
 }
 

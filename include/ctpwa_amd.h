@@ -26,6 +26,7 @@
  *   ctpwa_hessian_workspace_bytes <- the part of that estimate getHessian's buffers take
  *   ctpwa_amplitude_gram <- the integrals of getFitFractions / writeResult's interference
  *   ctpwa_partial_intensities <- writeResult's per-event total and partial weights
+ *   ctpwa_partial_amplitudes  <- writeResult's subset totals and pair interference terms
  */
 #ifndef CTPWA_AMD_H
 #define CTPWA_AMD_H
@@ -111,6 +112,13 @@ int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, dou
  * plan's `bwave` record, or the block count when the plan has none.
  * Fails for models with Custom lineshapes */
 int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
+
+/* per-event coherent partial-wave amplitudes of class `cls` (couplings applied): device
+ * [n_wave][n_cls][n_pol][2], P_w summed over the wave's blocks as above.  The per-event
+ * quantities of writeResult that need phases (the total over a subset of waves and the
+ * pair interference terms 2 Re(P_i conj P_j), src/ComputeResults.cu:49-185,
+ * src/Analysis.cu:1355-1700) follow from them.  Fails for models with Custom lineshapes */
+int ctpwa_partial_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
 /* average duration in ms of the fused per-event kernel over `iters` (<= 10000) launches
  * at `params_dev`: a HIP event pair on `stream` around every launch; the prep kernel (and

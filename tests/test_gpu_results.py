@@ -102,6 +102,27 @@ def test_fit_fractions_and_write_result(tmp_path, oracle_lib):
     assert list(z["legends"]) == R.partial_names(plan)
     assert np.array_equal(z["%s_E" % plan.final_order[0]], ph[:, 0, 0])
     assert np.array_equal(z["%s_pz" % plan.final_order[2]], ph[:, 2, 3])
+    # per-event pair terms: weights are conserved event by event, and their sums are the
+    # off-diagonal interference entries
+    pairs = sum(z["interf_%d_%d" % (i, j)] for i in range(nb) for j in range(i + 1, nb))
+    wsum = sum(z["weight_%d" % k] for k in range(nb))
+    assert np.abs(wsum + pairs - tw).max() <= 1e-10 * tw.max()
+    assert abs(z["interf_0_1"].sum() / tw.sum() - X[0, 1]) <= 1e-9
+    # coherent partial-wave amplitudes against the oracle's amplitudes and couplings
+    v = R.coupling_vector(plan, p)
+    Pw = np.stack([Ap[:, :, B.site:B.site + B.nsl] @ v[B.site:B.site + B.nsl] for B in plan.blocks])
+    Pg = ana.getPartialAmplitudes(pt, 0).cpu().numpy()
+    assert Pg.shape == Pw.shape and np.abs(Pg - Pw).max() <= 1e-10 * np.abs(Pw).max()
+    # a subset of waves: per-event total sum_pol |P_0 + P_2|^2, integral over it
+    path = ana.writeResult(pt, os.path.join(str(tmp_path), "subset"), is_saved_weight=1, waves=[0, 2])
+    zs = np.load(path, allow_pickle=False)
+    ev = np.sum(np.abs(Pw[0] + Pw[2]) ** 2, axis=1)
+    assert abs(float(zs["phsp_integral"]) - ev.sum()) <= 1e-9 * ev.sum()
+    di = float(zs["data_integral"])
+    assert np.abs(zs["totalweight"] - ev * di / ev.sum()).max() <= 1e-9 * zs["totalweight"].max()
+    assert np.allclose(zs["interference"], X, rtol=1e-9, atol=1e-13)
+    with pytest.raises(RuntimeError):
+        ana.writeResult(pt, os.path.join(str(tmp_path), "bad"), waves=[nb])
 
 
 def test_fit_fractions_need_a_truth_sample(tmp_path):
