@@ -21,6 +21,7 @@
 
 namespace ctpwa {
 std::string generate_eval_kernel(const Model_& M);
+std::string generate_custom2_kernel(const Model_& M);
 }
 
 extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctpwa::DTopo*, int, const ctpwa::DNode*,
@@ -274,6 +275,12 @@ struct ctpwa_handle {
     int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0, hnwg = 0;
     double *hJ = nullptr, *hU = nullptr, *hZ = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
     double* hmpart = nullptr;              // per-wave tile partials of the matrix-core Gram
+    // models with Custom lineshapes: second-order factor rows for the Hessian, from a
+    // module compiled on the first Hessian call
+    hipModule_t mod2 = nullptr;
+    hipFunction_t fn_custom2 = nullptr;
+    double* cfb2 = nullptr;
+    int *hbcp = nullptr, *hbcl = nullptr;   // per-block Custom factor lists (CSR)
     // amplitude-Gram workspace (allocated on the first ctpwa_amplitude_gram call)
     bool agram_ready = false;
     long long gchunk = 0;
@@ -309,6 +316,8 @@ struct ctpwa_handle {
         int prev = 0;
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(device);
+        // the modules belong to the process-wide compile cache (shared by handles of one
+        // model), not to the handle
         for (void* p : allocs) (void)hipFree(p);
         (void)hipSetDevice(prev);
     }
@@ -732,6 +741,19 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->aux = AuxArgs{h->store(), h->table, (long long)M.off_c(), (long long)M.off_rec(),
                          d_amps, d_blocks, d_dens, d_dterms, d_terms, P.namp, P.npol};
         {
+            std::vector<int> bcp(1, 0), bcl;
+            for (size_t b = 0; b < P.blocks.size(); ++b) {
+                for (int c : M.block_customs[b]) bcl.push_back(c);
+                bcp.push_back((int)bcl.size());
+            }
+            h->hbcp = h->upload(bcp);
+            h->hbcl = h->upload(bcl);
+            h->aux.bcp = h->hbcp;
+            h->aux.bcl = h->hbcl;
+            h->aux.cld = h->ntot;
+            h->aux.ncf = M.customs.empty() ? 0 : M.ncf();
+        }
+        {
             std::vector<int> wptr(P.nwave + 1, 0), wblk;
             for (int w = 0; w < P.nwave; ++w) {
                 for (size_t b = 0; b < P.blocks.size(); ++b)
@@ -750,6 +772,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             HIPCHECK(hipModuleGetFunction(&h->fn_aux, h->mod, "ctpwa_aux"));
             HIPCHECK(hipModuleGetFunction(&h->fn_custom, h->mod, "ctpwa_custom"));
             h->cfb = h->alloc<double>((size_t)M.customs.size() * M.ncf() * h->ntot);
+            h->aux.cfb = h->cfb;
         }
         h->used_inv.assign(M.inv.size(), 0);
         for (const DenSpec& D : M.dens) {
@@ -853,7 +876,7 @@ static long long hess_mfma_part_doubles(const HessDims& d) {
     if (ctpwa_mfma_gram_ok(d.NU)) n = std::max(n, ctpwa_mfma_gram_part_doubles(d.NU, HESS_MFMA_NWG));
     return n;
 }
-static long long hess_workspace_bytes(const Model_& M, long long maxn) {
+static long long hess_workspace_bytes(const Model_& M, long long maxn, long long ntot) {
     HessDims d = hess_dims(M);
     long long chunk = std::min<long long>(maxn, HESS_CHUNK);
     long long nG = (long long)(2 * d.C) * (2 * d.C + 1) / 2, nUu = (long long)d.NU * (d.NU + 1) / 2;
@@ -861,7 +884,8 @@ static long long hess_workspace_bytes(const Model_& M, long long maxn) {
     long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + d.NZ + 2) + HESS_NWG * maxcol + 3 * (nG + nUu + d.NZ + d.NU) +
                   (long long)d.NU * d.NU;
     w += hess_mfma_part_doubles(d);
-    return w * 8 + (nG + nUu) * 8 + 10 * 256;    // + the index lists and allocation slack
+    w += (long long)M.customs.size() * M.ncf2() * ntot;   // second-order Custom factor rows
+    return w * 8 + (nG + nUu) * 8 + 12 * 256;    // + the index lists and allocation slack
 }
 
 static void hess_setup(ctpwa_handle* h) {
@@ -896,6 +920,11 @@ static void hess_setup(ctpwa_handle* h) {
     h->htot = h->alloc<double>((size_t)3 * (h->hnG + h->hnUu + h->hNZ + h->hNU));
     h->hHext = h->alloc<double>((size_t)h->hNU * h->hNU);
     h->hmpart = h->alloc<double>((size_t)std::max<long long>(1, hess_mfma_part_doubles(hd)));
+    if (!M.customs.empty()) {
+        h->mod2 = compile_eval(h->device, generate_custom2_kernel(M));
+        HIPCHECK(hipModuleGetFunction(&h->fn_custom2, h->mod2, "ctpwa_custom2"));
+        h->cfb2 = h->alloc<double>((size_t)M.customs.size() * M.ncf2() * h->ntot);
+    }
     h->hess_ready = true;
 }
 
@@ -920,7 +949,6 @@ static void agram_setup(ctpwa_handle* h) {
 extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
     if (!h || !params_dev || !out_dev) return fail("ctpwa_amplitude_gram: NULL argument");
     if (cls < 0 || cls > 2) return fail("ctpwa_amplitude_gram: class must be 0 (phsp), 1 (data) or 2 (bkg)");
-    if (!h->M.customs.empty()) return fail("ctpwa_amplitude_gram: models with Custom lineshapes are not built");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     try {
@@ -929,6 +957,7 @@ extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, i
         const int n2 = 2 * P.namp;
         const long long ldx = h->gchunk * P.npol;
         HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s));
+        HIPCHECK(launch_custom(h, s));
         HIPCHECK(ctpwa_launch_zero(h->gtot, h->gnG, s));
         for (long long s0 = 0; s0 < h->n[cls]; s0 += h->gchunk) {
             long long cnt = std::min(h->gchunk, h->n[cls] - s0);
@@ -954,10 +983,10 @@ extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, i
 extern "C" int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
     if (!h || !params_dev || !out_dev) return fail("ctpwa_partial_intensities: NULL argument");
     if (cls < 0 || cls > 2) return fail("ctpwa_partial_intensities: class must be 0 (phsp), 1 (data) or 2 (bkg)");
-    if (!h->M.customs.empty()) return fail("ctpwa_partial_intensities: models with Custom lineshapes are not built");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess)
         e = ctpwa_launch_partial_int(&h->aux, h->wave_ptr, h->wave_blk, h->M.P.nwave, h->off[cls], h->n[cls],
                                      out_dev, s);
@@ -968,10 +997,10 @@ extern "C" int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_d
 extern "C" int ctpwa_partial_amplitudes(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
     if (!h || !params_dev || !out_dev) return fail("ctpwa_partial_amplitudes: NULL argument");
     if (cls < 0 || cls > 2) return fail("ctpwa_partial_amplitudes: class must be 0 (phsp), 1 (data) or 2 (bkg)");
-    if (!h->M.customs.empty()) return fail("ctpwa_partial_amplitudes: models with Custom lineshapes are not built");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess)
         e = ctpwa_launch_partial_amp(&h->aux, h->wave_ptr, h->wave_blk, h->M.P.nwave, h->off[cls], h->n[cls],
                                      out_dev, s);
@@ -981,7 +1010,6 @@ extern "C" int ctpwa_partial_amplitudes(ctpwa_handle* h, const double* params_de
 
 extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream) {
     if (!h || !params_dev || !hess_dev) return fail("ctpwa_hessian: NULL argument");
-    if (!h->M.customs.empty()) return fail("ctpwa_hessian: the Hessian of models with Custom lineshapes is not built");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     try {
@@ -989,6 +1017,11 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         const Model_& M = h->M;
         const Plan& P = M.P;
         HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, M.maxloc, 1, s));
+        if (h->fn_custom2) {
+            void* args[] = {&h->table, &h->inv, &h->invf, &h->ntot, &h->cfb2};
+            unsigned grid = (unsigned)std::min<long long>(16384, (h->ntot + 63) / 64);
+            HIPCHECK(hipModuleLaunchKernel(h->fn_custom2, grid, 1, 1, 64, 1, 1, 0, s, args, nullptr));
+        }
         size_t per_class = (size_t)h->hnG + h->hnUu + h->hNZ + h->hNU;
         HIPCHECK(ctpwa_launch_zero(h->htot, (long long)(3 * per_class), s));
         HessArgs a{};
@@ -998,6 +1031,8 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         a.namp = P.namp; a.nblock = (int)P.blocks.size(); a.npol = P.npol; a.ntheta = P.ntheta;
         a.J = h->hJ; a.U = h->hU; a.Z = h->hZ; a.W = h->hW; a.ld = h->hchunk;
         a.C = h->hC; a.NU = h->hNU; a.NZ = h->hNZ;
+        a.cfb2 = h->cfb2; a.cld = h->ntot; a.ncf2 = M.customs.empty() ? 0 : M.ncf2(); a.ncv = M.ncv();
+        a.bcp = h->hbcp; a.bcl = h->hbcl;
         HessAsm as{};
         for (int c = 0; c < 3; ++c) {
             double* base = h->htot + c * per_class;
@@ -1257,7 +1292,7 @@ extern "C" long long ctpwa_hessian_workspace_bytes(const char* plan_text, long l
                                                    long long n_bkg) {
     try {
         Model_ M = derive(parse_plan(plan_text));
-        return hess_workspace_bytes(M, std::max(n_phsp, std::max(n_data, n_bkg)));
+        return hess_workspace_bytes(M, std::max(n_phsp, std::max(n_data, n_bkg)), n_phsp + n_data + n_bkg);
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_hessian_workspace_bytes: ") + e.what();
         return -1;
@@ -1311,20 +1346,25 @@ extern "C" long long ctpwa_compile_check(const char* plan_text, int precision) {
         Model_ M = derive(parse_plan(plan_text));
         M.mixed = precision;
         apply_codegen_env(M);
-        std::string src = generate_eval_kernel(M);
-        hiprtcProgram prog;
-        if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
-            throw std::runtime_error("hiprtcCreateProgram failed");
-        hiprtcResult r = hiprtcCompileProgram(prog, RTC_NOPTS, RTC_OPTS);
-        size_t ls = 0;
-        hiprtcGetProgramLogSize(prog, &ls);
-        std::string log(ls + 1, '\0');
-        hiprtcGetProgramLog(prog, &log[0]);
-        size_t cs = 0;
-        if (r == HIPRTC_SUCCESS) hiprtcGetCodeSize(prog, &cs);
-        hiprtcDestroyProgram(&prog);
-        if (r != HIPRTC_SUCCESS) throw std::runtime_error(log);
-        return (long long)cs;
+        auto compile = [](const std::string& src) -> long long {
+            hiprtcProgram prog;
+            if (hiprtcCreateProgram(&prog, src.c_str(), "ctpwa_eval.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+                throw std::runtime_error("hiprtcCreateProgram failed");
+            hiprtcResult r = hiprtcCompileProgram(prog, RTC_NOPTS, RTC_OPTS);
+            size_t ls = 0;
+            hiprtcGetProgramLogSize(prog, &ls);
+            std::string log(ls + 1, '\0');
+            hiprtcGetProgramLog(prog, &log[0]);
+            size_t cs = 0;
+            if (r == HIPRTC_SUCCESS) hiprtcGetCodeSize(prog, &cs);
+            hiprtcDestroyProgram(&prog);
+            if (r != HIPRTC_SUCCESS) throw std::runtime_error(log);
+            return (long long)cs;
+        };
+        long long cs = compile(generate_eval_kernel(M));
+        // models with Custom lineshapes: also the Hessian's second-order factor module
+        if (!M.customs.empty()) cs += compile(generate_custom2_kernel(M));
+        return cs;
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_compile_check: ") + e.what();
         return -1;

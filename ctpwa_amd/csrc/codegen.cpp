@@ -205,12 +205,232 @@ __device__ __forceinline__ CD cd_bwr(int L, const CD& m, const CD& m0, const CD&
 }
 )";
 
+// Complex numbers carrying first and second derivatives over NCV block-local parameters
+// (second-order forward mode) for the Hessian of models with Custom lineshapes.  Second
+// derivatives are packed over k <= l (NH = NCV (NCV + 1) / 2 entries).  Every function
+// f applies z_k = f'(a) a_k, z_kl = f'(a) a_kl + f''(a) a_k a_l with the same branches as
+// the first-order CD functions.
+static const char* CH_PREAMBLE = R"(
+#define NH (NCV * (NCV + 1) / 2)
+__device__ __forceinline__ int hx(int k, int l) { return k * NCV - (k * (k - 1)) / 2 + (l - k); }
+struct CH { double r, i; double dr[NCV], di[NCV]; double hr[NH], hi[NH]; };
+__device__ __forceinline__ CH ch_c(double r, double i) {
+  CH z; z.r = r; z.i = i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = 0.0; z.di[k] = 0.0; }
+  for (int h = 0; h < NH; ++h) { z.hr[h] = 0.0; z.hi[h] = 0.0; }
+  return z;
+}
+__device__ __forceinline__ CH ch_var(double v, int j) { CH z = ch_c(v, 0.0); z.dr[j] = 1.0; return z; }
+__device__ __forceinline__ CH ch_add(const CH& a, const CH& b) {
+  CH z; z.r = a.r + b.r; z.i = a.i + b.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = a.dr[k] + b.dr[k]; z.di[k] = a.di[k] + b.di[k]; }
+  for (int h = 0; h < NH; ++h) { z.hr[h] = a.hr[h] + b.hr[h]; z.hi[h] = a.hi[h] + b.hi[h]; }
+  return z;
+}
+__device__ __forceinline__ CH ch_sub(const CH& a, const CH& b) {
+  CH z; z.r = a.r - b.r; z.i = a.i - b.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = a.dr[k] - b.dr[k]; z.di[k] = a.di[k] - b.di[k]; }
+  for (int h = 0; h < NH; ++h) { z.hr[h] = a.hr[h] - b.hr[h]; z.hi[h] = a.hi[h] - b.hi[h]; }
+  return z;
+}
+__device__ __forceinline__ CH ch_neg(const CH& a) {
+  CH z; z.r = -a.r; z.i = -a.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = -a.dr[k]; z.di[k] = -a.di[k]; }
+  for (int h = 0; h < NH; ++h) { z.hr[h] = -a.hr[h]; z.hi[h] = -a.hi[h]; }
+  return z;
+}
+// (ar + i ai) (br + i bi)
+#define CMR(ar, ai, br, bi) ((ar) * (br) - (ai) * (bi))
+#define CMI(ar, ai, br, bi) ((ar) * (bi) + (ai) * (br))
+__device__ __forceinline__ CH ch_mul(const CH& a, const CH& b) {
+  CH z; z.r = CMR(a.r, a.i, b.r, b.i); z.i = CMI(a.r, a.i, b.r, b.i);
+  for (int k = 0; k < NCV; ++k) {
+    z.dr[k] = CMR(a.dr[k], a.di[k], b.r, b.i) + CMR(a.r, a.i, b.dr[k], b.di[k]);
+    z.di[k] = CMI(a.dr[k], a.di[k], b.r, b.i) + CMI(a.r, a.i, b.dr[k], b.di[k]);
+  }
+  for (int k = 0; k < NCV; ++k)
+    for (int l = k; l < NCV; ++l) {
+      const int h = hx(k, l);
+      z.hr[h] = CMR(a.hr[h], a.hi[h], b.r, b.i) + CMR(a.dr[k], a.di[k], b.dr[l], b.di[l]) +
+                CMR(a.dr[l], a.di[l], b.dr[k], b.di[k]) + CMR(a.r, a.i, b.hr[h], b.hi[h]);
+      z.hi[h] = CMI(a.hr[h], a.hi[h], b.r, b.i) + CMI(a.dr[k], a.di[k], b.dr[l], b.di[l]) +
+                CMI(a.dr[l], a.di[l], b.dr[k], b.di[k]) + CMI(a.r, a.i, b.hr[h], b.hi[h]);
+    }
+  return z;
+}
+// value f(a) with f'(a) = p, f''(a) = q (complex)
+__device__ __forceinline__ CH ch_chain(double fr, double fi, double pr, double pi, double qr, double qi, const CH& a) {
+  CH z; z.r = fr; z.i = fi;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = CMR(pr, pi, a.dr[k], a.di[k]); z.di[k] = CMI(pr, pi, a.dr[k], a.di[k]); }
+  for (int k = 0; k < NCV; ++k)
+    for (int l = k; l < NCV; ++l) {
+      const int h = hx(k, l);
+      const double ur = CMR(a.dr[k], a.di[k], a.dr[l], a.di[l]), ui = CMI(a.dr[k], a.di[k], a.dr[l], a.di[l]);
+      z.hr[h] = CMR(pr, pi, a.hr[h], a.hi[h]) + CMR(qr, qi, ur, ui);
+      z.hi[h] = CMI(pr, pi, a.hr[h], a.hi[h]) + CMI(qr, qi, ur, ui);
+    }
+  return z;
+}
+__device__ __forceinline__ CH ch_inv(const CH& a) {
+  const double q = 1.0 / (a.r * a.r + a.i * a.i);
+  const double vr = a.r * q, vi = -a.i * q;                   // 1/a
+  const double v2r = vr * vr - vi * vi, v2i = 2.0 * vr * vi;  // 1/a^2
+  return ch_chain(vr, vi, -v2r, -v2i, 2.0 * CMR(v2r, v2i, vr, vi), 2.0 * CMI(v2r, v2i, vr, vi), a);
+}
+__device__ __forceinline__ CH ch_div(const CH& a, const CH& b) { return ch_mul(a, ch_inv(b)); }
+__device__ __forceinline__ CH ch_exp(const CH& a) {
+  const double e = exp(a.r), vr = e * cos(a.i), vi = e * sin(a.i);
+  return ch_chain(vr, vi, vr, vi, vr, vi, a);
+}
+__device__ __forceinline__ CH ch_log(const CH& a) {
+  const double q = 1.0 / (a.r * a.r + a.i * a.i);
+  const double ur = a.r * q, ui = -a.i * q;                   // 1/a
+  return ch_chain(0.5 * log(a.r * a.r + a.i * a.i), atan2(a.i, a.r), ur, ui, -(ur * ur - ui * ui), -2.0 * ur * ui, a);
+}
+__device__ __forceinline__ CH ch_pow(const CH& a, const CH& b) { return ch_exp(ch_mul(b, ch_log(a))); }
+__device__ __forceinline__ CH ch_sin(const CH& a) {
+  const double ep = exp(a.i), em = exp(-a.i);
+  const double sr = sin(a.r) * (ep + em) * 0.5, si = cos(a.r) * (ep - em) * 0.5;
+  const double cr = cos(a.r) * (ep + em) * 0.5, ci = -sin(a.r) * (ep - em) * 0.5;
+  return ch_chain(sr, si, cr, ci, -sr, -si, a);
+}
+__device__ __forceinline__ CH ch_cos(const CH& a) {
+  const double ep = exp(a.i), em = exp(-a.i);
+  const double sr = sin(a.r) * (ep + em) * 0.5, si = cos(a.r) * (ep - em) * 0.5;
+  const double cr = cos(a.r) * (ep + em) * 0.5, ci = -sin(a.r) * (ep - em) * 0.5;
+  return ch_chain(cr, ci, -sr, -si, -cr, -ci, a);
+}
+__device__ __forceinline__ CH ch_sqrt(const CH& a) {
+  double vr, vi;
+  if (a.i == 0.0) {
+    if (a.r >= 0.0) { vr = sqrt(a.r); vi = 0.0; } else { vr = 0.0; vi = sqrt(-a.r); }
+  } else {
+    const double r = sqrt(a.r * a.r + a.i * a.i);
+    vr = sqrt((r + a.r) * 0.5);
+    vi = sqrt((r - a.r) * 0.5);
+    if (a.i < 0.0) vi = -vi;
+  }
+  // f' = 1 / (2 sqrt a), f'' = -1 / (4 sqrt(a)^3)
+  const double q = 1.0 / (vr * vr + vi * vi);
+  const double ur = vr * q, ui = -vi * q;                     // 1/sqrt(a)
+  const double u2r = ur * ur - ui * ui, u2i = 2.0 * ur * ui;
+  return ch_chain(vr, vi, 0.5 * ur, 0.5 * ui, -0.25 * CMR(u2r, u2i, ur, ui), -0.25 * CMI(u2r, u2i, ur, ui), a);
+}
+__device__ __forceinline__ CH ch_abs(const CH& a) {
+  const double v = sqrt(a.r * a.r + a.i * a.i);
+  CH z = ch_c(v, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = (a.r * a.dr[k] + a.i * a.di[k]) / v;
+  for (int k = 0; k < NCV; ++k)
+    for (int l = k; l < NCV; ++l) {
+      const int h = hx(k, l);
+      z.hr[h] = (a.dr[k] * a.dr[l] + a.di[k] * a.di[l] + a.r * a.hr[h] + a.i * a.hi[h] - z.dr[k] * z.dr[l]) / v;
+    }
+  return z;
+}
+__device__ __forceinline__ CH ch_re(const CH& a) {
+  CH z = ch_c(a.r, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = a.dr[k];
+  for (int h = 0; h < NH; ++h) z.hr[h] = a.hr[h];
+  return z;
+}
+__device__ __forceinline__ CH ch_im(const CH& a) {
+  CH z = ch_c(a.i, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = a.di[k];
+  for (int h = 0; h < NH; ++h) z.hr[h] = a.hi[h];
+  return z;
+}
+__device__ __forceinline__ CH ch_conj(const CH& a) {
+  CH z; z.r = a.r; z.i = -a.i;
+  for (int k = 0; k < NCV; ++k) { z.dr[k] = a.dr[k]; z.di[k] = -a.di[k]; }
+  for (int h = 0; h < NH; ++h) { z.hr[h] = a.hr[h]; z.hi[h] = -a.hi[h]; }
+  return z;
+}
+// real function g(x) of the real part of a with g' = p, g'' = q
+__device__ __forceinline__ CH ch_real_chain(double g, double p, double q, const CH& a) {
+  return ch_chain(g, 0.0, p, 0.0, q, 0.0, ch_re(a));
+}
+// breakup at x with constant daughter masses (computeQ0AD), 0 below threshold
+__device__ __forceinline__ CH ch_q0(const CH& x, const CH& m1, const CH& m2) {
+  const double s = m1.r + m2.r, d = m1.r - m2.r, x2 = x.r * x.r;
+  const double q2 = (x2 - s * s) * (x2 - d * d) / (4.0 * x2);
+  if (q2 < 0.0) return ch_c(0.0, 0.0);
+  const double q = sqrt(q2);
+  if (q == 0.0) return ch_c(0.0, 0.0);
+  const double dq2 = (x2 * x2 - s * s * d * d) / (2.0 * x2 * x.r);          // dq2/dx
+  const double d2q2 = 0.5 + 1.5 * s * s * d * d / (x2 * x2);                  // d2q2/dx2
+  return ch_real_chain(q, dq2 / (2.0 * q), d2q2 / (2.0 * q) - dq2 * dq2 / (4.0 * q * q2), x);
+}
+__device__ __forceinline__ void bfpoly_d2(int L, double z, double& n, double& dn, double& d2n) {
+  // N_L(z) = P(z^2): N' = 2 z P'(u), N'' = 2 P'(u) + 4 z^2 P''(u), u = z^2
+  const double u = z * z;
+  double p, dp, d2p;
+  switch (L) {
+  case 1: p = 1.0 + u; dp = 1.0; d2p = 0.0; break;
+  case 2: p = 9.0 + u * (3.0 + u); dp = 3.0 + 2.0 * u; d2p = 2.0; break;
+  case 3: p = 225.0 + u * (45.0 + u * (6.0 + u)); dp = 45.0 + u * (12.0 + 3.0 * u); d2p = 12.0 + 6.0 * u; break;
+  case 4: p = 11025.0 + u * (1575.0 + u * (135.0 + u * (10.0 + u)));
+          dp = 1575.0 + u * (270.0 + u * (30.0 + 4.0 * u)); d2p = 270.0 + u * (60.0 + 12.0 * u); break;
+  case 5: p = 893025.0 + u * (99225.0 + u * (6300.0 + u * (315.0 + u * (15.0 + u))));
+          dp = 99225.0 + u * (12600.0 + u * (945.0 + u * (60.0 + 5.0 * u)));
+          d2p = 12600.0 + u * (1890.0 + u * (180.0 + 20.0 * u)); break;
+  case 6: p = 540326025.0 + u * (6185025.0 + u * (363825.0 + u * (17325.0 + u * (630.0 + u * (21.0 + u)))));
+          dp = 6185025.0 + u * (727650.0 + u * (51975.0 + u * (2520.0 + u * (105.0 + 6.0 * u))));
+          d2p = 727650.0 + u * (103950.0 + u * (7560.0 + u * (420.0 + 30.0 * u))); break;
+  default: p = 1.0; dp = 0.0; d2p = 0.0;
+  }
+  n = p;
+  dn = 2.0 * z * dp;
+  d2n = 2.0 * dp + 4.0 * u * d2p;
+}
+// Bf(L, q, q0, d) = sqrt(N_L(q0 d) / N_L(q d)) = exp(g), g = (ln N(z0) - ln N(z1)) / 2
+__device__ __forceinline__ CH ch_bf(int L, const CH& q, const CH& q0, const CH& d) {
+  if (L == 0 || L > 6) return ch_c(1.0, 0.0);
+  double n0, dn0, d2n0, n1, dn1, d2n1;
+  bfpoly_d2(L, q0.r * d.r, n0, dn0, d2n0);
+  bfpoly_d2(L, q.r * d.r, n1, dn1, d2n1);
+  const double v = sqrt(n0 / n1);
+  const double a0 = dn0 / n0, b0 = d2n0 / n0 - a0 * a0, a1 = dn1 / n1, b1 = d2n1 / n1 - a1 * a1;
+  double z0[NCV], z1[NCV], g[NCV];
+  for (int k = 0; k < NCV; ++k) {
+    z0[k] = q0.dr[k] * d.r + q0.r * d.dr[k];
+    z1[k] = q.dr[k] * d.r + q.r * d.dr[k];
+    g[k] = 0.5 * (a0 * z0[k] - a1 * z1[k]);
+  }
+  CH z = ch_c(v, 0.0);
+  for (int k = 0; k < NCV; ++k) z.dr[k] = v * g[k];
+  for (int k = 0; k < NCV; ++k)
+    for (int l = k; l < NCV; ++l) {
+      const int h = hx(k, l);
+      const double z0kl = q0.hr[h] * d.r + q0.dr[k] * d.dr[l] + q0.dr[l] * d.dr[k] + q0.r * d.hr[h];
+      const double z1kl = q.hr[h] * d.r + q.dr[k] * d.dr[l] + q.dr[l] * d.dr[k] + q.r * d.hr[h];
+      const double gkl = 0.5 * (b0 * z0[k] * z0[l] + a0 * z0kl - b1 * z1[k] * z1[l] - a1 * z1kl);
+      z.hr[h] = v * (gkl + g[k] * g[l]);
+    }
+  return z;
+}
+__device__ __forceinline__ CH ch_bw(const CH& m, const CH& m0, const CH& g0) {
+  return ch_inv(ch_sub(ch_sub(ch_mul(ch_re(m0), ch_re(m0)), ch_mul(ch_re(m), ch_re(m))),
+                       ch_mul(ch_c(0.0, 1.0), ch_mul(ch_re(m0), ch_re(g0)))));
+}
+__device__ __forceinline__ CH ch_bwr(int L, const CH& m, const CH& m0, const CH& g0, const CH& q, const CH& q0,
+                                     const CH& d) {
+  const CH bf = ch_bf(L, q, q0, d);
+  CH r = ch_div(ch_re(q), ch_re(q0));
+  CH p = ch_c(1.0, 0.0);
+  for (int k = 0; k < 2 * L + 1; ++k) p = ch_mul(p, r);
+  const CH gam = ch_mul(ch_mul(ch_mul(ch_re(g0), p), ch_div(ch_re(m0), ch_re(m))), ch_mul(bf, bf));
+  return ch_inv(ch_sub(ch_sub(ch_mul(ch_re(m0), ch_re(m0)), ch_mul(ch_re(m), ch_re(m))),
+                       ch_mul(ch_c(0.0, 1.0), ch_mul(ch_re(m0), gam))));
+}
+)";
+
 // code of one Custom expression (prefix tokens) as a nested CD expression
+// (prefix "cd_": first-order duals; "ch_": second-order duals)
 static std::string custom_code(const std::vector<std::string>& t, size_t& i, const CustomSpec& C,
-                               const std::vector<int>& vars) {
+                               const std::vector<int>& vars, const std::string& X = "cd_") {
     if (i >= t.size()) throw std::runtime_error("Custom expression is truncated");
     const std::string k = t[i++];
-    auto sub = [&]() { return custom_code(t, i, C, vars); };
+    auto sub = [&]() { return custom_code(t, i, C, vars, X); };
     auto num = [](const std::string& x) {
         std::ostringstream s;
         s.precision(17);
@@ -219,46 +439,85 @@ static std::string custom_code(const std::vector<std::string>& t, size_t& i, con
         if (r.find_first_of(".eEn") == std::string::npos) r += ".0";
         return r;
     };
-    if (k.rfind("n:", 0) == 0) return "cd_c(" + num(k.substr(2)) + ", 0.0)";
-    if (k == "I") return "cd_c(0.0, 1.0)";
-    if (k == "v:m") return "cd_c(xm, 0.0)";
-    if (k == "v:q") return "cd_c(xq, 0.0)";
+    if (k.rfind("n:", 0) == 0) return X + "c(" + num(k.substr(2)) + ", 0.0)";
+    if (k == "I") return X + "c(0.0, 1.0)";
+    if (k == "v:m") return X + "c(xm, 0.0)";
+    if (k == "v:q") return X + "c(xq, 0.0)";
     if (k.rfind("p:", 0) == 0) {
         int pk = std::stoi(k.substr(2));
         if (pk < 0 || pk >= (int)C.param_val.size()) throw std::runtime_error("Custom parameter index out of range");
         int s = C.param_slot[pk];
-        if (s < 0) return "cd_c(" + num(std::to_string(C.param_val[pk])) + ", 0.0)";
+        if (s < 0) return X + "c(" + num(std::to_string(C.param_val[pk])) + ", 0.0)";
         for (size_t li = 0; li < vars.size(); ++li)
-            if (vars[li] == s) return "cd_var(tb[OFFT + " + std::to_string(s) + "], " + std::to_string(li) + ")";
+            if (vars[li] == s) return X + "var(tb[OFFT + " + std::to_string(s) + "], " + std::to_string(li) + ")";
         throw std::runtime_error("Custom parameter slot is not a block variable");
     }
-    if (k == "+") { std::string a = sub(), b = sub(); return "cd_add(" + a + ", " + b + ")"; }
-    if (k == "-") { std::string a = sub(), b = sub(); return "cd_sub(" + a + ", " + b + ")"; }
-    if (k == "*") { std::string a = sub(), b = sub(); return "cd_mul(" + a + ", " + b + ")"; }
-    if (k == "/") { std::string a = sub(), b = sub(); return "cd_div(" + a + ", " + b + ")"; }
-    if (k == "^") { std::string a = sub(), b = sub(); return "cd_pow(" + a + ", " + b + ")"; }
-    if (k == "neg") return "cd_neg(" + sub() + ")";
+    if (k == "+") { std::string a = sub(), b = sub(); return X + "add(" + a + ", " + b + ")"; }
+    if (k == "-") { std::string a = sub(), b = sub(); return X + "sub(" + a + ", " + b + ")"; }
+    if (k == "*") { std::string a = sub(), b = sub(); return X + "mul(" + a + ", " + b + ")"; }
+    if (k == "/") { std::string a = sub(), b = sub(); return X + "div(" + a + ", " + b + ")"; }
+    if (k == "^") { std::string a = sub(), b = sub(); return X + "pow(" + a + ", " + b + ")"; }
+    if (k == "neg") return X + "neg(" + sub() + ")";
     if (k.rfind("f:", 0) == 0) {
         std::string f = k.substr(2);
         static const char* ok[] = {"exp", "log", "sin", "cos", "sqrt", "abs", "re", "im", "conj"};
         bool known = false;
         for (const char* o : ok) if (f == o) known = true;
         if (!known) throw std::runtime_error("Custom expression: unknown function " + f);
-        return "cd_" + f + "(" + sub() + ")";
+        return X + f + "(" + sub() + ")";
     }
-    if (k == "q0") { std::string a = sub(), b = sub(), c = sub(); return "cd_q0(" + a + ", " + b + ", " + c + ")"; }
+    if (k == "q0") { std::string a = sub(), b = sub(), c = sub(); return X + "q0(" + a + ", " + b + ", " + c + ")"; }
     if (k.rfind("bf:", 0) == 0) {
         std::string L = std::to_string(std::stoi(k.substr(3)));
         std::string a = sub(), b = sub(), c = sub();
-        return "cd_bf(" + L + ", " + a + ", " + b + ", " + c + ")";
+        return X + "bf(" + L + ", " + a + ", " + b + ", " + c + ")";
     }
-    if (k == "bw") { std::string a = sub(), b = sub(), c = sub(); return "cd_bw(" + a + ", " + b + ", " + c + ")"; }
+    if (k == "bw") { std::string a = sub(), b = sub(), c = sub(); return X + "bw(" + a + ", " + b + ", " + c + ")"; }
     if (k.rfind("bwr:", 0) == 0) {
         std::string L = std::to_string(std::stoi(k.substr(4)));
         std::string a = sub(), b = sub(), c = sub(), d = sub(), e = sub(), f = sub();
-        return "cd_bwr(" + L + ", " + a + ", " + b + ", " + c + ", " + d + ", " + e + ", " + f + ")";
+        return X + "bwr(" + L + ", " + a + ", " + b + ", " + c + ", " + d + ", " + e + ", " + f + ")";
     }
     throw std::runtime_error("Custom expression: unknown token " + k);
+}
+
+// The second-order Custom factors for the Hessian, in a module of their own (compiled on
+// the first Hessian call): rows [c * NCF2 + (r, i, dr[NCV], di[NCV], hr[NH], hi[NH])]
+// of the output for every stored event.
+std::string generate_custom2_kernel(const Model_& M) {
+    if (M.customs.empty()) throw std::runtime_error("model has no Custom lineshapes");
+    std::ostringstream o;
+    const int ncv = M.ncv(), nh = ncv * (ncv + 1) / 2, ncf2 = 2 + 2 * ncv + 2 * nh;
+    o << "// generated: second-order Custom lineshape factors\n"
+      << "#define NCV " << ncv << "\n#define OFFT " << M.off_th() << "LL\n" << CH_PREAMBLE;
+    std::vector<int> imap = M.imap();
+    auto inv_load = [&](size_t i) {
+        std::ostringstream q;
+        if (imap[i] >= 0) q << "inv[" << imap[i] << "LL * ld + e]";
+        else q << "(double)invf[" << (-imap[i] - 1) << "LL * ld + e]";
+        return q.str();
+    };
+    o << "extern \"C\" __global__ void __launch_bounds__(64) ctpwa_custom2(\n"
+         "    const double* __restrict__ tab, const double* __restrict__ inv, const float* __restrict__ invf,\n"
+         "    long long ld, double* __restrict__ out)\n{\n"
+         "  const double* tb = tab;\n"
+         "  for (long long e = (long long)blockIdx.x * 64 + threadIdx.x; e < ld; e += (long long)gridDim.x * 64) {\n";
+    for (size_t c = 0; c < M.customs.size(); ++c) {
+        const CustomSpec& C = M.customs[c];
+        size_t ti = 0;
+        std::string code = custom_code(C.expr, ti, C, M.bvars[C.block], "ch_");
+        if (ti != C.expr.size()) throw std::runtime_error("Custom expression has trailing tokens");
+        const long long r0 = (long long)c * ncf2;
+        o << "    {\n      const double xm = " << inv_load(C.inv_m) << ", xq = " << inv_load(C.inv_q) << ";\n"
+          << "      const CH F = " << code << ";\n"
+          << "      out[" << r0 << "LL * ld + e] = F.r; out[" << r0 + 1 << "LL * ld + e] = F.i;\n"
+          << "      for (int k = 0; k < NCV; ++k) { out[(" << r0 + 2 << "LL + k) * ld + e] = F.dr[k]; out[(" << r0 + 2
+          << "LL + NCV + k) * ld + e] = F.di[k]; }\n"
+          << "      for (int h = 0; h < NH; ++h) { out[(" << r0 + 2 + 2 * ncv << "LL + h) * ld + e] = F.hr[h]; out[("
+          << r0 + 2 + 2 * ncv << "LL + NH + h) * ld + e] = F.hi[h]; }\n    }\n";
+    }
+    o << "  }\n}\n";
+    return o.str();
 }
 
 std::string generate_eval_kernel(const Model_& M) {

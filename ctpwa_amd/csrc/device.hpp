@@ -147,6 +147,13 @@ struct AuxArgs {
     const DDenTerms* dterms;
     const DTerm* terms;
     int namp, npol;
+    // Custom factor values per stored event (rows c * ncf * cld and the next one: real,
+    // imaginary), block b's factors bcl[bcp[b] .. bcp[b+1])
+    const double* cfb;
+    long long cld;
+    int ncf;
+    const int* bcp;
+    const int* bcl;
 };
 
 // Hessian: per-event rows of one chunk of one class (csrc/hessian.hip)
@@ -173,6 +180,14 @@ struct HessArgs {
     double* W;               // [2][ld] alpha, beta
     long long ld;            // row capacity of a chunk
     int C, NU, NZ;
+    // Custom factors with first and second derivatives per stored event: factor c's rows
+    // start at c * ncf2 * cld (value, dr[ncv], di[ncv], hr[nh], hi[nh]); block b's
+    // factors are bcl[bcp[b] .. bcp[b+1])
+    const double* cfb2;
+    long long cld;
+    int ncf2, ncv;
+    const int* bcp;
+    const int* bcl;
 };
 
 // Hessian assembly from the per-class sums

@@ -44,12 +44,17 @@ __device__ inline double rec_h(const HessArgs& a, int rec, int i, int k) {
     return a.table[a.off_rec + (long long)rec * REC + 1 + NLOC + i * NLOC + k];
 }
 
-// propagator product of a block and its log-derivatives:
-//   d Psi / d theta_i = -Psi L_i,   d2 Psi = Psi (L_i L_k - M_ik)
+// the factor product of a block and its derivatives over the block's local variables:
+//   psi, P1_i = -d psi / d theta_i, P2_ik = d2 psi / d theta_i d theta_k.
+// The propagators enter through their log-derivatives (psi = prod 1/D_d, L_i = sum
+// D_i / D, M_ik = sum (D_ik / D - D_i D_k / D^2): P1 = psi L, P2 = psi (L L - M)); each
+// Custom factor F (value, F_i, F_ik per event, second-order rows) then multiplies in by
+// the product rule, so a factor that vanishes needs no division.
 // NL bounds the block's local variables (B.nloc <= NL): every array is indexed with
 // unrolled constant indices and lives in registers.
 template <int NL>
-__device__ inline void block_terms(const HessArgs& a, const DBlock& B, long long e, Cd& psi, Cd* L, Cd* Mm) {
+__device__ inline void block_terms(const HessArgs& a, int b, long long e, Cd& psi, Cd* L, Cd* Mm) {
+    const DBlock& B = a.blocks[b];
     const int nl = B.nloc;
     psi = {1.0, 0.0};
 #pragma unroll
@@ -103,21 +108,59 @@ __device__ inline void block_terms(const HessArgs& a, const DBlock& B, long long
             }
         }
     }
+    // log-derivatives -> derivatives of psi (P2 first: it reads L)
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+        if (i >= nl) break;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) {
+            if (k >= nl) break;
+            Mm[i * NL + k] = cm(psi, cs(cm(L[i], L[k]), Mm[i * NL + k]));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NL; ++i) L[i] = (i < nl) ? cm(psi, L[i]) : Cd{0.0, 0.0};
+    // Custom factors of the block
+    for (int q = a.bcp[b]; q < a.bcp[b + 1]; ++q) {
+        const double* f = a.cfb2 + (long long)a.bcl[q] * a.ncf2 * a.cld + e;
+        const long long ld = a.cld;
+        const int ncv = a.ncv, nh = ncv * (ncv + 1) / 2;
+        const Cd F{f[0], f[ld]};
+        Cd Fi[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i)
+            Fi[i] = (i < nl) ? Cd{f[(2 + i) * ld], f[(2 + ncv + i) * ld]} : Cd{0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            if (i >= nl) break;
+#pragma unroll
+            for (int k = 0; k < NL; ++k) {
+                if (k >= nl) break;
+                const int lo = i < k ? i : k, hi = i < k ? k : i;
+                const int h = lo * ncv - (lo * (lo - 1)) / 2 + (hi - lo);
+                const Cd Fik{f[(2 + 2 * ncv + h) * ld], f[(2 + 2 * ncv + nh + h) * ld]};
+                // d2(psi F) = psi_ik F + psi_i F_k + psi_k F_i + psi F_ik, psi_i = -P1_i
+                Mm[i * NL + k] = ca(cs(cs(cm(Mm[i * NL + k], F), cm(L[i], Fi[k])), cm(L[k], Fi[i])), cm(psi, Fik));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NL; ++i)
+            if (i < nl) L[i] = cs(cm(L[i], F), cm(psi, Fi[i]));   // -(psi F)_i = P1_i F - psi F_i
+        psi = cm(psi, F);
+    }
 }
 
-// F = K Psi, dF_i = Psi (dK_i - K L_i), d2F_ik = Psi [d2K_ik - dK_i L_k - dK_k L_i + K (L_i L_k - M_ik)]
+// F = K psi, dF_i = K_i psi - K P1_i, d2F_ik = K_ik psi - K_i P1_k - K_k P1_i + K P2_ik
 __device__ inline Cd amp_F(double K, Cd psi) { return csc(psi, K); }
-__device__ inline Cd amp_dF(const HessArgs& a, int rec, double K, Cd psi, const Cd* L, int i) {
-    Cd t = cs(Cd{rec_g(a, rec, i), 0.0}, csc(L[i], K));
-    return cm(psi, t);
+__device__ inline Cd amp_dF(const HessArgs& a, int rec, double K, Cd psi, const Cd* P1, int i) {
+    return cs(csc(psi, rec_g(a, rec, i)), csc(P1[i], K));
 }
 template <int NL>
-__device__ inline Cd amp_d2F(const HessArgs& a, int rec, double K, Cd psi, const Cd* L, const Cd* Mm, int i, int k) {
-    Cd t{rec_h(a, rec, i, k), 0.0};
-    t = cs(t, csc(L[k], rec_g(a, rec, i)));
-    t = cs(t, csc(L[i], rec_g(a, rec, k)));
-    t = ca(t, csc(cs(cm(L[i], L[k]), Mm[i * NL + k]), K));
-    return cm(psi, t);
+__device__ inline Cd amp_d2F(const HessArgs& a, int rec, double K, Cd psi, const Cd* P1, const Cd* P2, int i, int k) {
+    Cd t = csc(psi, rec_h(a, rec, i, k));
+    t = cs(t, csc(P1[k], rec_g(a, rec, i)));
+    t = cs(t, csc(P1[i], rec_g(a, rec, k)));
+    return ca(t, csc(P2[i * NL + k], K));
 }
 
 // one event per thread.  NL: block-size class (max local variables, 1/2/4/8); NPT: the
@@ -147,7 +190,7 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
-        block_terms<NL>(a, B, e, psi, L, Mm);
+        block_terms<NL>(a, b, e, psi, L, Mm);
         for (int c = 0; c < B.nsl; ++c) {
             int am = B.site + c;
             const DAmp& A = a.amps[am];
@@ -199,7 +242,7 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
-        block_terms<NL>(a, B, e, psi, L, Mm);
+        block_terms<NL>(a, b, e, psi, L, Mm);
         double Rb[NL * NL];
 #pragma unroll
         for (int q = 0; q < NL * NL; ++q) Rb[q] = 0.0;

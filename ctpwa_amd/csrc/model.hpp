@@ -254,6 +254,8 @@ struct Model_ {
         return (int)n;
     }
     int ncf() const { return 2 + 2 * ncv(); }
+    // the Hessian's second-order rows of one factor: value, dr[], di[], hr[], hi[] (packed k <= l)
+    int ncf2() const { return 2 + 2 * ncv() + ncv() * (ncv() + 1); }
     size_t off_th() const { return off_rec() + (size_t)n_coef_rec * REC; }   // theta values (for Custom)
 };
 

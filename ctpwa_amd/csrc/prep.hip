@@ -445,7 +445,8 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble_batch(const FinArgs* _
 // generic per-event amplitudes and intensities
 // ---------------------------------------------------------------------------
 
-__device__ inline double2 block_psi(const AuxArgs& a, const DBlock& B, long long e) {
+__device__ inline double2 block_psi(const AuxArgs& a, int b, long long e) {
+    const DBlock& B = a.blocks[b];
     double pr = 1.0, pi = 0.0;
     for (int d = 0; d < B.nden; ++d) {
         const DDen& D = a.dens[B.den0 + d];
@@ -466,6 +467,13 @@ __device__ inline double2 block_psi(const AuxArgs& a, const DBlock& B, long long
         double nr = pr * xr - pi * xi, ni = pr * xi + pi * xr;
         pr = nr; pi = ni;
     }
+    // Custom lineshape factors (written for this parameter point by the Custom kernel)
+    for (int q = a.bcp[b]; q < a.bcp[b + 1]; ++q) {
+        const double* f = a.cfb + (long long)a.bcl[q] * a.ncf * a.cld + e;
+        const double fr = f[0], fi = f[a.cld];
+        const double nr = pr * fr - pi * fi, ni = pr * fi + pi * fr;
+        pr = nr; pi = ni;
+    }
     return make_double2(pr, pi);
 }
 
@@ -476,8 +484,7 @@ __global__ void k_amplitudes(AuxArgs a, long long e0, long long n, double* __res
     long long e = e0 + i;
     for (int am = 0; am < a.namp; ++am) {
         const DAmp& A = a.amps[am];
-        const DBlock& B = a.blocks[A.block];
-        double2 psi = block_psi(a, B, e);
+        double2 psi = block_psi(a, A.block, e);
         double K = a.table[a.off_rec + (long long)A.rec * REC];
         for (int p = 0; p < a.npol; ++p) {
             double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
@@ -498,8 +505,7 @@ __global__ void k_amp_rows(AuxArgs a, long long e0, long long n, double* __restr
     long long e = e0 + i;
     for (int am = 0; am < a.namp; ++am) {
         const DAmp& A = a.amps[am];
-        const DBlock& B = a.blocks[A.block];
-        double2 psi = block_psi(a, B, e);
+        double2 psi = block_psi(a, A.block, e);
         double K = a.table[a.off_rec + (long long)A.rec * REC];
         for (int p = 0; p < a.npol; ++p) {
             double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
@@ -527,7 +533,7 @@ __global__ void k_partial_int(AuxArgs a, const int* __restrict__ wptr, const int
         for (int k = wptr[w]; k < wptr[w + 1]; ++k) {
             const DBlock& B = a.blocks[wblk[k]];
             if (B.nsl <= 0) continue;
-            double2 psi = block_psi(a, B, e);
+            double2 psi = block_psi(a, wblk[k], e);
             for (int p = 0; p < a.npol; ++p) {
                 double yr = 0.0, yi = 0.0;
                 for (int c = 0; c < B.nsl; ++c) {
@@ -567,7 +573,7 @@ __global__ void k_partial_amp(AuxArgs a, const int* __restrict__ wptr, const int
             for (int k = wptr[w]; k < wptr[w + 1]; ++k) {
                 const DBlock& B = a.blocks[wblk[k]];
                 if (B.nsl <= 0) continue;
-                double2 psi = block_psi(a, B, e);
+                double2 psi = block_psi(a, wblk[k], e);
                 double yr = 0.0, yi = 0.0;
                 for (int c = 0; c < B.nsl; ++c) {
                     const int am = B.site + c;
@@ -592,7 +598,7 @@ __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __rest
     for (int p = 0; p < a.npol; ++p) { Sr[p] = 0.0; Si[p] = 0.0; }
     for (int am = 0; am < a.namp; ++am) {
         const DAmp& A = a.amps[am];
-        double2 psi = block_psi(a, a.blocks[A.block], e);
+        double2 psi = block_psi(a, A.block, e);
         double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
         double yr = cr * psi.x - ci * psi.y, yi = cr * psi.y + ci * psi.x;
         for (int p = 0; p < a.npol; ++p) {

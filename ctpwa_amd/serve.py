@@ -546,9 +546,9 @@ class FitService:
                  device: int = 0, clock: Callable[[], float] = time.perf_counter, method: str = "auto",
                  edm_tol: float = 1e-5):
         """``method``: "newton" (exact Hessians, BatchNewton), "lbfgs" (BatchLBFGS), or
-        "auto": Newton unless the model has Custom lineshapes (no Hessian)."""
+        "auto" (Newton)."""
         if method == "auto":
-            method = "lbfgs" if any(l.startswith("cexpr") for l in plan_text.splitlines()) else "newton"
+            method = "newton"
         if method not in ("newton", "lbfgs"):
             raise ValueError("method must be 'auto', 'newton' or 'lbfgs'")
         self.method = method

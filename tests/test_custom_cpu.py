@@ -114,3 +114,12 @@ def test_fused_kernel_loads_custom_factors():
     body = src[src.index("void eval_body("):src.index('extern "C" __global__')]
     assert "cd_" not in body and "cfb[" in body
     assert "cd_exp(" in src[src.index("ctpwa_custom("):]
+
+
+def test_custom_modules_compile_for_gfx950():
+    """A model with Custom lineshapes compiles two modules with hipRTC: the fused pass with
+    its first-order factor kernel, and the Hessian's second-order factor kernel."""
+    from ctpwa_amd import _native
+    plan = plan_of(_custom("exp(-p1*m) * cos(p0*m) / sqrt(p0^2 + m^2) + BWR(p0, p1, 1, 3.0)"))
+    base = plan_of(W.JPSI_3PI)
+    assert _native.compile_check(plan.to_text(), "fp64") > _native.compile_check(base.to_text(), "fp64")
