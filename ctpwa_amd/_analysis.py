@@ -64,7 +64,7 @@ class NLLFunction(torch.autograd.Function):
     def forward(ctx, params, ana):
         out = ana._nll_grad_tensor(params)
         ctx.save_for_backward(out[1:])
-        return out[0].clone()
+        return out[0]
 
     @staticmethod
     def backward(ctx, grad_out):
@@ -123,8 +123,11 @@ class analysis:
         self._samples = s
         self._weights = w
         self._truth = None
-        self._theta = np.asarray(self.plan.theta_init() if hasattr(self.plan, "theta_init") else
-                                 [t.init for t in self.plan.theta], dtype=np.float64)
+        # theta of the stored amplitudes: a host copy (initial values, reCalcAmp) or, after
+        # any evaluation, the theta the device evaluated last (read back only on demand,
+        # so evaluations never wait for the host)
+        self._theta_host = np.asarray([t.init for t in self.plan.theta], dtype=np.float64)
+        self._theta_on_host = True
         self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
                                  w["phsp"], w["data"], w["bkg"], device=device, precision=precision)
         self._valid = True
@@ -189,7 +192,8 @@ class analysis:
         t = torch.as_tensor(theta, dtype=torch.float64).detach().cpu().reshape(-1)
         if t.numel() != self.plan.n_theta:
             raise RuntimeError("theta must have %d entries" % self.plan.n_theta)
-        self._theta = t.numpy().copy()
+        self._theta_host = t.numpy().copy()
+        self._theta_on_host = True
 
     def getSLVectors(self) -> torch.Tensor:
         """Number of SL amplitudes of every partial wave, int32 on the device
@@ -399,7 +403,15 @@ class analysis:
         return A.reshape(-1)
 
     def _remember_theta(self, p: torch.Tensor):
-        self._theta = p[2 * self.plan.n_free:].detach().cpu().numpy().copy()
+        # the native call that follows writes p's theta into the handle
+        self._theta_on_host = False
+
+    @property
+    def _theta(self) -> np.ndarray:
+        if not self._theta_on_host:
+            self._theta_host = self._h.last_theta(torch.cuda.current_stream(self.device).cuda_stream)
+            self._theta_on_host = True
+        return self._theta_host
 
     def _params_from_vector(self, vector: torch.Tensor) -> torch.Tensor:
         if not isinstance(vector, torch.Tensor) or not vector.is_cuda or vector.device != self.device:

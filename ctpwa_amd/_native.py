@@ -22,7 +22,7 @@ EXPORTS = [
     "ctpwa_footprint_bytes", "ctpwa_hessian_workspace_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
     "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
-    "ctpwa_amplitude_gram", "ctpwa_partial_intensities", "ctpwa_partial_amplitudes",
+    "ctpwa_amplitude_gram", "ctpwa_partial_intensities", "ctpwa_partial_amplitudes", "ctpwa_last_theta",
 ]
 
 _lib = None
@@ -51,6 +51,7 @@ def lib():
     for f in ("ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_amplitude_gram", "ctpwa_partial_intensities",
               "ctpwa_partial_amplitudes"):
         getattr(L, f).argtypes = [_vp, _vp, ctypes.c_int, _vp, _vp]
+    L.ctpwa_last_theta.argtypes = [_vp, _dp, _vp]
     L.ctpwa_time_eval_kernel.argtypes = [_vp, _vp, ctypes.c_int, _vp, _dp]
     L.ctpwa_footprint_bytes.restype = _ll
     L.ctpwa_footprint_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll, ctypes.c_int]
@@ -210,6 +211,13 @@ class Handle:
         """[n_wave + 1][n_cls]: per-event partial-wave intensities, then the total"""
         check(lib().ctpwa_partial_intensities(self.h, ctypes.c_void_p(params_ptr), cls, ctypes.c_void_p(out_ptr),
                                               ctypes.c_void_p(stream)), "ctpwa_partial_intensities")
+
+    def last_theta(self, stream: int):
+        """theta of the most recent evaluation (host float64 [ntheta])"""
+        import numpy as np
+        out = np.zeros(max(self.n_theta, 1))
+        check(lib().ctpwa_last_theta(self.h, out.ctypes.data_as(_dp), ctypes.c_void_p(stream)), "ctpwa_last_theta")
+        return out[:self.n_theta].copy()
 
     def partial_amplitudes(self, params_ptr: int, cls: int, out_ptr: int, stream: int):
         """[n_wave][n_cls][n_pol][2]: per-event coherent partial-wave amplitudes"""

@@ -1008,6 +1008,18 @@ extern "C" int ctpwa_partial_amplitudes(ctpwa_handle* h, const double* params_de
     return 0;
 }
 
+extern "C" int ctpwa_last_theta(ctpwa_handle* h, double* theta_host, void* stream) {
+    if (!h || (!theta_host && h->M.P.ntheta > 0)) return fail("ctpwa_last_theta: NULL argument");
+    if (h->M.P.ntheta == 0) return 0;
+    DeviceGuard g(h->device);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(theta_host, h->table + h->M.off_th(), (size_t)h->M.P.ntheta * sizeof(double),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(std::string("ctpwa_last_theta: ") + hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream) {
     if (!h || !params_dev || !hess_dev) return fail("ctpwa_hessian: NULL argument");
     DeviceGuard g(h->device);

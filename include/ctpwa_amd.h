@@ -113,6 +113,13 @@ int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, dou
  * Fails for models with Custom lineshapes */
 int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
+/* theta (the last ntheta parameters) of the handle's most recent evaluation of any kind,
+ * copied to `theta_host` [ntheta] after the work queued on `stream`: the resonance
+ * parameters at which the reference's stored amplitudes stand after getNLL / getHessian
+ * / writeResult (src/AmpGen.cu:2344-2726), for the post-fit methods that use them.
+ * Undefined before the first evaluation */
+int ctpwa_last_theta(ctpwa_handle* h, double* theta_host, void* stream);
+
 /* per-event coherent partial-wave amplitudes of class `cls` (couplings applied): device
  * [n_wave][n_cls][n_pol][2], P_w summed over the wave's blocks as above.  The per-event
  * quantities of writeResult that need phases (the total over a subset of waves and the
