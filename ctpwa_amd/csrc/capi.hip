@@ -34,7 +34,7 @@ extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::Pre
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep_batch(const double* const*, double* const*, const ctpwa::PrepArgs*, int, int,
                                               hipStream_t);
-extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, int, size_t, hipStream_t);
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
 
@@ -724,7 +724,12 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             h->fin.the = h->upload(the);
             h->fin.cpp = h->upload(cpp);
             h->fin.cpe = h->upload(cpe);
-            if ((size_t)h->nacc * sizeof(double) > 60 * 1024)
+            h->fin.nthe = (int)the.size();
+            h->fin.ncpe = (int)cpe.size();
+            h->fin.nacc = h->nacc;
+            h->fin.nfree = P.nfree;
+            h->fin.ntheta = P.ntheta;
+            if (assemble_lds_bytes(h->fin) > 60 * 1024)
                 throw std::runtime_error("too many gradient accumulators for the assembly kernel");
         }
         h->fin.namp = P.namp;
@@ -1231,12 +1236,14 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
     try {
         const ctpwa_handle* h0 = hs[0];
         int max_nacc = 0;
+        size_t max_lds = 0;
         for (int j = 0; j < n; ++j) {
             const ctpwa_handle* h = hs[j];
             if (!h || !params_dev[j] || !out_dev[j]) throw std::runtime_error("NULL handle, parameter or output pointer");
             if (h->device != b->device) throw std::runtime_error("a handle lives on another device than the batch");
             if (h->mod != h0->mod) throw std::runtime_error("the handles of one batch must share one model and precision");
             max_nacc = std::max(max_nacc, h->nacc);
+            max_lds = std::max(max_lds, assemble_lds_bytes(h->fin));
         }
         // the staging buffer is reused: wait for the previous upload from it
         if (b->pending) HIPCHECK(hipEventSynchronize(b->copied));
@@ -1265,7 +1272,7 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
         const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
         void* args[] = {&d_jobs, &n};
         HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
-        HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, max_nacc, s));
+        HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, max_nacc, max_lds, s));
         return 0;
     } catch (const std::exception& e) {
         return fail(std::string("ctpwa_batch_nll_grad: ") + e.what());

@@ -187,6 +187,15 @@ __device__ __forceinline__ CD cd_bf(int L, const CD& q, const CD& q0, const CD& 
   }
   return z;
 }
+// Flatte channel phase-space factor of the event mass m (real part above threshold,
+// imaginary part below); the channel masses are taken as constants
+__device__ __forceinline__ double rho_val(double m, double ma, double mb, int im) {
+  const double s = m * m;
+  const double f1 = 1.0 - (ma + mb) * (ma + mb) / s, f2 = 1.0 - (ma - mb) * (ma - mb) / s;
+  const double g2 = f2 > 0.0 ? sqrt(f2) : 0.0;
+  return im ? ((f1 >= 0.0 ? 0.0 : sqrt(-f1)) * g2) : ((f1 >= 0.0 ? sqrt(f1) : 0.0) * g2);
+}
+__device__ __forceinline__ CD cd_rho(double m, const CD& ma, const CD& mb, int im) { return cd_c(rho_val(m, ma.r, mb.r, im), 0.0); }
 // 1 / (m0^2 - m^2 - i m0 g0)
 __device__ __forceinline__ CD cd_bw(const CD& m, const CD& m0, const CD& g0) {
   return cd_inv(cd_sub(cd_sub(cd_mul(cd_re(m0), cd_re(m0)), cd_mul(cd_re(m), cd_re(m))),
@@ -408,6 +417,13 @@ __device__ __forceinline__ CH ch_bf(int L, const CH& q, const CH& q0, const CH& 
     }
   return z;
 }
+__device__ __forceinline__ double rho_val(double m, double ma, double mb, int im) {
+  const double s = m * m;
+  const double f1 = 1.0 - (ma + mb) * (ma + mb) / s, f2 = 1.0 - (ma - mb) * (ma - mb) / s;
+  const double g2 = f2 > 0.0 ? sqrt(f2) : 0.0;
+  return im ? ((f1 >= 0.0 ? 0.0 : sqrt(-f1)) * g2) : ((f1 >= 0.0 ? sqrt(f1) : 0.0) * g2);
+}
+__device__ __forceinline__ CH ch_rho(double m, const CH& ma, const CH& mb, int im) { return ch_c(rho_val(m, ma.r, mb.r, im), 0.0); }
 __device__ __forceinline__ CH ch_bw(const CH& m, const CH& m0, const CH& g0) {
   return ch_inv(ch_sub(ch_sub(ch_mul(ch_re(m0), ch_re(m0)), ch_mul(ch_re(m), ch_re(m))),
                        ch_mul(ch_c(0.0, 1.0), ch_mul(ch_re(m0), ch_re(g0)))));
@@ -443,6 +459,16 @@ static std::string custom_code(const std::vector<std::string>& t, size_t& i, con
     if (k == "I") return X + "c(0.0, 1.0)";
     if (k == "v:m") return X + "c(xm, 0.0)";
     if (k == "v:q") return X + "c(xq, 0.0)";
+    {
+        static const char* dvar[8] = {"v:p1p", "v:p1e", "v:p1costheta", "v:p1phi",
+                                      "v:p2p", "v:p2e", "v:p2costheta", "v:p2phi"};
+        for (int j = 0; j < 8; ++j)
+            if (k == dvar[j]) return X + "c(xd" + std::to_string(j) + ", 0.0)";
+    }
+    if (k == "rr" || k == "ri") {
+        std::string a = sub(), b = sub();
+        return X + "rho(xm, " + a + ", " + b + (k == "ri" ? ", 1)" : ", 0)");
+    }
     if (k.rfind("p:", 0) == 0) {
         int pk = std::stoi(k.substr(2));
         if (pk < 0 || pk >= (int)C.param_val.size()) throw std::runtime_error("Custom parameter index out of range");
@@ -508,7 +534,10 @@ std::string generate_custom2_kernel(const Model_& M) {
         std::string code = custom_code(C.expr, ti, C, M.bvars[C.block], "ch_");
         if (ti != C.expr.size()) throw std::runtime_error("Custom expression has trailing tokens");
         const long long r0 = (long long)c * ncf2;
-        o << "    {\n      const double xm = " << inv_load(C.inv_m) << ", xq = " << inv_load(C.inv_q) << ";\n"
+        o << "    {\n      const double xm = " << inv_load(C.inv_m) << ", xq = " << inv_load(C.inv_q) << ";\n";
+        for (int j = 0; j < 8; ++j)
+            if (C.inv_d[j] >= 0) o << "      const double xd" << j << " = " << inv_load(C.inv_d[j]) << ";\n";
+        o
           << "      const CH F = " << code << ";\n"
           << "      out[" << r0 << "LL * ld + e] = F.r; out[" << r0 + 1 << "LL * ld + e] = F.i;\n"
           << "      for (int k = 0; k < NCV; ++k) { out[(" << r0 + 2 << "LL + k) * ld + e] = F.dr[k]; out[(" << r0 + 2
@@ -866,7 +895,10 @@ std::string generate_eval_kernel(const Model_& M) {
             std::string code = custom_code(C.expr, ti, C, M.bvars[C.block]);
             if (ti != C.expr.size()) throw std::runtime_error("Custom expression has trailing tokens");
             const long long r0 = (long long)c * M.ncf();
-            o << "    {\n      const double xm = " << inv_load(C.inv_m, "e") << ", xq = " << inv_load(C.inv_q, "e") << ";\n"
+            o << "    {\n      const double xm = " << inv_load(C.inv_m, "e") << ", xq = " << inv_load(C.inv_q, "e") << ";\n";
+            for (int j = 0; j < 8; ++j)
+                if (C.inv_d[j] >= 0) o << "      const double xd" << j << " = " << inv_load(C.inv_d[j], "e") << ";\n";
+            o
               << "      const CD F = " << code << ";\n"
               << "      cfb[" << r0 << "LL * ld + e] = F.r; cfb[" << r0 + 1 << "LL * ld + e] = F.i;\n"
               << "      for (int k = 0; k < NCV; ++k) { cfb[(" << r0 + 2 << "LL + k) * ld + e] = F.dr[k]; cfb[(" << r0 + 2

@@ -100,6 +100,7 @@ struct FinArgs {
     const DGEnt* the;        //   theta-gradient entries
     const int* cpp;          // [nfree + 1] CSR row pointers into the
     const DGEnt* cpe;        //   coupling-gradient entries
+    int ncpe, nthe;          // entry counts of the two lists
     int namp, nfree, ntheta, ngauss;
     const int* g_slot;
     const double* g_mu;
@@ -109,6 +110,13 @@ struct FinArgs {
     double* out;             // [1 + 2 nfree + ntheta]
     double* totals;          // [3][nacc] scratch (kept for inspection)
 };
+
+// LDS of the assembly kernel: C[nacc] | wcp[ncpe] | wth, vre, vim [nthe] (doubles), then
+// icp[ncpe] | ith[nthe] | cpp[nfree + 1] | thp[ntheta + 1] (ints)
+inline size_t assemble_lds_bytes(const FinArgs& f) {
+    return (size_t)(f.nacc + f.ncpe + 3 * f.nthe) * sizeof(double) +
+           (size_t)(f.ncpe + f.nthe + f.nfree + f.ntheta + 2) * sizeof(int);
+}
 
 // Per-event inputs of a handle: SL tensors [NU][npol][ld] and invariants [K][ld].
 // precision 0: everything FP64.  precision 1 (mixed): SL tensors FP32, invariants FP32

@@ -436,6 +436,22 @@ k_init_events(const double* __restrict__ mom, long long N, int nfinal,
         case 7: val = q * q * q / mm; break;
         case 8: val = mm; break;                                           // m (Custom)
         case 9: val = q; break;                                            // q (Custom)
+        case 12: case 13: case 14: case 15:                                // daughter momenta in the
+        case 16: case 17: case 18: case 19: {                              // event frame (Custom)
+            const int j = I.kind - 12;
+            const uint32_t set = (j < 4) ? kins[I.kin].d1set : kins[I.kin].d2set;
+            LV d = {0.0, 0.0, 0.0, 0.0};
+            for (int f = 0; f < nfinal; ++f)
+                if ((set >> f) & 1u) d = lv_add(d, load_final(m, f));
+            const double P = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            switch (j & 3) {
+            case 0: val = P; break;
+            case 1: val = d.E; break;
+            case 2: val = (P > 0.0) ? d.z / P : 0.0; break;
+            default: val = atan2(d.y, d.x);
+            }
+            break;
+        }
         case 10:
         case 11: {                                                         // table lineshapes
             const double* t = tabs + I.L;

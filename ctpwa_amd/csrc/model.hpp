@@ -38,7 +38,9 @@ namespace ctpwa {
 enum Model { M_BWR = 0, M_BW = 1, M_ONE = 2, M_FLATTE = 3, M_GS = 4, M_CUSTOM = 5, M_HIST = 6, M_INTERP = 7 };
 enum InvKind { INV_ONE = 0, INV_S = 1, INV_E = 2, INV_RHO_RE = 3, INV_RHO_IM = 4,
                INV_GS_Q2H = 5, INV_GS_Q2 = 6, INV_GS_Q3M = 7, INV_M = 8, INV_Q = 9,
-               INV_TAB_RE = 10, INV_TAB_IM = 11 };   // table lineshape value at m (L: table offset, d: model)
+               INV_TAB_RE = 10, INV_TAB_IM = 11,     // table lineshape value at m (L: table offset, d: model)
+               INV_DAUGHTER = 12 };                  // 12..19: |p|, E, cos(theta), phi of daughter 1, then 2
+                                                     // (event frame; Custom expressions)
 
 struct PNode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md1, md2; };
 struct PTopo { std::vector<PNode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
@@ -191,6 +193,7 @@ struct ThTerm { int den, coef, im, inv; };
 struct CustomSpec {
     int block, res, node;
     int inv_m, inv_q;                  // invariant rows of m and q at the node
+    int inv_d[8];                      // rows of the daughter-momentum variables (-1: unused)
     std::vector<std::string> expr;
     std::vector<int> param_slot;       // parameter k -> theta slot (-1: fixed value)
     std::vector<double> param_val;
@@ -230,7 +233,7 @@ struct Model_ {
             // squared masses, and the m and q of Custom expressions (arbitrary formulas
             // that may cancel like m0^2 - m^2), stay FP64
             if (!mixed || inv[i].kind == INV_S || inv[i].kind == INV_M || inv[i].kind == INV_Q ||
-                inv[i].kind == INV_TAB_RE || inv[i].kind == INV_TAB_IM) m[i] = nd++;
+                inv[i].kind == INV_TAB_RE || inv[i].kind == INV_TAB_IM || inv[i].kind >= INV_DAUGHTER) m[i] = nd++;
             else m[i] = -(nf++) - 1;
         }
         return m;
@@ -374,6 +377,13 @@ inline Model_ derive(const Plan& P) {
                 C.inv_m = add_inv({kin, INV_M, 0, 0.0, 0.0, 0.0});
                 C.inv_q = add_inv({kin, INV_Q, 0, 0.0, 0.0, 0.0});
                 C.expr = R.expr;
+                static const char* dvar[8] = {"v:p1p", "v:p1e", "v:p1costheta", "v:p1phi",
+                                              "v:p2p", "v:p2e", "v:p2costheta", "v:p2phi"};
+                for (int j = 0; j < 8; ++j) {
+                    C.inv_d[j] = -1;
+                    if (std::find(C.expr.begin(), C.expr.end(), std::string(dvar[j])) != C.expr.end())
+                        C.inv_d[j] = add_inv({kin, INV_DAUGHTER + j, 0, 0.0, 0.0, 0.0});
+                }
                 if (C.expr.empty()) throw std::runtime_error("Custom lineshape without expression");
                 C.param_slot = R.slots;
                 C.param_val = R.params;

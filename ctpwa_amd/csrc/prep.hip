@@ -374,11 +374,22 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __r
     if (threadIdx.x == 0) f.totals[pair] = sh[0];
 }
 
-// stage 2: NLL and gradient from the class totals.  The class-combined accumulators
-// go to LDS first; the gradients are then CSR contractions over host-built entry lists
-// (independent loads, one wave per theta with a fixed-order butterfly sum).
+// stage 2: NLL and gradient from the class totals.  Every global read happens first, in
+// parallel over the threads: the class-combined accumulators, then each gradient entry's
+// weight (scale times its table coefficient) and its accumulator indices.  The gradient
+// rows are then fixed-order sums over LDS (one wave per theta with a butterfly sum), so
+// no thread waits on a chain of dependent global loads.
+// LDS layout: assemble_lds_bytes (device.hpp)
 __device__ __forceinline__ void assemble_body(const FinArgs& f) {
     extern __shared__ double C[];            // [nacc] = -data + bkg + cp * phsp
+    double* wcp = C + f.nacc;
+    double* wth = wcp + f.ncpe;
+    double* vre = wth + f.nthe;
+    double* vim = vre + f.nthe;
+    int* icp = (int*)(vim + f.nthe);
+    int* ith = icp + f.ncpe;
+    int* cpp = ith + f.nthe;
+    int* thp = cpp + f.nfree + 1;
     int tid = threadIdx.x;
     const double* tot = f.totals;            // tot[c * nacc + col]
     double Np = f.Np, Neff = f.Wd - f.Wb;
@@ -388,6 +399,26 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
     const double* theta = f.params + 2 * f.nfree;
     for (int col = tid; col < f.nacc; col += ASM_THREADS)
         C[col] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
+    // coupling entries: weight r_a K_a, accumulator pair (1 + 2a, 2 + 2a)
+    for (int k = tid; k < f.ncpe; k += ASM_THREADS) {
+        DGEnt e = f.cpe[k];
+        wcp[k] = e.scale * f.table[f.off_rec + e.rec];
+        icp[k] = e.idx;
+    }
+    // theta entries: idx >= 0 -> w C[idx]; idx < 0 -> w (vr C[1 + 2a] + vi C[2 + 2a])
+    for (int k = tid; k < f.nthe; k += ASM_THREADS) {
+        DGEnt e = f.the[k];
+        double d = (e.rec < 0) ? 1.0 : f.table[f.off_rec + e.rec];
+        wth[k] = e.scale * d;
+        ith[k] = e.idx;
+        if (e.idx < 0) {
+            int a = -e.idx - 1;
+            vre[k] = f.table[f.off_v + 2 * a];
+            vim[k] = f.table[f.off_v + 2 * a + 1];
+        }
+    }
+    for (int k = tid; k <= f.nfree; k += ASM_THREADS) cpp[k] = f.cpp[k];
+    for (int k = tid; k <= f.ntheta; k += ASM_THREADS) thp[k] = f.thp[k];
     double loss = -tot[f.nacc] + tot[2 * f.nacc] + Neff * log(pf);
     for (int g = 0; g < f.ngauss; ++g) {
         double d = theta[f.g_slot[g]] - f.g_mu[g];
@@ -401,11 +432,10 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
     // output [2 Re, 2 Im]
     for (int fi = tid; fi < f.nfree; fi += ASM_THREADS) {
         double gr = 0.0, gi = 0.0;
-        for (int k = f.cpp[fi]; k < f.cpp[fi + 1]; ++k) {
-            DGEnt e = f.cpe[k];
-            double w = e.scale * f.table[f.off_rec + e.rec];
-            gr += w * C[1 + 2 * e.idx];
-            gi += w * C[2 + 2 * e.idx];
+        for (int k = cpp[fi]; k < cpp[fi + 1]; ++k) {
+            double w = wcp[k];
+            gr += w * C[1 + 2 * icp[k]];
+            gi += w * C[2 + 2 * icp[k]];
         }
         out[1 + fi] = reset ? 0.0 : 2.0 * gr;
         out[1 + f.nfree + fi] = reset ? 0.0 : 2.0 * gi;
@@ -415,15 +445,13 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
     const int lane = tid & 63, nw = ASM_THREADS / 64;
     for (int j = tid >> 6; j < f.ntheta; j += nw) {
         double g = 0.0;
-        for (int k = f.thp[j] + lane; k < f.thp[j + 1]; k += 64) {
-            DGEnt e = f.the[k];
-            double d = (e.rec < 0) ? 1.0 : f.table[f.off_rec + e.rec];
-            if (e.idx >= 0) {
-                g += e.scale * d * C[e.idx];
+        for (int k = thp[j] + lane; k < thp[j + 1]; k += 64) {
+            int idx = ith[k];
+            if (idx >= 0) {
+                g += wth[k] * C[idx];
             } else {
-                int a = -e.idx - 1;
-                double vr = f.table[f.off_v + 2 * a], vi = f.table[f.off_v + 2 * a + 1];
-                g += e.scale * d * (vr * C[1 + 2 * a] + vi * C[2 + 2 * a]);
+                int a = -idx - 1;
+                g += wth[k] * (vre[k] * C[1 + 2 * a] + vim[k] * C[2 + 2 * a]);
             }
         }
         for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o);
@@ -656,13 +684,13 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep_ba
 
 // fins: device array [njobs]; max_nacc: the largest accumulator count among them
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs* fins, int njobs,
-                                                                                        int max_nacc, hipStream_t stream) {
+                                                                                        int max_nacc, size_t lds_bytes,
+                                                                                        hipStream_t stream) {
     if (njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(ctpwa::k_reduce_batch, dim3(3 * max_nacc, njobs), dim3(ctpwa::RED_THREADS), 0, stream, fins);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ctpwa::k_assemble_batch, dim3(njobs), dim3(ctpwa::ASM_THREADS), (size_t)max_nacc * sizeof(double),
-                       stream, fins);
+    hipLaunchKernelGGL(ctpwa::k_assemble_batch, dim3(njobs), dim3(ctpwa::ASM_THREADS), lds_bytes, stream, fins);
     return hipGetLastError();
 }
 
@@ -670,8 +698,7 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finaliz
     hipLaunchKernelGGL(ctpwa::k_reduce, dim3(3 * f->nacc), dim3(ctpwa::RED_THREADS), 0, stream, *f);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    size_t shmem = (size_t)f->nacc * sizeof(double);
-    hipLaunchKernelGGL(ctpwa::k_assemble, dim3(1), dim3(ctpwa::ASM_THREADS), shmem, stream, *f);
+    hipLaunchKernelGGL(ctpwa::k_assemble, dim3(1), dim3(ctpwa::ASM_THREADS), ctpwa::assemble_lds_bytes(*f), stream, *f);
     return hipGetLastError();
 }
 

@@ -14,15 +14,22 @@ The expression language follows the reference's DSL (src/CustomExpr.cu:230-560):
   its q0 is the breakup at the first parameter), Bf(L, q, q0, d) (L literal),
   q0(x[, md1, md2]).
 
-The daughter-momentum variables (p1p, p1e, ...) and Flatte(...) inside an expression are
-not built and raise.  ``L``, ``d``, ``md1`` and ``md2`` are resolved to constants when the
+* the daughters' momenta in the event frame: ``p1p``, ``p1e``, ``p1costheta``, ``p1phi``
+  (|p|, E, p_z/|p|, atan2(p_y, p_x) of the first daughter) and ``p2...`` of the second;
+* ``Flatte(m0, g_1..g_n, ma_1, mb_1, .., ma_n, mb_n)``, expanded as in the reference
+  (src/CustomExpr.cu:499-511,844-870) into 1 / (m0^2 - m^2 + sum_i (g_i m0 / 2)
+  (rho_i,im - i rho_i,re)) with the channel phase-space factors of the event mass.
+
+``L``, ``d``, ``md1`` and ``md2`` are resolved to constants when the
 plan is built (the GPU evaluates one lineshape per block, so ``L`` must be the same for
 every wave at the node); ``md1``/``md2`` must be configured particle masses.
 
 Serialized form (one token per node, prefix order): ``n:<value>`` number, ``I`` the
 imaginary unit, ``v:m`` / ``v:q`` event variables, ``p:<k>`` parameter k, operators
 ``+ - * / ^ neg``, functions ``f:<name>`` (one argument), ``q0`` (3 arguments),
-``bf:<L>`` (q, q0, d), ``bw`` (m, m0, g0), ``bwr:<L>`` (m, m0, g0, q, q0, d).
+``bf:<L>`` (q, q0, d), ``bw`` (m, m0, g0), ``bwr:<L>`` (m, m0, g0, q, q0, d), ``rr`` /
+``ri`` (ma, mb: the real / imaginary phase-space factor of the event mass), event
+variables ``v:p1p`` .. ``v:p2phi``.
 """
 from __future__ import annotations
 
@@ -34,7 +41,7 @@ from .config import ConfigError
 
 _FUNCS = {"exp": "exp", "log": "log", "ln": "log", "sin": "sin", "cos": "cos", "sqrt": "sqrt",
           "csqrt": "sqrt", "abs": "abs", "Re": "re", "real": "re", "Im": "im", "imag": "im", "conj": "conj"}
-_UNBUILT_VARS = {"p1p", "p1e", "p1costheta", "p1phi", "p2p", "p2e", "p2costheta", "p2phi"}
+_DAUGHTER_VARS = ("p1p", "p1e", "p1costheta", "p1phi", "p2p", "p2e", "p2costheta", "p2phi")
 
 _TOKEN = re.compile(r"\s*(?:(1[ji])(?![A-Za-z0-9_])|(\d+\.?\d*(?:[eE][+-]?\d+)?|\.\d+(?:[eE][+-]?\d+)?)|"
                     r"([A-Za-z_][A-Za-z0-9_]*)|(\S))")
@@ -197,8 +204,8 @@ class _Parser:
             return self.md(1)
         if name == "md2":
             return self.md(2)
-        if name in _UNBUILT_VARS:
-            raise ConfigError("Custom expression variable '%s' is not built" % name)
+        if name in _DAUGHTER_VARS:
+            return ("v", name)
         if name in _FUNCS:
             a = self.args()
             if len(a) != 1:
@@ -230,7 +237,18 @@ class _Parser:
                 return ("q0", a[0], a[1], a[2])
             raise ConfigError("Custom expression: q0(x) or q0(x, md1, md2)")
         if name == "Flatte":
-            raise ConfigError("Custom expression: Flatte() inside an expression is not built")
+            a = self.args()
+            if len(a) < 4 or (len(a) - 1) % 3 != 0:
+                raise ConfigError("Custom expression: Flatte(m0, g..., (ma, mb)...) needs 1 + 3n arguments")
+            n = (len(a) - 1) // 3
+            m0, gs, ch = a[0], a[1:1 + n], a[1 + n:]
+            m = ("v", "m")
+            D = ("-", ("*", m0, m0), ("*", m, m))
+            for i in range(n):
+                gw = ("*", ("*", gs[i], m0), ("n", 0.5))
+                rho = ("-", ("ri", ch[2 * i], ch[2 * i + 1]), ("*", ("I",), ("rr", ch[2 * i], ch[2 * i + 1])))
+                D = ("+", D, ("*", gw, rho))
+            return ("/", ("n", 1.0), D)
         if name in self.ctx.params:
             return ("p", self.ctx.params[name])
         raise ConfigError("Custom expression: unknown identifier '%s'" % name)
@@ -262,6 +280,8 @@ def serialize(node) -> List[str]:
         return ["bf:%d" % node[1]] + serialize(node[2]) + serialize(node[3]) + serialize(node[4])
     if k == "bw":
         return ["bw"] + serialize(node[1]) + serialize(node[2]) + serialize(node[3])
+    if k in ("rr", "ri"):
+        return [k] + serialize(node[1]) + serialize(node[2])
     if k == "bwr":
         out = ["bwr:%d" % node[1]]
         for sub in node[2:]:

@@ -552,7 +552,8 @@ struct Sample {
     std::vector<double> w;                   // per-event weight (empty = 1)
     // per topology: slamp[slc][e][pol]
     std::vector<std::vector<cd>> slamp;
-    // per topology: per node kinematics [e][node] (mm, qq, md1, md2)
+    // per topology: per node kinematics [e][node][KS] (mm, qq, md1, md2, then |p|, E,
+    // cos(theta), phi of daughter 1 and of daughter 2 in the event frame)
     std::vector<std::vector<double>> kin;
 };
 
@@ -563,6 +564,8 @@ struct Oracle {
     // per block: local variable -> global theta slot
     std::vector<std::vector<int>> bvars;
 };
+
+static const int KS = 12;   // kinematic values per node
 
 // momenta of a topology's particles: finals, root (sum of finals in data order),
 // intermediates (d1 + d2 of their node) (src/AmpGen.cu:531-653, src/Analysis.cu:148-160)
@@ -597,10 +600,21 @@ static void event_slamp(const OPlan& P, const OTopo& T, const double* m, cd* out
         pd1[k] = ref_lv(T, m, P.nfinal, T.nodes[k].d1ref, done, cache);
         pd2[k] = ref_lv(T, m, P.nfinal, T.nodes[k].d2ref, done, cache);
         double mm = pm[k].M(), m1 = pd1[k].M(), m2 = pd2[k].M();
-        kin[4 * k + 0] = mm;
-        kin[4 * k + 1] = breakup_q(mm, m1, m2);
-        kin[4 * k + 2] = m1;
-        kin[4 * k + 3] = m2;
+        double* kk = kin + KS * k;
+        kk[0] = mm;
+        kk[1] = breakup_q(mm, m1, m2);
+        kk[2] = m1;
+        kk[3] = m2;
+        // daughter momenta for Custom expressions: |p|, E, p_z/|p| (0 at rest),
+        // atan2(p_y, p_x) (src/AmpGen.cu:1707-1712)
+        const LV* dd[2] = {&pd1[k], &pd2[k]};
+        for (int j = 0; j < 2; ++j) {
+            double P = dd[j]->P();
+            kk[4 + 4 * j] = P;
+            kk[5 + 4 * j] = dd[j]->E;
+            kk[6 + 4 * j] = (P > 0) ? dd[j]->z / P : 0.0;
+            kk[7 + 4 * j] = std::atan2(dd[j]->y, dd[j]->x);
+        }
     }
     for (size_t c = 0; c < T.slc.size(); ++c) {
         std::vector<cd> cur;
@@ -657,7 +671,7 @@ static void build_sample(Oracle& O, int cls) {
         const OTopo& T = O.P.topos[t];
         size_t nslc = T.slc.size();
         s.slamp[t].assign(nslc * (size_t)s.n * O.P.npol, 0.0);
-        s.kin[t].assign((size_t)s.n * T.nodes.size() * 4, 0.0);
+        s.kin[t].assign((size_t)s.n * T.nodes.size() * KS, 0.0);
         int npol = O.P.npol;
         int nn = (int)T.nodes.size();
 #pragma omp parallel
@@ -665,7 +679,7 @@ static void build_sample(Oracle& O, int cls) {
             std::vector<cd> buf(nslc * npol);
 #pragma omp for schedule(static)
             for (int e = 0; e < s.n; ++e) {
-                event_slamp(O.P, T, &s.mom[(size_t)e * O.P.nfinal * 4], buf.data(), &s.kin[t][(size_t)e * nn * 4]);
+                event_slamp(O.P, T, &s.mom[(size_t)e * O.P.nfinal * 4], buf.data(), &s.kin[t][(size_t)e * nn * KS]);
                 for (size_t c = 0; c < nslc; ++c)
                     for (int p = 0; p < npol; ++p) s.slamp[t][(c * s.n + e) * npol + p] = buf[c * npol + p];
             }
@@ -741,12 +755,30 @@ struct CExpr {
     const std::vector<int>& vars;
     const std::vector<double>& theta;
     double m, q;
+    const double* dv;      // |p|, E, cos(theta), phi of daughter 1, then of daughter 2
     CJ eval() {
         const std::string& k = t.at(i++);
         auto real = [&](const Jet& x) { return CJ{x, Jet(0.0)}; };
         if (k.rfind("n:", 0) == 0) return real(Jet(std::stod(k.substr(2))));
         if (k == "I") return CJ{Jet(0.0), Jet(1.0)};
         if (k == "v:m") return real(Jet(m));
+        {
+            static const char* dn[8] = {"v:p1p", "v:p1e", "v:p1costheta", "v:p1phi",
+                                        "v:p2p", "v:p2e", "v:p2costheta", "v:p2phi"};
+            for (int j = 0; j < 8; ++j) if (k == dn[j]) return real(Jet(dv[j]));
+        }
+        // Flatte channel phase-space factor of the event mass (src/CustomExpr.cu:844-870,
+        // include/ResModel.cuh:418-451): rho = sqrt(1 - (ma+mb)^2/s) sqrt(1 - (ma-mb)^2/s)
+        // above threshold (real part), its analytic continuation below (imaginary part);
+        // no parameter dependence
+        if (k == "rr" || k == "ri") {
+            CJ a = eval(), b = eval();
+            double ma = a.re.v, mb = b.re.v, s2 = m * m;
+            double f1 = 1.0 - (ma + mb) * (ma + mb) / s2, f2 = 1.0 - (ma - mb) * (ma - mb) / s2;
+            double g2 = f2 > 0.0 ? std::sqrt(f2) : 0.0;
+            double v = (k == "rr") ? ((f1 >= 0.0 ? std::sqrt(f1) : 0.0) * g2) : ((f1 >= 0.0 ? 0.0 : std::sqrt(-f1)) * g2);
+            return real(Jet(v));
+        }
         if (k == "v:q") return real(Jet(q));
         if (k.rfind("p:", 0) == 0) return real(res_param(R, std::stoi(k.substr(2)), vars, theta));
         if (k == "+") { CJ a = eval(), b = eval(); return cadd(a, b); }
@@ -851,7 +883,7 @@ static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin
     for (int k = 0; k < nn; ++k) {
         const ONode& n = T.nodes[k];
         int L = T.slc[c][k].second;
-        double mm = kin[4 * k], qq = kin[4 * k + 1], md1 = kin[4 * k + 2], md2 = kin[4 * k + 3];
+        double mm = kin[KS * k], qq = kin[KS * k + 1], md1 = kin[KS * k + 2], md2 = kin[KS * k + 3];
         // q0 daughter masses: config mass, else the mass parameter of the block resonance
         // occupying the daughter (1 when it has no parameters, src/AmpGen.cu:1504-1527),
         // else the event mass
@@ -912,7 +944,7 @@ static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin
             break;
         }
         case 5: {   // Custom expression: no vertex barrier besides what it contains
-            CExpr ce{R.expr, 0, R, vars, theta, mm, qq};
+            CExpr ce{R.expr, 0, R, vars, theta, mm, qq, kin + KS * k + 4};
             f = ce.eval();
             if (ce.i != R.expr.size()) throw std::runtime_error("custom expression: trailing tokens");
             break;
@@ -991,7 +1023,7 @@ static void eval_class(const Oracle& O, int cls, const std::vector<cd>& v, const
                 const std::vector<int>& vars = O.bvars[&B - &P.blocks[0]];
                 g_nv = (int)vars.size();
                 const OTopo& T = P.topos[B.topo];
-                const double* kin = &s.kin[B.topo][(size_t)e * T.nodes.size() * 4];
+                const double* kin = &s.kin[B.topo][(size_t)e * T.nodes.size() * KS];
                 for (int c = 0; c < B.nsl; ++c) {
                     CJ F = block_factor(P, B, c, kin, vars, theta);
                     int a = B.site + c;
@@ -1281,7 +1313,7 @@ int orc_amplitudes(void* h, const double* params, int cls, double* out) {
             const OBlock& B = P.blocks[b];
             g_nv = (int)O.bvars[b].size();
             const OTopo& T = P.topos[B.topo];
-            const double* kin = &s.kin[B.topo][(size_t)e * T.nodes.size() * 4];
+            const double* kin = &s.kin[B.topo][(size_t)e * T.nodes.size() * KS];
             for (int c = 0; c < B.nsl; ++c) {
                 CJ F = block_factor(P, B, c, kin, O.bvars[b], theta);
                 for (int p = 0; p < P.npol; ++p) {

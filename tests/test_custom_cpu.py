@@ -36,7 +36,8 @@ def test_parser(text, expected):
     assert ce.serialize(ce.parse(text, _ctx())) == expected
 
 
-@pytest.mark.parametrize("text", ["a +", "foo(1)", "p1p * 2", "Flatte(a, b, 1, 2)", "BWR(a, b, c)", "(a", "a $ b"])
+@pytest.mark.parametrize("text", ["a +", "foo(1)", "p3p * 2", "Flatte(a, b, 1)", "Flatte(a, b, c, 1, 2)",
+                                  "BWR(a, b, c)", "(a", "a $ b"])
 def test_parser_rejects(text):
     with pytest.raises(ConfigError):
         ce.parse(text, _ctx())
@@ -68,6 +69,55 @@ def test_custom_bwr_equals_builtin(oracle_lib):
     assert abs(nc - nb) <= 1e-12 * abs(nb)
     assert rel(gc, gb) <= 1e-10
     assert rel(Hc, Hb) <= 1e-9
+
+
+F0_BWR = "  f0_980:  {J: 0, P: 1,  model: BWR, parameters: [0.99, 0.05],   free: [0, 1]}"
+
+
+def test_custom_flatte_equals_builtin(oracle_lib):
+    """Flatte(m0, g1, g2, ma1, mb1, ma2, mb2) in an expression is the built-in two-channel
+    Flatte (f0 -> pi pi has L = 0, so the built-in model's vertex barrier is 1): same NLL,
+    gradient and Hessian, the couplings free."""
+    chans = "[[0.13957, 0.13957], [0.493677, 0.493677]]"
+    built = W.JPSI_3PI.replace(F0_BWR, "  f0_980:  {J: 0, P: 1, model: Flatte, parameters: [0.99, 0.2, 0.4], "
+                                       "free: [0, 1, 2], channels: %s}" % chans)
+    cust = W.JPSI_3PI.replace(F0_BWR, '  f0_980:  {J: 0, P: 1, model: Custom, paramValues: [0.99, 0.2, 0.4], '
+                                      'paramNames: [m0, g1, g2], free: [0, 1, 2], '
+                                      'expression: "Flatte(m0, g1, g2, 0.13957, 0.13957, 0.493677, 0.493677)"}')
+    (pc, oc), (pb, ob) = _oracles(oracle_lib, [cust, built])
+    p = perturbed(W.nominal_params(pb, 1), pb, 6)
+    nc, gc, Hc = oc.hessian(p)
+    nb, gb, Hb = ob.hessian(p)
+    assert abs(nc - nb) <= 1e-12 * abs(nb)
+    assert rel(gc, gb) <= 1e-10
+    assert rel(Hc, Hb) <= 1e-9
+
+
+def test_custom_daughter_momentum_variables(oracle_lib):
+    """p1p, p1e, p1costheta, p1phi and p2... are the daughters' |p|, E, p_z/|p| and
+    atan2(p_y, p_x) in the event frame: a BW times a factor built from them equals the
+    built-in BW amplitude times that factor evaluated in numpy on the events."""
+    f = "(1 + 0.3*p1costheta - 0.2*p2p + 0.1*p1e*p2e + 0.05*sin(p1phi - p2phi) + 0.02*p2costheta)"
+    cust = W.JPSI_RHO.replace("model: BWR, parameters: [0.775, 0.149], free: [0, 1]",
+                              'model: Custom, paramValues: [0.775, 0.149], free: [0, 1], '
+                              'expression: "p0^2/(p0^2 - m^2 - i*p0*p1) * %s"' % f)
+    bw = W.JPSI_RHO.replace("model: BWR", "model: BW")
+    pc, pb = plan_of(cust), plan_of(bw)
+    ph, da, _ = samples(2000, 500, 0, 17)
+    Oc = oracle_lib.Oracle(pc.to_text(), 3, ph, da)
+    Ob = oracle_lib.Oracle(pb.to_text(), 3, ph, da)
+    p = perturbed(W.nominal_params(pb, None), pb, 2)
+    Ac, Ab = Oc.amplitudes(p, 1), Ob.amplitudes(p, 1)
+    # rho -> pi+ pi-: daughter 1 = pip (column 0), daughter 2 = pim (column 1)
+    def var(ev):
+        P = np.linalg.norm(ev[:, 1:], axis=1)
+        return P, ev[:, 0], ev[:, 3] / P, np.arctan2(ev[:, 2], ev[:, 1])
+    p1p, p1e, p1c, p1f = var(da[:, 0])
+    p2p, p2e, p2c, p2f = var(da[:, 1])
+    fac = 1 + 0.3 * p1c - 0.2 * p2p + 0.1 * p1e * p2e + 0.05 * np.sin(p1f - p2f) + 0.02 * p2c
+    m0 = p[2 * pb.n_free]
+    ref = m0 ** 2 * Ab * fac[:, None, None]
+    assert np.abs(Ac - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
 def test_custom_bw_is_scaled_builtin(oracle_lib):

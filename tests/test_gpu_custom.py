@@ -17,6 +17,8 @@ EXPRS = [
     "p0^2/(p0^2 - m^2 - i*p0*p1)",
     ("exp(-p1*m) * (cos(p0*m) + i*sin(p1)) / sqrt(p0^2 + m^2) * abs(p0 - i*p1) "
      "+ log(p0) * conj(m + i*p1)^2 - Im(p0 * (q + 1j)) + Re(csqrt(p1 - q0))"),
+    # Flatte() expanded in the expression and the daughters' momentum variables
+    "Flatte(p0, p1, 0.13957, 0.13957) * (1 + 0.3*p1costheta - 0.2*p2p + 0.1*sin(p1phi - p2phi) + 0.05*p1e)",
 ]
 
 
