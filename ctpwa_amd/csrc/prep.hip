@@ -218,10 +218,14 @@ __device__ __forceinline__ void prep_body(const double* __restrict__ params, dou
         a.dens = (const DDen*)(base + a.o_dens);
         a.amps = (const DAmp*)(base + a.o_amps);
     }
-    int item = blockIdx.x * blockDim.x + threadIdx.x;
+    // one item (denominator or amplitude record) per workgroup: the items' chains of
+    // dependent jet arithmetic then run on different compute units at the same time instead
+    // of one after another in a single divergent wave
+    int item = blockIdx.x;
     const double* theta = params + 2 * a.nfree;
     if (blockIdx.x == 0)
         for (int j = threadIdx.x; j < a.ntheta; j += blockDim.x) table[a.off_th + j] = theta[j];
+    if (threadIdx.x != 0) return;
     if (item < a.nden) {
         const DDen& D = a.dens[item];
         const DBlock& B = a.blocks[D.block];
@@ -645,9 +649,7 @@ __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __rest
 template <int NV, bool H>
 static hipError_t launch_prep_t(const double* params, double* table, const ctpwa::PrepArgs* a, hipStream_t stream) {
     int items = a->nden + a->namp;
-    int bs = 64;
-    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3((items + bs - 1) / bs), dim3(bs), (size_t)a->meta_bytes, stream,
-                       params, table, *a);
+    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3(items), dim3(64), (size_t)a->meta_bytes, stream, params, table, *a);
     return hipGetLastError();
 }
 
@@ -665,9 +667,8 @@ template <int NV, bool H>
 static hipError_t launch_prep_batch_t(const double* const* params, double* const* tables, const ctpwa::PrepArgs* a,
                                       int njobs, hipStream_t stream) {
     int items = a->nden + a->namp;
-    int bs = 64;
-    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3((items + bs - 1) / bs, njobs), dim3(bs), (size_t)a->meta_bytes,
-                       stream, params, tables, *a);
+    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3(items, njobs), dim3(64), (size_t)a->meta_bytes, stream, params,
+                       tables, *a);
     return hipGetLastError();
 }
 
