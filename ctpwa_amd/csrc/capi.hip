@@ -42,8 +42,8 @@ extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, 
 extern "C" hipError_t ctpwa_launch_zero(double*, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, long long, const double*, long long,
                                           const int2*, int, int, double*, int, hipStream_t);
-extern "C" hipError_t ctpwa_launch_wcolsum(const double*, int, long long, long long, const double*, double*, int,
-                                            hipStream_t);
+extern "C" long long ctpwa_hess_rows_waves(long long);
+extern "C" hipError_t ctpwa_launch_accum_cols(const double*, long long, long long, int, double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
 extern "C" int ctpwa_mfma_gram_ok(int);
 extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, hipStream_t);
@@ -271,9 +271,9 @@ struct ctpwa_handle {
     std::string src;
     // Hessian workspace (allocated on the first ctpwa_hessian call)
     bool hess_ready = false;
-    long long hchunk = 0;
+    long long hchunk = 0, hcs_ld = 0;
     int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0, hnwg = 0;
-    double *hJ = nullptr, *hU = nullptr, *hZ = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
+    double *hJ = nullptr, *hU = nullptr, *hcs = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
     double* hmpart = nullptr;              // per-wave tile partials of the matrix-core Gram
     // models with Custom lineshapes: second-order factor rows for the Hessian, from a
     // module compiled on the first Hessian call
@@ -886,7 +886,8 @@ static long long hess_workspace_bytes(const Model_& M, long long maxn, long long
     long long chunk = std::min<long long>(maxn, HESS_CHUNK);
     long long nG = (long long)(2 * d.C) * (2 * d.C + 1) / 2, nUu = (long long)d.NU * (d.NU + 1) / 2;
     long long maxcol = std::max<long long>(2048, std::max(d.NU, d.NZ));
-    long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + d.NZ + 2) + HESS_NWG * maxcol + 3 * (nG + nUu + d.NZ + d.NU) +
+    long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + 2) + (long long)(d.NZ + d.NU) * ctpwa_hess_rows_waves(chunk) +
+                  HESS_NWG * maxcol + 3 * (nG + nUu + d.NZ + d.NU) +
                   (long long)d.NU * d.NU;
     w += hess_mfma_part_doubles(d);
     w += (long long)M.customs.size() * M.ncf2() * ntot;   // second-order Custom factor rows
@@ -917,7 +918,8 @@ static void hess_setup(ctpwa_handle* h) {
     h->hnwg = HESS_NWG;
     h->hJ = h->alloc<double>((size_t)h->hchunk * P.npol * 2 * h->hC);
     h->hU = h->alloc<double>((size_t)h->hchunk * h->hNU);
-    h->hZ = h->alloc<double>((size_t)h->hchunk * h->hNZ);
+    h->hcs_ld = ctpwa_hess_rows_waves(h->hchunk);
+    h->hcs = h->alloc<double>((size_t)(h->hNZ + h->hNU) * h->hcs_ld);
     h->hW = h->alloc<double>((size_t)h->hchunk * 2);
     size_t maxcol = std::max<size_t>(2048, std::max<size_t>(h->hNU, h->hNZ));
     h->hpart = h->alloc<double>((size_t)h->hnwg * maxcol);
@@ -1046,7 +1048,7 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         a.off_v = (long long)M.off_v(); a.off_rec = (long long)M.off_rec();
         a.amps = h->aux.amps; a.blocks = h->aux.blocks; a.dens = h->aux.dens; a.dterms = h->aux.dterms; a.terms = h->aux.terms;
         a.namp = P.namp; a.nblock = (int)P.blocks.size(); a.npol = P.npol; a.ntheta = P.ntheta;
-        a.J = h->hJ; a.U = h->hU; a.Z = h->hZ; a.W = h->hW; a.ld = h->hchunk;
+        a.J = h->hJ; a.U = h->hU; a.cs = h->hcs; a.cs_ld = h->hcs_ld; a.W = h->hW; a.ld = h->hchunk;
         a.C = h->hC; a.NU = h->hNU; a.NZ = h->hNZ;
         a.cfb2 = h->cfb2; a.cld = h->ntot; a.ncf2 = M.customs.empty() ? 0 : M.ncf2(); a.ncv = M.ncv();
         a.bcp = h->hbcp; a.bcl = h->hbcl;
@@ -1085,10 +1087,8 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
                         HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.Uu[c] + e0, s));
                     }
                 }
-                HIPCHECK(ctpwa_launch_wcolsum(h->hZ, h->hNZ, h->hchunk, cnt, h->hW, h->hpart, h->hnwg, s));
-                HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, h->hNZ, (double*)as.Zs[c], s));
-                HIPCHECK(ctpwa_launch_wcolsum(h->hU, h->hNU, h->hchunk, cnt, nullptr, h->hpart, h->hnwg, s));
-                HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, h->hNU, (double*)as.gs[c], s));
+                // Zs[c] and gs[c] are adjacent: one pass adds both from the rows kernel's sums
+                HIPCHECK(ctpwa_launch_accum_cols(h->hcs, h->hcs_ld, cnt, h->hNZ + h->hNU, (double*)as.Zs[c], s));
             }
         }
         as.klG = h->hklG; as.nG = h->hnG; as.klU = h->hklU; as.nUu = h->hnUu;

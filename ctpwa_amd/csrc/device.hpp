@@ -184,7 +184,8 @@ struct HessArgs {
     // contiguous run): row R = p * n_ev + r (polarisation p of chunk event r) for J
     double* J;               // [2C][ld * npol]
     double* U;               // [NU = 2 namp + ntheta][ld]
-    double* Z;               // [NZ][ld]
+    double* cs;              // per-wave column sums [NZ + NU][cs_ld]: the alpha-weighted
+    long long cs_ld;         // second-part scalars, then the u columns unweighted
     double* W;               // [2][ld] alpha, beta
     long long ld;            // row capacity of a chunk
     int C, NU, NZ;

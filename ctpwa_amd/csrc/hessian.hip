@@ -163,14 +163,55 @@ __device__ inline Cd amp_d2F(const HessArgs& a, int rec, double K, Cd psi, const
     return ca(t, csc(P2[i * NL + k], K));
 }
 
+// column sums over a wave's 64 events, taken inside k_hess_rows.  Every lane hands in the
+// same sequence of columns; the wave stages SR_K values per lane in its LDS slab (rows
+// padded to SR_K + 1 doubles against bank conflicts) and then lane 4k + q adds column k of
+// lanes 16q .. 16q + 15 in lane order, and the four quarters combine by two exchanges: a
+// fixed order.  The wave's sum of column c is stored at out[c * ld].
+constexpr int HR_THREADS = 128;
+constexpr int SR_K = 16;
+struct WaveColSums {
+    double (*slab)[SR_K + 1];
+    double* out;
+    long long ld;
+    int lane, n, col;
+    __device__ void put(double v) {
+        slab[lane][n] = v;
+        if (++n == SR_K) flush();
+    }
+    __device__ void flush() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int k = lane >> 2, q = lane & 3;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) s += slab[16 * q + t][k];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (q == 0 && k < n) out[(long long)(col + k) * ld] = s;
+        col += n;
+        n = 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
 // one event per thread.  NL: block-size class (max local variables, 1/2/4/8); NPT: the
 // number of polarisations when it is 1..4 (S in registers), 0 for any other (S in
-// private memory, runtime bounds)
+// private memory, runtime bounds).  Lanes past the chunk's end repeat its last event,
+// store nothing and hand zeros to the column sums, so every lane of a wave reaches them.
 template <int NL, int NPT>
-__global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, long long n) {
-    long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    long long e = e0 + r;
+__global__ void __launch_bounds__(HR_THREADS) k_hess_rows(HessArgs a, long long e0, long long n) {
+    __shared__ double slab[HR_THREADS / 64][64][SR_K + 1];
+    const long long r0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = r0 < n;
+    const long long r = live ? r0 : n - 1;
+    const long long e = e0 + r;
+    const int wv = threadIdx.x >> 6;
+    WaveColSums cs{slab[wv], a.cs + (long long)blockIdx.x * (HR_THREADS / 64) + wv, a.cs_ld, (int)(threadIdx.x & 63), 0,
+                   0};
     constexpr int SP = NPT > 0 ? NPT : MAXPOL;
     const int NP = NPT > 0 ? NPT : a.npol;
     const int C = a.C, na = a.namp;
@@ -180,17 +221,31 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     Cd S[SP];
 #pragma unroll
     for (int p = 0; p < SP; ++p) S[p] = {0.0, 0.0};
-    for (int p = 0; p < NP; ++p)
-        for (int c = na; c < C; ++c) {
-            Jc[c * ldJ + p * n] = 0.0;
-            Jc[(C + c) * ldJ + p * n] = 0.0;
-        }
+    if (live)
+        for (int p = 0; p < NP; ++p)
+            for (int c = na; c < C; ++c) {
+                Jc[c * ldJ + p * n] = 0.0;
+                Jc[(C + c) * ldJ + p * n] = 0.0;
+            }
     Cd psi, L[NL], Mm[NL * NL];
-    // pass 1: A columns, S, dS/dtheta columns
+    // pass 1: A columns, S, dS/dtheta columns.  For small blocks (NL * SP <= 4) a block's
+    // dS/dtheta entries are loaded once, added over its amplitudes in order and stored once
+    // (the same additions as adding into memory amplitude by amplitude)
+    constexpr bool JREG = NL * SP <= 4;
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
         block_terms<NL>(a, b, e, psi, L, Mm);
+        Cd dJ[JREG ? NL * SP : 1];
+        if constexpr (JREG) {
+#pragma unroll
+            for (int i = 0; i < NL; ++i)
+#pragma unroll
+                for (int p = 0; p < SP; ++p)
+                    dJ[i * SP + p] = (live && i < B.nloc && p < NP)
+                                         ? Cd{Jc[(na + B.vars[i]) * ldJ + p * n], Jc[(C + na + B.vars[i]) * ldJ + p * n]}
+                                         : Cd{0.0, 0.0};
+        }
         for (int c = 0; c < B.nsl; ++c) {
             int am = B.site + c;
             const DAmp& A = a.amps[am];
@@ -206,18 +261,35 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
                 double2 s = ev_sl(a.st, (long long)A.u * NP + p, e);
                 Cd sp{s.x, s.y};
                 Cd Aap = cm(F, sp);
-                Jc[am * ldJ + p * n] = Aap.r;
-                Jc[(C + am) * ldJ + p * n] = Aap.i;
                 S[p] = ca(S[p], cm(v, Aap));
+                if (live) {
+                    Jc[am * ldJ + p * n] = Aap.r;
+                    Jc[(C + am) * ldJ + p * n] = Aap.i;
+                }
 #pragma unroll
                 for (int i = 0; i < NL; ++i) {
                     if (i >= B.nloc) break;
-                    int j = B.vars[i];
                     Cd d = cm(dF[i], sp);
-                    Jc[(na + j) * ldJ + p * n] += d.r;
-                    Jc[(C + na + j) * ldJ + p * n] += d.i;
+                    if constexpr (JREG) {
+                        dJ[i * SP + p] = ca(dJ[i * SP + p], d);
+                    } else if (live) {
+                        int j = B.vars[i];
+                        Jc[(na + j) * ldJ + p * n] += d.r;
+                        Jc[(C + na + j) * ldJ + p * n] += d.i;
+                    }
                 }
             }
+        }
+        if constexpr (JREG) {
+            if (live)
+#pragma unroll
+                for (int i = 0; i < NL; ++i)
+#pragma unroll
+                    for (int p = 0; p < SP; ++p)
+                        if (i < B.nloc && p < NP) {
+                            Jc[(na + B.vars[i]) * ldJ + p * n] = dJ[i * SP + p].r;
+                            Jc[(C + na + B.vars[i]) * ldJ + p * n] = dJ[i * SP + p].i;
+                        }
         }
     }
     double I = 0.0;
@@ -231,14 +303,16 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
     if (a.cls == 0) { alpha = 1.0; beta = 0.0; }
     else if (I == 0.0) { alpha = 0.0; beta = 0.0; }
     else { alpha = w / I; beta = -w / (I * I); }
-    a.W[r] = alpha;
-    a.W[a.ld + r] = beta;
-    // pass 2: u = dI/dx and the second-part scalars (column k of this event: X[k * ld])
+    if (live) {
+        a.W[r] = alpha;
+        a.W[a.ld + r] = beta;
+    }
+    // pass 2: u = dI/dx (column k of this event: Ur[k * ld]) and the second-part scalars,
+    // which only enter through their alpha-weighted sums over the events
     double* Ur = a.U + r;
-    double* Zr = a.Z + r;
     const long long ld = a.ld;
-    for (int j = 0; j < a.ntheta; ++j) Ur[(2 * na + j) * ld] = 0.0;
-    int zq = 0;
+    if (live)
+        for (int j = 0; j < a.ntheta; ++j) Ur[(2 * na + j) * ld] = 0.0;
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
@@ -246,6 +320,11 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
         double Rb[NL * NL];
 #pragma unroll
         for (int q = 0; q < NL * NL; ++q) Rb[q] = 0.0;
+        // this block's dI/dtheta columns: loaded once, added over its amplitudes in order,
+        // stored once (the same additions as adding into memory amplitude by amplitude)
+        double ut[NL];
+#pragma unroll
+        for (int i = 0; i < NL; ++i) ut[i] = (live && i < B.nloc) ? Ur[(2 * na + B.vars[i]) * ld] : 0.0;
         for (int c = 0; c < B.nsl; ++c) {
             int am = B.site + c;
             const DAmp& A = a.amps[am];
@@ -262,17 +341,19 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             }
             Cd cT = ccj(T);
             Cd g = cm(ccj(F), T);                     // dI/dv_a* = <A_a, S>
-            Ur[am * ld] = 2.0 * g.r;
-            Ur[(na + am) * ld] = 2.0 * g.i;
+            if (live) {
+                Ur[am * ld] = 2.0 * g.r;
+                Ur[(na + am) * ld] = 2.0 * g.i;
+            }
 #pragma unroll
             for (int i = 0; i < NL; ++i) {
                 if (i >= B.nloc) break;
                 Cd dF = amp_dF(a, A.rec, K, psi, L, i);
                 Cd q = cm(dF, cT);                    // <S, dA_a/dtheta_i>
-                Zr[(zq++) * ld] = q.r;
-                Zr[(zq++) * ld] = q.i;
+                cs.put(live ? alpha * q.r : 0.0);
+                cs.put(live ? alpha * q.i : 0.0);
                 Cd vq = cm(v, q);                     // contributes to dI/dtheta
-                Ur[(2 * na + B.vars[i]) * ld] += 2.0 * vq.r;
+                ut[i] += 2.0 * vq.r;
 #pragma unroll
                 for (int k = 0; k < NL; ++k) {
                     if (k < i) continue;
@@ -289,11 +370,25 @@ __global__ void __launch_bounds__(128) k_hess_rows(HessArgs a, long long e0, lon
             for (int k = 0; k < NL; ++k) {
                 if (k < i) continue;
                 if (k >= B.nloc) break;
-                Zr[(zq++) * ld] = Rb[i * NL + k];
+                cs.put(live ? alpha * Rb[i * NL + k] : 0.0);
             }
         }
+        if (live)
+#pragma unroll
+            for (int i = 0; i < NL; ++i)
+                if (i < B.nloc) Ur[(2 * na + B.vars[i]) * ld] = ut[i];
     }
-    Zr[(zq++) * ld] = I;
+    cs.put(live ? alpha * I : 0.0);
+    // then the columns of u, unweighted, read back eight at a time (one load latency each)
+    for (int k0 = 0; k0 < a.NU; k0 += 8) {
+        double u[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) u[j] = (live && k0 + j < a.NU) ? Ur[(k0 + j) * ld] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k0 + j < a.NU) cs.put(u[j]);
+    }
+    if (cs.n > 0) cs.flush();
 }
 
 // weighted Gram of a column-major matrix X[col][row] (column stride ldx, rows
@@ -493,43 +588,26 @@ __global__ void __launch_bounds__(256) k_accum_mfma(const double* __restrict__ p
     }
 }
 
-// weighted column sums of a column-major X[col][row]: part[wg][c] = sum_rows w(row) X[c][row]
-// (w == null: weight 1).  The threads of a workgroup stride over its rows (coalesced),
-// eight columns per pass; per-thread sums are combined by wave shuffles and then the
-// waves' sums in wave order: a fixed order
-__global__ void __launch_bounds__(GR_THREADS)
-k_wcolsum(const double* __restrict__ X, int ncol, long long ldx, long long nrows, const double* __restrict__ Wt,
-          double* __restrict__ part) {
-    constexpr int CB = 8, NWV = GR_THREADS / 64;
-    __shared__ double sh[NWV][CB];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int nwg = gridDim.x, wg = blockIdx.x;
-    long long per = (nrows + nwg - 1) / nwg;
-    long long r0 = (long long)wg * per, r1 = r0 + per < nrows ? r0 + per : nrows;
-    for (int c0 = 0; c0 < ncol; c0 += CB) {
-        double s[CB];
-#pragma unroll
-        for (int k = 0; k < CB; ++k) s[k] = 0.0;
-        for (long long r = r0 + threadIdx.x; r < r1; r += GR_THREADS) {
-            const double w = Wt ? Wt[r] : 1.0;
-#pragma unroll
-            for (int k = 0; k < CB; ++k)
-                if (c0 + k < ncol) s[k] += w * X[(long long)(c0 + k) * ldx + r];
-        }
-#pragma unroll
-        for (int k = 0; k < CB; ++k)
-            for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
-        if (lane == 0)
-#pragma unroll
-            for (int k = 0; k < CB; ++k) sh[wave][k] = s[k];
-        __syncthreads();
-        if (threadIdx.x < CB && c0 + threadIdx.x < ncol) {
-            double v = sh[0][threadIdx.x];
-            for (int w = 1; w < NWV; ++w) v += sh[w][threadIdx.x];
-            part[(long long)wg * ncol + c0 + threadIdx.x] = v;
-        }
-        __syncthreads();
+// tot[c] += the sum over the nw wave sums of column c (cs[c * ld + w]) left by
+// k_hess_rows: one workgroup per column; each thread adds its strided share in two
+// interleaved sums, the waves combine by exchanges and thread 0 adds the four waves in
+// order: a fixed order
+__global__ void __launch_bounds__(256) k_accum_cols(const double* __restrict__ cs, long long ld, int nw,
+                                                    double* __restrict__ tot) {
+    __shared__ double sh[4];
+    const double* p = cs + (long long)blockIdx.x * ld;
+    double s0 = 0.0, s1 = 0.0;
+    int w = threadIdx.x;
+    for (; w + 256 < nw; w += 512) {
+        s0 += p[w];
+        s1 += p[w + 256];
     }
+    if (w < nw) s0 += p[w];
+    double s = s0 + s1;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) tot[blockIdx.x] += (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // tot[c] += sum_wg part[wg][c]  (fixed order)
@@ -679,6 +757,8 @@ __global__ void __launch_bounds__(256) k_hess_assemble(HessAsm h) {
 
 using namespace ctpwa;
 
+extern "C" __attribute__((visibility("hidden"))) long long ctpwa_hess_rows_waves(long long n);
+
 template <int NL>
 static void launch_rows_np(const HessArgs* a, long long e0, long long n, dim3 g, dim3 b, hipStream_t s) {
     switch (a->npol) {
@@ -693,8 +773,8 @@ static void launch_rows_np(const HessArgs* a, long long e0, long long n, dim3 g,
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_hess_rows(const HessArgs* a, long long e0,
                                                                                    long long n, int maxloc, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    int bs = 128;
-    dim3 g((unsigned)((n + bs - 1) / bs)), b(bs);
+    if (ctpwa_hess_rows_waves(n) > a->cs_ld) return hipErrorInvalidValue;
+    dim3 g((unsigned)((n + HR_THREADS - 1) / HR_THREADS)), b(HR_THREADS);
     if (maxloc <= 1) launch_rows_np<1>(a, e0, n, g, b, s);
     else if (maxloc <= 2) launch_rows_np<2>(a, e0, n, g, b, s);
     else if (maxloc <= 4) launch_rows_np<4>(a, e0, n, g, b, s);
@@ -750,10 +830,19 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wgram_m
     return hipGetLastError();
 }
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_wcolsum(const double* X, int ncol, long long ldx,
-                                                                                 long long nrows, const double* Wt,
-                                                                                 double* part, int nwg, hipStream_t s) {
-    hipLaunchKernelGGL(k_wcolsum, dim3(nwg), dim3(GR_THREADS), 0, s, X, ncol, ldx, nrows, Wt, part);
+// waves of k_hess_rows over n events: the row count of its column-sum partials
+extern "C" __attribute__((visibility("hidden"))) long long ctpwa_hess_rows_waves(long long n) {
+    return (n + HR_THREADS - 1) / HR_THREADS * (HR_THREADS / 64);
+}
+
+// tot[c] += the column sums left by k_hess_rows over n events, for the ncol columns
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_accum_cols(const double* cs, long long ld,
+                                                                                    long long n, int ncol, double* tot,
+                                                                                    hipStream_t s) {
+    const long long nw = ctpwa_hess_rows_waves(n);
+    if (ncol <= 0 || nw <= 0) return hipSuccess;
+    if (nw > ld) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_accum_cols, dim3(ncol), dim3(256), 0, s, cs, ld, (int)nw, tot);
     return hipGetLastError();
 }
 
