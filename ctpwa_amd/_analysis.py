@@ -198,7 +198,11 @@ class analysis:
     def getSLVectors(self) -> torch.Tensor:
         """Number of SL amplitudes of every partial wave, int32 on the device
         (src/Analysis.cu:1346-1351, src/Info.cu:288)."""
-        return torch.tensor([self.plan.blocks[b].nsl for b in self.plan.waves], dtype=torch.int32,
+        n = [0] * len(self.plan.waves)
+        for w, blk in zip(self.plan.wave_of(), self.plan.blocks):
+            if blk.sigma == 0:
+                n[w] += blk.nsl
+        return torch.tensor(n, dtype=torch.int32,
                             device=self.device)
 
     def getConstraintsIndex(self):
@@ -395,7 +399,7 @@ class analysis:
             for blk in plan.blocks:
                 if blk.sigma == 0:
                     continue
-                orig = plan.blocks[blk.wave]
+                orig = plan.blocks[blk.src]
                 for c in range(blk.nsl):
                     a, a0 = blk.site + c, orig.site + c
                     out[:, :, ref.index(a0)] += (plan.amp_ratio[a] / plan.amp_ratio[a0]) * A[:, :, a]

@@ -54,7 +54,9 @@ struct PRes {
     // (equal spacing) or [-(method+1), N, x..., re..., im...]
     std::vector<double> tab;
 };
-struct PBlock { int topo, site, nsl; std::vector<PRes> res; };
+// c0: the first of the topology's SL combinations that the block's waves use (a block
+// split into runs of one L for a Custom expression that takes each wave's L)
+struct PBlock { int topo, site, nsl, c0 = 0; std::vector<PRes> res; };
 
 struct Plan {
     int nfinal = 0, npol = 0;
@@ -128,6 +130,10 @@ inline Plan parse_plan(const std::string& text) {
         else if (key == "namp") ls >> P.namp >> P.nfree;
         else if (key == "amap") { P.amap.resize(P.namp); for (auto& x : P.amap) ls >> x; }
         else if (key == "aratio") { P.aratio.resize(P.namp); for (auto& x : P.aratio) ls >> x; }
+        else if (key == "bslc0") {
+            for (auto& B : P.blocks)
+                if (!(ls >> B.c0)) throw std::runtime_error("plan: bslc0 is truncated");
+        }
         else if (key == "bwave") {
             P.bwave.resize(P.blocks.size());
             for (auto& x : P.bwave)
@@ -142,6 +148,11 @@ inline Plan parse_plan(const std::string& text) {
     if (P.nfinal <= 0 || P.nfinal > 16) throw std::runtime_error("plan: nfinal must be in [1, 16]");
     if (P.npol <= 0) throw std::runtime_error("plan: npol must be positive");
     if ((int)P.th_init.size() != P.ntheta) throw std::runtime_error("plan: theta count mismatch");
+    for (const PBlock& B : P.blocks) {
+        if (B.topo < 0 || B.topo >= (int)P.topos.size()) throw std::runtime_error("plan: block topology out of range");
+        if (B.nsl < 0 || B.c0 < 0 || (size_t)B.c0 + B.nsl > P.topos[B.topo].slc.size())
+            throw std::runtime_error("plan: a block's SL combinations exceed its topology's");
+    }
     if (P.bwave.empty())
         for (size_t b = 0; b < P.blocks.size(); ++b) P.bwave.push_back((int)b);
     if (P.bwave.size() != P.blocks.size()) throw std::runtime_error("plan: bwave count mismatch");
@@ -477,17 +488,18 @@ inline Model_ derive(const Plan& P) {
         }
         // amplitudes
         for (int c = 0; c < B.nsl; ++c) {
-            auto key = std::make_tuple(B.topo, c, mask);
+            const int sc = B.c0 + c;
+            auto key = std::make_tuple(B.topo, sc, mask);
             int u;
             auto it = u_idx.find(key);
             if (it == u_idx.end()) {
                 u = (int)M.us.size();
                 u_idx[key] = u;
-                M.us.push_back(USpec{B.topo, c, mask});
+                M.us.push_back(USpec{B.topo, sc, mask});
             } else u = it->second;
             int a = B.site + c;
             if (a < 0 || a >= P.namp) throw std::runtime_error("amplitude index out of range");
-            M.amps[a] = AmpSpec{(int)b, c, u, 0};
+            M.amps[a] = AmpSpec{(int)b, sc, u, 0};
         }
     }
     for (size_t d = 0; d < M.dens.size(); ++d) {

@@ -71,6 +71,12 @@ def tokenize(s: str) -> List[Tuple[str, object]]:
     return out
 
 
+
+class PerWaveL(ConfigError):
+    """The expression uses L and the waves at its node differ in L: the plan builder then
+    parses it once per run of waves with one L (src/CustomExpr.cu:413, the reference's
+    run-time per-wave L)."""
+
 class Context:
     """What the expression may refer to at one node of one block."""
 
@@ -195,8 +201,7 @@ class _Parser:
             return self.q0_default()
         if name == "L":
             if len(self.ctx.L_values) != 1:
-                raise ConfigError("Custom expression uses L but the waves at its node have different L "
-                                  "(a per-wave lineshape is not built)")
+                raise PerWaveL("Custom expression uses L and the waves at its node have different L")
             return ("n", float(self.ctx.L_values[0]))
         if name == "d":
             return ("n", float(self.ctx.d))

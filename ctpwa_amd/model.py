@@ -21,13 +21,16 @@ line-oriented text document, see :meth:`Plan.to_text`.
 """
 from __future__ import annotations
 
+import copy
 import itertools
 import os
 import warnings
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
+from . import customexpr
 from .config import ConfigError, ConfigParser, Particle, ResonanceConfig
+from .customexpr import PerWaveL
 
 # lineshape model codes shared with the native library
 MODEL_CODES = {"BWR": 0, "BW": 1, "ONE": 2, "Flatte": 3, "GS": 4, "Custom": 5, "Hist": 6, "Interp": 7}
@@ -141,8 +144,11 @@ class Block:
     nsl: int
     res: List[ResInst]
     chain_key: str
-    wave: int = -1           # partial wave: the block itself, or the block a permuted copy sums into
+    wave: int = -1           # partial wave: its first block (the block itself unless the wave
+                             # is split into runs of one L, or the block is a permuted copy)
     sigma: int = 0           # identical-particle permutation (0: the identity)
+    c0: int = 0              # first of the topology's SL combinations that the block's waves use
+    src: int = -1            # a permuted copy: the block it copies
 
 
 @dataclass
@@ -232,6 +238,8 @@ class Plan:
         out.append("aratio " + " ".join(repr(float(x)) for x in self.amp_ratio))
         if any(blk.wave != b for b, blk in enumerate(self.blocks)):
             out.append("bwave " + " ".join(str(x) for x in self.wave_of()))
+        if any(blk.c0 for blk in self.blocks):
+            out.append("bslc0 " + " ".join(str(blk.c0) for blk in self.blocks))
         out.append("ngauss %d" % len(self.gauss))
         for (s, mu, sig) in self.gauss:
             out.append("gauss %d %r %r" % (s, mu, sig))
@@ -552,6 +560,14 @@ def _identical_permutations(steps, root: str, groups, final_order: List[str]):
     return out
 
 
+def _custom_expr(text: str, names: List[str], L_values: List[int], nd) -> List[str]:
+    """Parse a Custom expression at node ``nd`` whose waves take the given L values."""
+    ctx = customexpr.Context(names, L_values, nd.bf_d,
+                             nd.masses[1] if nd.masses[1] > 0 else None,
+                             nd.masses[2] if nd.masses[2] > 0 else None)
+    return customexpr.serialize(customexpr.parse(text, ctx))
+
+
 def _symmetrise(topologies: List[Topology], blocks: List[Block], perms: Dict[int, list],
                 amp_names: List[str], amp_free: List[int], amp_ratio: List[float]) -> int:
     """Append one permuted copy of every topology with identical particles and of each of
@@ -559,11 +575,11 @@ def _symmetrise(topologies: List[Topology], blocks: List[Block], perms: Dict[int
     parameters, and its amplitudes take the original's coupling times the permutation's
     sign, so that sum_a v_a A_a carries A(x) + sum_sigma sgn(sigma) A(x_sigma)
     (src/AmpGen.cu:1440-1460).  Returns the new amplitude count."""
-    import copy
     n_amp = len(amp_names)
     n_orig = len(blocks)
     for b in range(n_orig):
-        blocks[b].wave = b
+        if blocks[b].wave < 0:
+            blocks[b].wave = b
     copies: Dict[Tuple[int, int], int] = {}
     for b in range(n_orig):
         blk = blocks[b]
@@ -580,7 +596,7 @@ def _symmetrise(topologies: List[Topology], blocks: List[Block], perms: Dict[int
                 copies[key] = len(topologies)
                 topologies.append(Topology(t.chain, list(t.comb), nodes, list(t.sl_combos)))
             blocks.append(Block(copies[key], n_amp, blk.nsl, copy.deepcopy(blk.res), blk.chain_key,
-                                wave=b, sigma=k))
+                                wave=blk.wave, sigma=k, c0=blk.c0, src=b))
             for c in range(blk.nsl):
                 a = blk.site + c
                 amp_names.append("%s#perm%d" % (amp_names[a], k))
@@ -692,6 +708,7 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
             res_lists = [ir_map[key] for key in comb]
             for rcombo in (itertools.product(*res_lists) if res_lists else [()]):
                 insts: List[ResInst] = []
+                per_wave: List[tuple] = []    # Custom resonances whose expression takes each wave's L
                 free_lists: List[List[int]] = []
                 range_lists: List[List[List[float]]] = []
                 for key, rconf in zip(comb, rcombo):
@@ -721,15 +738,14 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
                     if model in ("Hist", "Interp"):
                         inst.table = _table(cfg, rconf, model)
                     if model == "Custom":
-                        from . import customexpr
                         text = rconf.options.get("expression", rconf.options.get("expr"))
                         if not text:
                             raise ConfigError("Custom %s has no expression" % rconf.name)
-                        nd = nodes[node_k]
-                        ctx = customexpr.Context(names, [c[node_k][1] for c in sl_combos], nd.bf_d,
-                                                 nd.masses[1] if nd.masses[1] > 0 else None,
-                                                 nd.masses[2] if nd.masses[2] > 0 else None)
-                        inst.expr = customexpr.serialize(customexpr.parse(text, ctx))
+                        try:
+                            inst.expr = _custom_expr(text, names, [c[node_k][1] for c in sl_combos],
+                                                     nodes[node_k])
+                        except PerWaveL:
+                            per_wave.append((len(insts), text, names, node_k))
                     insts.append(inst)
                     if rconf.free:
                         if len(rconf.free) == 1 and rconf.free[0] == -1:
@@ -768,7 +784,22 @@ def build_plan(cfg: ConfigParser, fit_mode: int = 1) -> Plan:
 
                 tagmap = {inst.tag: inst.name for inst in insts}
                 chain_key = chain.name + "".join("_%s[%s]" % (i.tag, i.name) for i in insts)
-                blocks.append(Block(topo_idx, gls, len(sl_combos), insts, chain_key))
+                if not per_wave:
+                    blocks.append(Block(topo_idx, gls, len(sl_combos), insts, chain_key))
+                else:
+                    # one block per run of SL combinations with one L at every such node: each
+                    # run's expressions take that L; the runs form one partial wave
+                    lkey = [tuple(c[k][1] for (_, _, _, k) in per_wave) for c in sl_combos]
+                    first, c0 = len(blocks), 0
+                    while c0 < len(sl_combos):
+                        c1 = c0 + 1
+                        while c1 < len(sl_combos) and lkey[c1] == lkey[c0]:
+                            c1 += 1
+                        sub = [copy.copy(x) for x in insts]
+                        for (ii, text, nms, k) in per_wave:
+                            sub[ii].expr = _custom_expr(text, nms, [sl_combos[c0][k][1]], nodes[k])
+                        blocks.append(Block(topo_idx, gls + c0, c1 - c0, sub, chain_key, wave=first, c0=c0))
+                        c0 = c1
                 for c in sl_combos:
                     parts = []
                     for nd, (S, L) in zip(nodes, c):

@@ -476,7 +476,8 @@ struct ONode { int dj, dj1, dj2, d1ref, d2ref, has_bf, lmin; double bf_d, mm, md
 struct OTopo { std::vector<ONode> nodes; std::vector<std::vector<std::pair<int, int>>> slc; };
 struct ORes { int node, model; std::vector<double> params; std::vector<int> slots; std::vector<double> ch;
               std::vector<std::string> expr; std::vector<double> tab; };
-struct OBlock { int topo, site, nsl; std::vector<ORes> res; };
+// c0: first SL combination of the topology used by the block (runs of one L)
+struct OBlock { int topo, site, nsl, c0 = 0; std::vector<ORes> res; };
 struct OPlan {
     int nfinal = 0, npol = 0;
     std::vector<int> polmap;
@@ -533,6 +534,8 @@ static OPlan parse_plan(const char* text) {
             std::vector<double> t((size_t)n);
             for (auto& x : t) ls >> x;
             P.blocks.back().res.back().tab = t;
+        } else if (key == "bslc0") {
+            for (auto& B : P.blocks) ls >> B.c0;
         } else if (key == "theta") { int j; double a, lo, hi; ls >> j >> a >> lo >> hi; P.th_init.push_back(a); }
         else if (key == "ntheta") ls >> P.ntheta;
         else if (key == "namp") ls >> P.namp >> P.nfree;
@@ -882,7 +885,7 @@ static CJ block_factor(const OPlan& P, const OBlock& B, int c, const double* kin
     CJ F{Jet(1.0), Jet(0.0)};
     for (int k = 0; k < nn; ++k) {
         const ONode& n = T.nodes[k];
-        int L = T.slc[c][k].second;
+        int L = T.slc[B.c0 + c][k].second;
         double mm = kin[KS * k], qq = kin[KS * k + 1], md1 = kin[KS * k + 2], md2 = kin[KS * k + 3];
         // q0 daughter masses: config mass, else the mass parameter of the block resonance
         // occupying the daughter (1 when it has no parameters, src/AmpGen.cu:1504-1527),
@@ -1028,7 +1031,7 @@ static void eval_class(const Oracle& O, int cls, const std::vector<cd>& v, const
                     CJ F = block_factor(P, B, c, kin, vars, theta);
                     int a = B.site + c;
                     for (int p = 0; p < npol; ++p) {
-                        cd sl = s.slamp[B.topo][((size_t)c * s.n + e) * npol + p];
+                        cd sl = s.slamp[B.topo][((size_t)(B.c0 + c) * s.n + e) * npol + p];
                         A[a * npol + p] = cd(F.re.v, F.im.v) * sl;
                         for (int i = 0; i < g_nv; ++i)
                             dA[((size_t)a * npol + p) * nt + vars[i]] += cd(F.re.g[i], F.im.g[i]) * sl;
@@ -1317,7 +1320,7 @@ int orc_amplitudes(void* h, const double* params, int cls, double* out) {
             for (int c = 0; c < B.nsl; ++c) {
                 CJ F = block_factor(P, B, c, kin, O.bvars[b], theta);
                 for (int p = 0; p < P.npol; ++p) {
-                    cd a = cd(F.re.v, F.im.v) * s.slamp[B.topo][((size_t)c * s.n + e) * P.npol + p];
+                    cd a = cd(F.re.v, F.im.v) * s.slamp[B.topo][((size_t)(B.c0 + c) * s.n + e) * P.npol + p];
                     size_t i = ((size_t)e * P.npol + p) * P.namp + B.site + c;
                     out[2 * i] = a.real();
                     out[2 * i + 1] = a.imag();

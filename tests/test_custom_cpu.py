@@ -173,3 +173,81 @@ def test_custom_modules_compile_for_gfx950():
     plan = plan_of(_custom("exp(-p1*m) * cos(p0*m) / sqrt(p0^2 + m^2) + BWR(p0, p1, 1, 3.0)"))
     base = plan_of(W.JPSI_3PI)
     assert _native.compile_check(plan.to_text(), "fp64") > _native.compile_check(base.to_text(), "fp64")
+
+
+# X(1-) -> n nbar: the waves at the X node have L = 0 and L = 2 (S = 1)
+PER_WAVE_L = """\
+Particles:
+  Jpsi: {J: 1, P: -1, mass: 3.0969, polarization: [1, -1]}
+  n1:  {J: 1/2, P: 1, mass: 0.93957}
+  n2:  {J: 1/2, P: -1, mass: 0.93957}
+  pi0: {J: 0, P: -1, mass: 0.13498}
+Data:
+  order: [n1, n2, pi0]
+  data: [npy, data.npy]
+  phsp: [npy, phsp.npy]
+DecayChains:
+  nstar:
+    Jpsi: [[n1, Nst]]
+    Nst: [n2, pi0]
+    intermediates:
+      Nst:
+        - [J: 1/2, P: -1]: [N1535]
+        - [J: 3/2, P: -1]: [N1520]
+  x:
+    Jpsi: [[pi0, R_nn]]
+    R_nn: [n1, n2]
+    intermediates:
+      R_nn:
+        - [J: 1, P: -1]: [X2000]
+Resonances:
+  N1535: {J: 1/2, P: -1, model: BWR, parameters: [1.53, 0.15], free: [0, 1]}
+  N1520: {J: 3/2, P: -1, model: BWR, parameters: [1.515, 0.11], free: [0, 1]}
+  X2000: {J: 1, P: -1, model: BWR, parameters: [2.0, 0.3], free: [0, 1]}
+"""
+X_BWR = "  X2000: {J: 1, P: -1, model: BWR, parameters: [2.0, 0.3], free: [0, 1]}"
+# the built-in BWR: running width with the node's lowest L, vertex barrier with each wave's L
+X_CUSTOM = ("  X2000: {J: 1, P: -1, model: Custom, paramValues: [2.0, 0.3], paramNames: [p0, p1], "
+            "free: [0, 1], expression: 'BWR(p0, p1, 0, 3.0) * Bf(L, q, q0(p0), 3.0)'}")
+PER_WAVE_MASSES = (0.93957, 0.93957, 0.13498)
+
+
+def test_per_wave_L_splits_the_block_and_equals_builtin(oracle_lib):
+    """A Custom expression using L at a node whose waves differ in L (src/CustomExpr.cu:413,
+    evaluated per wave at run time in the reference) is parsed once per run of waves with
+    one L; the runs are blocks over consecutive SL combinations of one partial wave.  The
+    Custom spelling of the built-in BWR then gives the built-in model's NLL, gradient and
+    Hessian, amplitude list, partial waves and SL vector counts."""
+    from test_identical_cpu import _sample
+    from ctpwa_amd import _native
+    with pytest.raises(ce.PerWaveL):
+        ce.parse("BW(a, 0.1) * Bf(L, q, 0.3, 3.0)", ce.Context(["a"], [0, 2], 3.0, 0.9, 0.9))
+    assert X_BWR in PER_WAVE_L
+    pb, pc = plan_of(PER_WAVE_L), plan_of(PER_WAVE_L.replace(X_BWR, X_CUSTOM))
+    assert len(pc.blocks) == len(pb.blocks) + 1
+    xb = [b for b, B in enumerate(pc.blocks) if B.res[0].name == "X2000"]
+    assert [(pc.blocks[b].c0, pc.blocks[b].nsl, pc.blocks[b].wave) for b in xb] == [(0, 1, xb[0]), (1, 1, xb[0])]
+    L_at = [pc.topologies[pc.blocks[b].topo].sl_combos[pc.blocks[b].c0][1][1] for b in xb]
+    assert L_at == [0, 2]
+    assert pc.amp_names == pb.amp_names and pc.n_amp == pb.n_amp
+    assert len(pc.waves) == len(pb.waves) and pc.wave_of()[xb[1]] == pc.wave_of()[xb[0]]
+    assert "bslc0 " in pc.to_text() and "bwave " in pc.to_text()
+    counts = [0] * len(pc.waves)
+    for w, B in zip(pc.wave_of(), pc.blocks):
+        counts[w] += B.nsl
+    assert counts == [B.nsl for B in pb.blocks]
+    # the native plan reader accepts the split plan and builds its kernels
+    assert _native.compile_check(pc.to_text(), "fp64") > 0
+    ph, da = _sample(3000, 1, PER_WAVE_MASSES), _sample(800, 2, PER_WAVE_MASSES)
+    ob = oracle_lib.Oracle(pb.to_text(), 3, ph, da)
+    oc = oracle_lib.Oracle(pc.to_text(), 3, ph, da)
+    p = perturbed(W.nominal_params(pb, 1), pb, 2)
+    nb, gb, Hb = ob.hessian(p)
+    nc, gc, Hc = oc.hessian(p)
+    assert abs(nc - nb) <= 1e-12 * abs(nb)
+    assert rel(gc, gb) <= 1e-10
+    assert rel(Hc, Hb) <= 1e-9
+    # a planted mistake: one L for every wave moves the NLL
+    bad = plan_of(PER_WAVE_L.replace(X_BWR, X_CUSTOM.replace("Bf(L,", "Bf(0,")))
+    nx, _ = oracle_lib.Oracle(bad.to_text(), 3, ph, da).nll_grad(p)
+    assert abs(nx - nb) > 1e-6 * abs(nb)

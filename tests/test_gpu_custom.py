@@ -154,3 +154,41 @@ def test_custom_every_handle_and_batch(oracle_lib):
     for j in range(3):
         assert rel(G[j, 1:], g_o) <= 1e-7
     b.close()
+
+
+def test_custom_per_wave_L(oracle_lib):
+    """A Custom expression taking each wave's L (a block split into runs of one L) against
+    the oracle of the built-in model it spells: NLL 1e-9, gradient 1e-7, Hessian 1e-6, and
+    per-event partial-wave intensities equal to the built-in plan's on the device."""
+    from test_custom_cpu import PER_WAVE_L, PER_WAVE_MASSES, X_BWR, X_CUSTOM
+    from test_identical_cpu import _sample
+    from ctpwa_amd import _native
+    pb, pc = plan_of(PER_WAVE_L), plan_of(PER_WAVE_L.replace(X_BWR, X_CUSTOM))
+    ph, da = _sample(20000, 81, PER_WAVE_MASSES), _sample(4000, 82, PER_WAVE_MASSES)
+    O = oracle_lib.Oracle(pb.to_text(), 3, ph, da)
+    print("case custom per-wave L", flush=True)
+    hc = _native.Handle(pc.to_text(), ph, da, None)
+    hb = _native.Handle(pb.to_text(), ph, da, None)
+    s = torch.cuda.current_stream().cuda_stream
+    for k, p in enumerate([W.nominal_params(pb, 1), perturbed(W.nominal_params(pb, 1), pb, 7, 0.1)]):
+        nll_o, g_o, H_o = O.hessian(p)
+        t = torch.tensor(p, dtype=torch.float64, device="cuda")
+        out = torch.empty(1 + hc.n_params, dtype=torch.float64, device="cuda")
+        Ht = torch.empty((hc.n_params, hc.n_params), dtype=torch.float64, device="cuda")
+        hc.nll_grad(t.data_ptr(), out.data_ptr(), s)
+        hc.hessian(t.data_ptr(), Ht.data_ptr(), s)
+        torch.cuda.synchronize()
+        o = out.cpu().numpy()
+        print("per-wave L", k, nll_o, o[0], rel(o[1:], g_o), rel(Ht.cpu().numpy(), H_o), flush=True)
+        assert abs(o[0] - nll_o) <= 1e-9 * abs(nll_o)
+        assert rel(o[1:], g_o) <= 1e-7
+        assert rel(Ht.cpu().numpy(), H_o) <= 1e-6
+        for cls in (0, 1):
+            n = hc.n_events[cls]
+            pw = [torch.empty((len(pb.waves) + 1, n), dtype=torch.float64, device="cuda") for _ in range(2)]
+            hc.partial_intensities(t.data_ptr(), cls, pw[0].data_ptr(), s)
+            hb.partial_intensities(t.data_ptr(), cls, pw[1].data_ptr(), s)
+            torch.cuda.synchronize()
+            assert rel(pw[0].cpu().numpy(), pw[1].cpu().numpy()) <= 1e-10
+    hc.close()
+    hb.close()
