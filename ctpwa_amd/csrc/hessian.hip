@@ -52,7 +52,8 @@ __device__ inline double rec_h(const HessArgs& a, int rec, int i, int k) {
 // the product rule, so a factor that vanishes needs no division.
 // NL bounds the block's local variables (B.nloc <= NL): every array is indexed with
 // unrolled constant indices and lives in registers.
-template <int NL>
+// H2 = false: value and first derivatives only (Mm is left unset)
+template <int NL, bool H2 = true>
 __device__ inline void block_terms(const HessArgs& a, int b, long long e, Cd& psi, Cd* L, Cd* Mm) {
     const DBlock& B = a.blocks[b];
     const int nl = B.nloc;
@@ -85,6 +86,7 @@ __device__ inline void block_terms(const HessArgs& a, int b, long long e, Cd& ps
                 if (i >= nl) break;
                 double gi = rec_g(a, D.rec + tm.coef, i) * x;
                 if (im) dD[i].i -= gi; else dD[i].r += gi;
+                if constexpr (!H2) continue;
 #pragma unroll
                 for (int k = 0; k < NL; ++k) {
                     if (k >= nl) break;
@@ -101,6 +103,7 @@ __device__ inline void block_terms(const HessArgs& a, int b, long long e, Cd& ps
             if (i >= nl) break;
             Cd li = cm(pd, dD[i]);
             L[i] = ca(L[i], li);
+            if constexpr (!H2) continue;
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 if (k >= nl) break;
@@ -111,7 +114,7 @@ __device__ inline void block_terms(const HessArgs& a, int b, long long e, Cd& ps
     // log-derivatives -> derivatives of psi (P2 first: it reads L)
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
-        if (i >= nl) break;
+        if (!H2 || i >= nl) break;
 #pragma unroll
         for (int k = 0; k < NL; ++k) {
             if (k >= nl) break;
@@ -132,7 +135,7 @@ __device__ inline void block_terms(const HessArgs& a, int b, long long e, Cd& ps
             Fi[i] = (i < nl) ? Cd{f[(2 + i) * ld], f[(2 + ncv + i) * ld]} : Cd{0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < NL; ++i) {
-            if (i >= nl) break;
+            if (!H2 || i >= nl) break;
 #pragma unroll
             for (int k = 0; k < NL; ++k) {
                 if (k >= nl) break;
@@ -235,7 +238,7 @@ __global__ void __launch_bounds__(HR_THREADS) k_hess_rows(HessArgs a, long long 
     for (int b = 0; b < a.nblock; ++b) {
         const DBlock& B = a.blocks[b];
         if (B.nsl == 0) continue;
-        block_terms<NL>(a, b, e, psi, L, Mm);
+        block_terms<NL, false>(a, b, e, psi, L, Mm);
         Cd dJ[JREG ? NL * SP : 1];
         if constexpr (JREG) {
 #pragma unroll
