@@ -561,10 +561,13 @@ def _identical_permutations(steps, root: str, groups, final_order: List[str]):
 
 
 def _custom_expr(text: str, names: List[str], L_values: List[int], nd) -> List[str]:
-    """Parse a Custom expression at node ``nd`` whose waves take the given L values."""
-    ctx = customexpr.Context(names, L_values, nd.bf_d,
-                             nd.masses[1] if nd.masses[1] > 0 else None,
-                             nd.masses[2] if nd.masses[2] > 0 else None)
+    """Parse a Custom expression at node ``nd`` whose waves take the given L values.  A
+    daughter's mass is its configured mass when positive, 0 for a final-state particle
+    configured massless (a photon), as in the built-in models' q0."""
+    def dmass(m, ref):
+        return m if (m > 0 or (m == 0 and ref >= 0)) else None
+    ctx = customexpr.Context(names, L_values, nd.bf_d, dmass(nd.masses[1], nd.d1_ref),
+                             dmass(nd.masses[2], nd.d2_ref))
     return customexpr.serialize(customexpr.parse(text, ctx))
 
 

@@ -251,3 +251,40 @@ def test_per_wave_L_splits_the_block_and_equals_builtin(oracle_lib):
     bad = plan_of(PER_WAVE_L.replace(X_BWR, X_CUSTOM.replace("Bf(L,", "Bf(0,")))
     nx, _ = oracle_lib.Oracle(bad.to_text(), 3, ph, da).nll_grad(p)
     assert abs(nx - nb) > 1e-6 * abs(nb)
+
+
+def _b1_case():
+    """J/psi -> gamma pi0 pi0 with b1(1235)(1+) -> pi0 gamma (waves with L = 0 and 2) next to
+    the omega, the two pi0 identical: the split runs get permuted copies."""
+    from test_identical_cpu import BOSONS
+    b1 = "  b1_1235:  {J: 1, P: 1, model: BWR, parameters: [1.23, 0.14], free: [0, 1]}"
+    tb = BOSONS.replace("        - [J: 1, P: -1]: [omega782]\n",
+                        "        - [J: 1, P: -1]: [omega782]\n        - [J: 1, P: 1]: [b1_1235]\n")
+    tb = tb.replace("Constraints:", b1 + "\nConstraints:")
+    tc = tb.replace(b1, "  b1_1235:  {J: 1, P: 1, model: Custom, paramValues: [1.23, 0.14], paramNames: [p0, p1], "
+                        "free: [0, 1], expression: 'BWR(p0, p1, 0, 3.0) * Bf(L, q, q0(p0), 3.0)'}")
+    return tb, tc, (0.0, 0.13498, 0.13498)
+
+
+def test_per_wave_L_with_identical_particles(oracle_lib):
+    """The runs of a split block are permuted like any block: each copy keeps its run's
+    first SL combination and sums into the same partial wave.  The photon daughter is a
+    configured massless final-state particle (mass 0 in q0).  Same NLL, gradient and Hessian
+    as the built-in BWR."""
+    from test_identical_cpu import _sample
+    tb, tc, masses = _b1_case()
+    pb, pc = plan_of(tb), plan_of(tc)
+    runs = [b for b, B in enumerate(pc.blocks) if B.res[0].name == "b1_1235" and B.sigma == 0]
+    assert len(runs) > 1 and len({pc.blocks[b].wave for b in runs}) == 1
+    for b, B in enumerate(pc.blocks):
+        if B.sigma:
+            S = pc.blocks[B.src]
+            assert (B.c0, B.nsl, B.wave, B.res[0].expr) == (S.c0, S.nsl, S.wave, S.res[0].expr)
+    assert pc.amp_names == pb.amp_names and len(pc.waves) == len(pb.waves)
+    ph, da = _sample(3000, 1, masses), _sample(800, 2, masses)
+    p = perturbed(W.nominal_params(pb, 1), pb, 2)
+    nb, gb, Hb = oracle_lib.Oracle(pb.to_text(), 3, ph, da).hessian(p)
+    nc, gc, Hc = oracle_lib.Oracle(pc.to_text(), 3, ph, da).hessian(p)
+    assert abs(nc - nb) <= 1e-12 * abs(nb)
+    assert rel(gc, gb) <= 1e-10
+    assert rel(Hc, Hb) <= 1e-9

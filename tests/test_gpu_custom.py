@@ -156,17 +156,23 @@ def test_custom_every_handle_and_batch(oracle_lib):
     b.close()
 
 
-def test_custom_per_wave_L(oracle_lib):
-    """A Custom expression taking each wave's L (a block split into runs of one L) against
-    the oracle of the built-in model it spells: NLL 1e-9, gradient 1e-7, Hessian 1e-6, and
-    per-event partial-wave intensities equal to the built-in plan's on the device."""
-    from test_custom_cpu import PER_WAVE_L, PER_WAVE_MASSES, X_BWR, X_CUSTOM
+@pytest.mark.parametrize("which", ["nnbar", "b1_identical"])
+def test_custom_per_wave_L(oracle_lib, which):
+    """A Custom expression taking each wave's L (a block split into runs of one L; with
+    identical particles also their permuted copies) against the oracle of the built-in
+    model it spells: NLL 1e-9, gradient 1e-7, Hessian 1e-6, and per-event partial-wave
+    intensities equal to the built-in plan's on the device."""
+    from test_custom_cpu import PER_WAVE_L, PER_WAVE_MASSES, X_BWR, X_CUSTOM, _b1_case
     from test_identical_cpu import _sample
     from ctpwa_amd import _native
-    pb, pc = plan_of(PER_WAVE_L), plan_of(PER_WAVE_L.replace(X_BWR, X_CUSTOM))
-    ph, da = _sample(20000, 81, PER_WAVE_MASSES), _sample(4000, 82, PER_WAVE_MASSES)
+    if which == "nnbar":
+        tb, tc, masses = PER_WAVE_L, PER_WAVE_L.replace(X_BWR, X_CUSTOM), PER_WAVE_MASSES
+    else:
+        tb, tc, masses = _b1_case()
+    pb, pc = plan_of(tb), plan_of(tc)
+    ph, da = _sample(20000, 81, masses), _sample(4000, 82, masses)
     O = oracle_lib.Oracle(pb.to_text(), 3, ph, da)
-    print("case custom per-wave L", flush=True)
+    print("case custom per-wave L", which, flush=True)
     hc = _native.Handle(pc.to_text(), ph, da, None)
     hb = _native.Handle(pb.to_text(), ph, da, None)
     s = torch.cuda.current_stream().cuda_stream
