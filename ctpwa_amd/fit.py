@@ -13,6 +13,9 @@ Follows the reference's driver (example/fit.py:84-260, ``UnifiedPWAOptimizer``):
   uses the standard projection;
 * ``torch.optim.LBFGS`` with the strong-Wolfe line search (max_iter 500,
   tolerance_grad 1e-8, tolerance_change 1e-10, history 100);
+* after L-BFGS, a damped-Newton polish on the exact Hessian (example/fit.py:308-385,
+  on unless ``polish=False``, as the reference's FIT_POLISH=1 default), kept when it
+  lowers the NLL;
 * errors from the inverse of the Hessian with the two fixed parameters removed.
   ``getHessian`` uses the grouped layout [Re p, Im p, theta]; the reference's error
   extraction reads it as interleaved (example/fit.py:255-258) — here the errors are
@@ -40,6 +43,7 @@ class FitResult:
     pos_def: bool = False
     unconstrained: List[int] = field(default_factory=list)   # parameters the NLL does not depend on
     nll_history: List[float] = field(default_factory=list)
+    polish_steps: int = 0              # damped-Newton steps taken after L-BFGS (0: none)
 
 
 class Fitter:
@@ -87,7 +91,8 @@ class Fitter:
         return nll, g
 
     def minimize(self, p_init, max_iter: int = 500, tolerance_grad: float = 1e-8,
-                 tolerance_change: float = 1e-10, history_size: int = 100, hessian: bool = True) -> FitResult:
+                 tolerance_change: float = 1e-10, history_size: int = 100, hessian: bool = True,
+                 polish: bool = True) -> FitResult:
         t0 = time.perf_counter()
         self.n_evals = 0
         p = torch.as_tensor(np.asarray(p_init, dtype=np.float64)).to(self.ana.device).clone().requires_grad_(True)
@@ -109,12 +114,77 @@ class Fitter:
         nll, g = self.loss_and_grad(final.clone().requires_grad_(True))
         state = opt.state[opt._params[0]]
         converged = bool(state.get("n_iter", 0) < max_iter)
+        steps = 0
+        if polish:
+            p2, nll2, steps = self.polish(final)
+            if nll2 < float(nll.item()):
+                final, nll = p2, torch.tensor(nll2)
         res = FitResult(params=final.cpu().numpy(), nll=float(nll.item()), n_evals=self.n_evals,
-                        converged=converged, seconds=0.0, nll_history=hist)
+                        converged=converged, seconds=0.0, nll_history=hist, polish_steps=steps)
         if hessian:
             self.errors(res)
         res.seconds = time.perf_counter() - t0
         return res
+
+    def polish(self, p_init, max_steps: int = 40, tol: float = 1e-6, lam0: float = 1.0):
+        """Damped Newton on the exact Hessian (example/fit.py:308-385).  In the subspace
+        without the two fixed parameters, solve (H + lam I) d = -g with lam at least
+        -lambda_min(H) + 1e-6, so that d descends where H is not positive definite; the
+        step is clamped into the box; an accepted step (NLL lower by more than tol) scales
+        lam by 0.3 (not below 1e-6), a rejected one by 10, and lam above 1e10 ends it.  Once
+        H is positive definite, one undamped Newton step (kept if it lowers the NLL) ends
+        it.  Returns (params, nll, steps)."""
+        dev = self.ana.device
+        n = self.n
+        mask = torch.ones(self.np, dtype=torch.bool, device=dev)
+        mask[0] = False
+        mask[n] = False
+        nf = int(mask.sum().item())
+        best = torch.as_tensor(np.asarray(p_init.cpu() if torch.is_tensor(p_init) else p_init,
+                                          dtype=np.float64)).to(dev).clone()
+        self._clamp_(best)
+
+        def nll_at(q):
+            self.n_evals += 1
+            return float(self.ana.getNLL(q).item())
+
+        nll_best = nll_at(best)
+        lam = lam0
+        steps = 0
+        for step in range(max_steps):
+            steps = step + 1
+            H = self.ana.getHessian(best)[mask][:, mask]
+            eig_min = float(torch.linalg.eigvalsh(H)[0].item())
+            q = best.clone().requires_grad_(True)
+            self.n_evals += 1
+            (g,) = torch.autograd.grad(self.ana.getNLL(q), q)
+            gm = g[mask]
+            if eig_min > 1e-8:
+                cand = best.clone()
+                cand[mask] = best[mask] + torch.linalg.solve(H, -gm)
+                self._clamp_(cand)
+                nc = nll_at(cand)
+                if nc < nll_best - tol:
+                    best, nll_best = cand, nc
+                break
+            lam = max(lam, -eig_min + 1e-6)
+            Hd = H + lam * torch.eye(nf, dtype=torch.float64, device=dev)
+            try:
+                d = torch.linalg.solve(Hd, -gm)
+            except RuntimeError:
+                d = torch.linalg.lstsq(Hd, -gm.unsqueeze(1)).solution.squeeze(1)
+            cand = best.clone()
+            cand[mask] = best[mask] + d
+            self._clamp_(cand)
+            nc = nll_at(cand)
+            if nc < nll_best - tol:
+                best, nll_best = cand, nc
+                lam = max(lam * 0.3, 1e-6)
+            else:
+                lam *= 10.0
+                if lam > 1e10:
+                    break
+        return best.detach(), nll_best, steps
 
     def errors(self, res: FitResult) -> FitResult:
         """Hessian at the minimum, positive-definiteness, and 1-sigma errors (grouped layout)."""

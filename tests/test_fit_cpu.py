@@ -84,3 +84,34 @@ def test_bounds_and_projected_gradient():
     p = torch.tensor(r.params, requires_grad=True)
     _, g = f.loss_and_grad(p)
     assert float(g.abs().max()) < 1e-6
+
+
+def test_polish_after_early_stop():
+    """The damped-Newton polish (example/fit.py:308-385) takes an L-BFGS run stopped after
+    one iteration to the minimum of the quadratic (with a flat direction: the damped
+    branch), and is kept only when it lowers the NLL."""
+    n, P, mu, A, lo, hi = _problem(3)
+    ana = QuadraticAnalysis(n, P, mu, A, lo, hi)
+    start = mu + 0.4
+    start[-1] = mu[-1]
+    raw = Fitter(ana).minimize(start, max_iter=1, hessian=False, polish=False)
+    r = Fitter(ana).minimize(start, max_iter=1, hessian=False)
+    assert raw.nll > 1e-3 and raw.polish_steps == 0
+    assert r.polish_steps >= 1
+    assert np.allclose(r.params[:-1], mu[:-1], atol=1e-6)
+    assert r.nll < 1e-6          # steps gaining less than the acceptance tolerance (1e-6) stop it
+    assert r.params[0] == 1.0 and r.params[n] == 0.0
+
+
+def test_polish_keeps_the_box():
+    """Steps are clamped into the bounds: with the minimum of the last theta above its
+    upper bound the polish ends on the bound, no worse than L-BFGS alone."""
+    n, P, mu, A, lo, hi = _problem(2, dead=False)
+    hi = np.array([5.0, 5.0, 0.2])
+    ana = QuadraticAnalysis(n, P, mu, A, lo, hi)
+    f = Fitter(ana)
+    p2, nll2, steps = f.polish(mu.copy())
+    assert steps >= 1 and float(p2[-1]) <= 0.2
+    r0 = Fitter(ana).minimize(mu.copy(), tolerance_grad=1e-12, tolerance_change=1e-14, hessian=False, polish=False)
+    r1 = Fitter(ana).minimize(mu.copy(), tolerance_grad=1e-12, tolerance_change=1e-14, hessian=False)
+    assert r1.nll <= r0.nll and r1.params[-1] <= 0.2
