@@ -220,16 +220,37 @@ def fit(ana, p_init, **kw) -> FitResult:
     return Fitter(ana).minimize(p_init, **kw)
 
 
-def multistart(ana, n_starts: int, seed: int = 0, scale: float = 1.0, **kw) -> List[FitResult]:
+def multistart(ana, n_starts: int, seed: int = 0, scale: float = 1.0, checkpoint: Optional[str] = None,
+               warm_start=None, **kw) -> List[FitResult]:
     """Fits from random couplings (|v| ~ U[0, scale], arg ~ U[0, 2 pi)) and resonance
-    parameters at their initial values; results sorted by NLL."""
+    parameters at their initial values; results sorted by NLL.  As the reference's
+    multi-start run (example/fit.py:600-650): ``warm_start`` (parameters, or the path of a
+    checkpoint) gives run 0 its couplings, the resonance parameters staying at their
+    initial values; with ``checkpoint`` the best parameters so far are saved there (a CPU
+    float64 tensor, ``torch.save``) after every run that reaches or lowers the best NLL."""
     f = Fitter(ana)
     rng = np.random.default_rng(seed)
     init = ana.getFreeResParams()[0].numpy()
+    if isinstance(warm_start, str):
+        warm_start = load_checkpoint(warm_start)
     out = []
-    for _ in range(n_starts):
+    best = float("inf")
+    for i in range(n_starts):
         mag = rng.uniform(0, scale, f.n)
         ph = rng.uniform(0, 2 * np.pi, f.n)
         p0 = np.concatenate([mag * np.cos(ph), mag * np.sin(ph), init])
-        out.append(f.minimize(p0, **kw))
+        if i == 0 and warm_start is not None:
+            w = np.asarray(warm_start.cpu() if torch.is_tensor(warm_start) else warm_start, dtype=np.float64)
+            p0[:2 * f.n] = w[:2 * f.n]
+        r = f.minimize(p0, **kw)
+        out.append(r)
+        if checkpoint and r.nll <= best:
+            best = r.nll
+            torch.save(torch.as_tensor(r.params, dtype=torch.float64), checkpoint)
     return sorted(out, key=lambda r: r.nll)
+
+
+def load_checkpoint(path: str) -> torch.Tensor:
+    """Parameters saved by ``multistart(checkpoint=...)`` (or the reference's
+    best_params.pt), read with the loader that executes nothing from the file."""
+    return torch.load(path, weights_only=True).to(torch.float64)

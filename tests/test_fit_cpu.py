@@ -115,3 +115,19 @@ def test_polish_keeps_the_box():
     r0 = Fitter(ana).minimize(mu.copy(), tolerance_grad=1e-12, tolerance_change=1e-14, hessian=False, polish=False)
     r1 = Fitter(ana).minimize(mu.copy(), tolerance_grad=1e-12, tolerance_change=1e-14, hessian=False)
     assert r1.nll <= r0.nll and r1.params[-1] <= 0.2
+
+
+def test_multistart_checkpoint_and_warm_start(tmp_path):
+    """multistart saves the best parameters after every run that reaches or lowers the best
+    NLL (example/fit.py:600-650) and a warm start gives run 0 its couplings only."""
+    from ctpwa_amd.fit import load_checkpoint, multistart
+    n, P, mu, A, lo, hi = _problem(4)
+    ana = QuadraticAnalysis(n, P, mu, A, lo, hi)
+    cp = str(tmp_path / "best_params.pt")
+    rs = multistart(ana, 3, seed=1, checkpoint=cp, hessian=False)
+    saved = load_checkpoint(cp)
+    assert saved.dtype == torch.float64 and saved.shape == (2 * n + P,)
+    assert np.array_equal(saved.numpy(), rs[0].params)
+    # run 0 from the saved couplings: one L-BFGS iteration and the polish reach the minimum
+    w = multistart(ana, 1, seed=9, warm_start=cp, max_iter=1, hessian=False)
+    assert w[0].nll <= rs[0].nll + 1e-6
