@@ -1331,6 +1331,25 @@ int orc_amplitudes(void* h, const double* params, int cls, double* out) {
     return 0;
 }
 
+// contracted node tensors of topology `topo` for class cls, before any lineshape or
+// barrier factor: out[c][e][p] (interleaved complex) for SL combination c, event e and
+// polarisation p, exactly as event_slamp leaves them (src/AmpGen.cu:924-1217)
+int orc_slamp(void* h, int cls, int topo, double* out) {
+    Oracle& O = *(Oracle*)h;
+    if (topo < 0 || topo >= (int)O.P.topos.size()) return -1;
+    const Sample& s = O.S[cls];
+    const size_t nslc = O.P.topos[topo].slc.size(), npol = O.P.npol;
+    for (size_t c = 0; c < nslc; ++c)
+        for (int e = 0; e < s.n; ++e)
+            for (size_t p = 0; p < npol; ++p) {
+                const cd a = s.slamp[topo][((size_t)c * s.n + e) * npol + p];
+                const size_t i = ((size_t)c * s.n + e) * npol + p;
+                out[2 * i] = a.real();
+                out[2 * i + 1] = a.imag();
+            }
+    return 0;
+}
+
 // per-event intensity I = sum_p |sum_a v_a A|^2 of class cls
 int orc_intensity(void* h, const double* params, int cls, double* out) {
     Oracle& O = *(Oracle*)h;

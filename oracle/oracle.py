@@ -47,6 +47,7 @@ def lib():
         L.orc_hessian.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_amplitudes.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.orc_intensity.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
+        L.orc_slamp.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_dims.argtypes = [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_int)] * 4
         L.orc_cg.restype = ctypes.c_double
         L.orc_cg.argtypes = [ctypes.c_int] * 6
@@ -138,6 +139,15 @@ class Oracle:
         n = self.n_events[cls]
         out = np.zeros((n, self.n_pol, self.n_amp, 2))
         lib().orc_amplitudes(self.h, _ptr(p), cls, _ptr(out))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def slamp(self, cls: int, topo: int, n_slc: int):
+        """Contracted node tensors of one topology, before lineshapes and barriers:
+        S[c, e, p] complex128 (SL combination c, event e, polarisation p)."""
+        n = self.n_events[cls]
+        out = np.zeros((n_slc, n, self.n_pol, 2))
+        if lib().orc_slamp(self.h, cls, topo, _ptr(out)) != 0:
+            raise ValueError("no topology %d" % topo)
         return out[..., 0] + 1j * out[..., 1]
 
     def intensity(self, params, cls: int):
