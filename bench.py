@@ -41,9 +41,22 @@ def make_samples(workdir, name, n_data, n_phsp, n_bkg):
     return cfg
 
 
-def cpu_baseline(cfg_path, frac, threads):
-    """FP64 oracle on the host cores over a bounded sample (the first `frac` of every
-    sample); evaluations per second scaled to the full workload."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg_path, frac, threads, hessian=True):
+    """FP64 oracle (the reference's likelihood restated in C++/OpenMP) on the host cores:
+    NLL+gradient evaluations per second, the median of 5 evaluations at new parameter
+    points after 2 warm-ups, on the first `frac` of every sample (1.0: the whole workload),
+    scaled linearly in events when frac < 1; then one Hessian, timed alone."""
     from ctpwa_amd import workloads as W
     from ctpwa_amd.config import ConfigParser
     from ctpwa_amd.model import build_plan
@@ -60,19 +73,32 @@ def cpu_baseline(cfg_path, frac, threads):
     O = Oracle(plan.to_text(), 3, ph, da, bk, w_bkg=None if bk is None else 0.8, threads=threads)
     p0 = W.nominal_params(plan, 1)
     rng = np.random.default_rng(7)
-    O.nll_grad(p0)
+    pts = [p0 + 1e-3 * rng.standard_normal(p0.shape) for _ in range(7)]
     ts = []
-    for k in range(3):
-        p = p0 + 1e-3 * rng.standard_normal(p0.shape)
+    for k, p in enumerate(pts):
         t = time.perf_counter()
         O.nll_grad(p)
-        ts.append(time.perf_counter() - t)
+        if k >= 2:
+            ts.append(time.perf_counter() - t)
     t_sample = float(np.median(ts))
     t_full = t_sample * full / n
-    return {"value": 1.0 / t_full, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle (C++/OpenMP) NLL+gradient on the first %.0f%% of every sample "
-                      "(%d of %d events), median of 3 after 1 warm-up = %.3f s, scaled linearly "
-                      "in events to the full workload" % (100 * frac, n, full, t_sample)}
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    out = {"value": 1.0 / t_full, "unit": "evals/s", "cores": threads, "kind": "port",
+           "nproc": nproc, "cpu_model": cpu_model(), "omp_threads": threads,
+           "ms_per_eval": round(t_full * 1e3, 1),
+           "sample": "FP64 oracle (C++/OpenMP, %d threads) NLL+gradient on %s (%d of %d events), median of 5 "
+                     "evaluations at new points after 2 warm-ups = %.3f s%s"
+                     % (threads, "the whole workload" if frac >= 1.0 else "the first %.0f%% of every sample" % (100 * frac),
+                        n, full, t_sample, "" if frac >= 1.0 else ", scaled linearly in events to the full workload")}
+    if hessian:
+        t = time.perf_counter()
+        O.hessian(pts[-1])
+        th = (time.perf_counter() - t) * full / n
+        out["hessian_ms"] = round(th * 1e3, 1)
+    return out
 
 
 def run_form(ana, args, spec, torch, W, prec, sizes):
@@ -106,6 +132,16 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
     value = args.steps / dt
     nll_last = float(last[0].item())
     log("[bench] %s: %d steps in %.4f s: %.4f ms/step, NLL %.6f" % (prec, args.steps, dt, ms_per_step, nll_last))
+    # the rate a fit loop sees: the same steps with the loss read on the host every step
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(args.warmup, nsteps):
+        nll, g = step(pts[k])
+        float(nll.item())
+    dt_sync = time.perf_counter() - t
+    sync = {"value": round(args.steps / dt_sync, 3), "ms_per_step": round(dt_sync / args.steps * 1e3, 4),
+            "def": "the same steps with the loss copied to the host (nll.item()) after every step"}
+    log("[bench] %s synchronous: %.4f ms/step" % (prec, sync["ms_per_step"]))
 
     # dominant kernel: the fused per-event kernel, timed with HIP events on the stream it runs on
     lay = ana.layout()
@@ -149,7 +185,8 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
         torch.cuda.synchronize()
         hess_ms = (time.perf_counter() - t) / args.hessian_reps * 1e3
         log("[bench] %s getHessian %.3f ms" % (prec, hess_ms))
-    return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof, "hessian_ms": hess_ms}
+    return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof, "hessian_ms": hess_ms,
+            "sync": sync}
 
 
 def run_serving(args, samples, plan, spec, W, precisions):
@@ -182,13 +219,14 @@ def run_serving(args, samples, plan, spec, W, precisions):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="jpsi_3pi")
     ap.add_argument("--n-data", type=int, default=None)
     ap.add_argument("--n-phsp", type=int, default=None)
     ap.add_argument("--n-bkg", type=int, default=None)
-    ap.add_argument("--cpu-frac", type=float, default=0.05)
+    ap.add_argument("--cpu-frac", type=float, default=1.0,
+                    help="fraction of every sample the CPU baseline evaluates (1.0: the whole workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
@@ -247,7 +285,7 @@ def main():
     if len(precisions) == 2:
         other = results[precisions[1]]
         compare = {"precision": precisions[1], "value": round(other["value"], 3),
-                   "ms_per_step": round(other["ms_per_step"], 4), "roofline": other["roofline"],
+                   "ms_per_step": round(other["ms_per_step"], 4), "sync": other["sync"], "roofline": other["roofline"],
                    "nll": other["nll"],
                    "speedup_mixed_vs_fp64": round(r["value"] / other["value"], 3),
                    "delta_nll_mixed_minus_fp64": r["nll"] - other["nll"],
@@ -271,7 +309,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
         t0 = time.time()
-        cpu = cpu_baseline(cfg, args.cpu_frac, threads)
+        cpu = cpu_baseline(cfg, args.cpu_frac, threads, hessian=args.hessian_reps > 0)
         log("[bench] cpu baseline %.4f evals/s (%.1f s)" % (cpu["value"], time.time() - t0))
 
     desc = {"mixed": "mixed: FP32 amplitude (SL) tensors and invariants, FP64 squared masses, "
@@ -280,7 +318,9 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if primary == "fp64" else "f64 arithmetic, f32 amplitude storage",
+        "sync": r["sync"],
         "data": "synthetic",
         "config": {"workload": args.workload + ": J/psi -> pi+ pi- pi0, 3 chains x (rho770, rho1450, "
                    "rho1700) + f0(980) + f2(1270), %d amplitudes, %d free resonance parameters, "

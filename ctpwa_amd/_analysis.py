@@ -124,9 +124,11 @@ class analysis:
         self._weights = w
         self._truth = None
         # theta of the stored amplitudes: a host copy (initial values, reCalcAmp) or, after
-        # any evaluation, the theta the device evaluated last (read back only on demand,
-        # so evaluations never wait for the host)
+        # getNLL / getHessian / writeResult, a device copy of the theta they were called at
+        # (taken on the stream without a host wait, read back only on demand).  Other
+        # calls (intensities, amplitudes, Grams) do not move it.
         self._theta_host = np.asarray([t.init for t in self.plan.theta], dtype=np.float64)
+        self._theta_dev: Optional[torch.Tensor] = None
         self._theta_on_host = True
         self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
                                  w["phsp"], w["data"], w["bkg"], device=device, precision=precision)
@@ -407,13 +409,14 @@ class analysis:
         return A.reshape(-1)
 
     def _remember_theta(self, p: torch.Tensor):
-        # the native call that follows writes p's theta into the handle
+        # an asynchronous device copy of p's theta (no host synchronisation)
+        self._theta_dev = p[2 * self.plan.n_free:].clone()
         self._theta_on_host = False
 
     @property
     def _theta(self) -> np.ndarray:
         if not self._theta_on_host:
-            self._theta_host = self._h.last_theta(torch.cuda.current_stream(self.device).cuda_stream)
+            self._theta_host = self._theta_dev.cpu().numpy().astype(np.float64)
             self._theta_on_host = True
         return self._theta_host
 

@@ -20,6 +20,7 @@ EXPORTS = [
     "ctpwa_create", "ctpwa_destroy", "ctpwa_dims", "ctpwa_param_info", "ctpwa_nll_grad",
     "ctpwa_hessian", "ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_time_eval_kernel",
     "ctpwa_footprint_bytes", "ctpwa_hessian_workspace_bytes", "ctpwa_layout", "ctpwa_kernel_source", "ctpwa_generate_source",
+    "ctpwa_generate_hessian_source",
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
     "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
     "ctpwa_amplitude_gram", "ctpwa_partial_intensities", "ctpwa_partial_amplitudes", "ctpwa_last_theta",
@@ -63,6 +64,8 @@ def lib():
     L.ctpwa_kernel_source.argtypes = [_vp]
     L.ctpwa_generate_source.restype = _ll
     L.ctpwa_generate_source.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, _ll]
+    L.ctpwa_generate_hessian_source.restype = _ll
+    L.ctpwa_generate_hessian_source.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, _ll]
     L.ctpwa_compile_check.restype = _ll
     L.ctpwa_compile_check.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.ctpwa_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
@@ -106,6 +109,16 @@ def generate_source(plan_text: str, precision=FP64) -> str:
         raise NativeError(last_error())
     buf = ctypes.create_string_buffer(int(n))
     lib().ctpwa_generate_source(plan_text.encode(), pc, buf, n)
+    return buf.value.decode()
+
+
+def generate_hessian_source(plan_text: str, precision=FP64) -> str:
+    pc = precision_code(precision)
+    n = lib().ctpwa_generate_hessian_source(plan_text.encode(), pc, None, 0)
+    if n < 0:
+        raise NativeError(last_error())
+    buf = ctypes.create_string_buffer(int(n))
+    lib().ctpwa_generate_hessian_source(plan_text.encode(), pc, buf, n)
     return buf.value.decode()
 
 

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libctpwa_amd.so")
-SOURCES = ["init.hip", "prep.hip", "hessian.hip", "capi.hip", "codegen.cpp"]
+SOURCES = ["init.hip", "prep.hip", "hessian.hip", "capi.hip", "codegen.cpp", "codegen_hess.cpp"]
 ARCH = os.environ.get("CTPWA_ARCH", "gfx950")
 
 

@@ -22,6 +22,7 @@
 namespace ctpwa {
 std::string generate_eval_kernel(const Model_& M);
 std::string generate_custom2_kernel(const Model_& M);
+std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm);
 }
 
 extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctpwa::DTopo*, int, const ctpwa::DNode*,
@@ -38,23 +39,20 @@ extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, in
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
 
-extern "C" hipError_t ctpwa_launch_hess_rows(const ctpwa::HessArgs*, long long, long long, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_zero(double*, long long, hipStream_t);
-extern "C" hipError_t ctpwa_launch_wgram(const double*, int, long long, long long, const double*, long long,
-                                          const int2*, int, int, double*, int, hipStream_t);
 extern "C" long long ctpwa_hess_rows_waves(long long);
-extern "C" hipError_t ctpwa_launch_accum_cols(const double*, long long, long long, int, double*, hipStream_t);
-extern "C" hipError_t ctpwa_launch_accum(const double*, int, int, double*, hipStream_t);
-extern "C" int ctpwa_mfma_gram_ok(int);
+extern "C" hipError_t ctpwa_launch_accum_cols(const double*, long long, long long, int, const int*, double*, hipStream_t);
+extern "C" int ctpwa_gram_tables(int, int4*, int*, int2*);
+extern "C" int ctpwa_gram_nwg(int);
+extern "C" hipError_t ctpwa_launch_gram_panels(const void*, int, int, long long, long long, const double*, long long,
+                                                const int4*, int, int, double*, int, int, hipStream_t);
+extern "C" hipError_t ctpwa_launch_accum_panels(const double*, int, int, const int2*, int, double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, hipStream_t);
 extern "C" hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
                                                double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_partial_amp(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
                                                double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_gram_to_complex(const double*, int, double*, hipStream_t);
-extern "C" long long ctpwa_mfma_gram_part_doubles(int, int);
-extern "C" hipError_t ctpwa_launch_wgram_mfma(const double*, int, long long, long long, const double*, long long,
-                                               const int2*, int, double*, int, double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_hess_assemble(const ctpwa::HessAsm*, hipStream_t);
 
 using namespace ctpwa;
@@ -240,6 +238,20 @@ static Staging* staging(int device) {          // called with `device` current
 }
 
 // ---------------------------------------------------------------------------
+// matrix-core Gram of ncol columns: panel pairs, tile-pair list, row-share workgroups
+// and the partials buffer [nwg][ntp][256]
+struct GramTabs {
+    int ncol = 0, npp = 0, ntp = 0, nwg = 0;
+    int4* pp = nullptr;
+    int2* tiles = nullptr;
+    double* part = nullptr;
+};
+static long long gram_part_doubles(int ncol) {
+    int npp = 0;
+    const int ntp = ctpwa_gram_tables(ncol, nullptr, &npp, nullptr);
+    return (long long)ctpwa_gram_nwg(npp) * ntp * 256;
+}
+
 struct ctpwa_handle {
     int device = 0;
     Model_ M;
@@ -269,12 +281,21 @@ struct ctpwa_handle {
     int* wave_ptr = nullptr;               // partial wave w = blocks wave_blk[wave_ptr[w] .. wave_ptr[w+1])
     int* wave_blk = nullptr;
     std::string src;
-    // Hessian workspace (allocated on the first ctpwa_hessian call)
+    // Hessian workspace (allocated on the first ctpwa_hessian call): the generated rows
+    // kernel, one chunk of Jacobian rows (FP64, or FP32 in the mixed form), u rows,
+    // weights, per-wave column sums, and the matrix-core Gram tables and partials of the
+    // Jacobian (2C columns) and of u (NU columns)
     bool hess_ready = false;
     long long hchunk = 0, hcs_ld = 0;
-    int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0, hnwg = 0;
-    double *hJ = nullptr, *hU = nullptr, *hcs = nullptr, *hW = nullptr, *hpart = nullptr, *htot = nullptr, *hHext = nullptr;
-    double* hmpart = nullptr;              // per-wave tile partials of the matrix-core Gram
+    int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0;
+    hipModule_t hmod = nullptr;
+    hipFunction_t fn_hrows = nullptr;
+    void* hJ = nullptr;
+    double *hU = nullptr, *hcs = nullptr, *hW = nullptr, *htot = nullptr, *hHext = nullptr;
+    GramTabs hgJ, hgU;
+    int *hmap_ptr = nullptr, *hmap_x = nullptr;   // fit parameter -> extended entries (CSR)
+    int* hperm = nullptr;                         // rows kernel's summed columns -> [Z | u] order
+    double* hmap_r = nullptr;
     // models with Custom lineshapes: second-order factor rows for the Hessian, from a
     // module compiled on the first Hessian call
     hipModule_t mod2 = nullptr;
@@ -284,10 +305,8 @@ struct ctpwa_handle {
     // amplitude-Gram workspace (allocated on the first ctpwa_amplitude_gram call)
     bool agram_ready = false;
     long long gchunk = 0;
-    int gnG = 0;
-    double *gX = nullptr, *gW = nullptr, *gpart = nullptr, *gtot = nullptr;
-    int2* gkl = nullptr;
-    int2 *hklG = nullptr, *hklU = nullptr;
+    double *gX = nullptr, *gW = nullptr, *gtot = nullptr;
+    GramTabs gg;
     std::vector<int> used_inv;
     // storage precision (0 FP64, 1 mixed) and the FP32 stores of the mixed form
     int mixed = 0;
@@ -853,12 +872,6 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     return 0;
 }
 
-static std::vector<int2> upper_triangle(int n) {
-    std::vector<int2> v;
-    for (int a = 0; a < n; ++a)
-        for (int b = a; b < n; ++b) v.push_back(make_int2(a, b));
-    return v;
-}
 
 // Hessian workspace: Jacobian columns C, u-vector length NU, second-part scalars NZ per event
 struct HessDims { int C, NU, NZ; };
@@ -872,26 +885,43 @@ static HessDims hess_dims(const Model_& M) {
     }
     return d;
 }
-static const long long HESS_CHUNK = 1 << 19;     // events per Jacobian pass
-static const int HESS_NWG = 512;                 // split-K workgroups of the Gram
-static const int HESS_MFMA_NWG = 512;            // workgroups of the matrix-core Gram
-static long long hess_mfma_part_doubles(const HessDims& d) {
-    long long n = 0;
-    if (ctpwa_mfma_gram_ok(2 * d.C)) n = std::max(n, ctpwa_mfma_gram_part_doubles(2 * d.C, HESS_MFMA_NWG));
-    if (ctpwa_mfma_gram_ok(d.NU)) n = std::max(n, ctpwa_mfma_gram_part_doubles(d.NU, HESS_MFMA_NWG));
-    return n;
+// events per chunk of Jacobian rows: the rows of one chunk take at most this many bytes
+// (environment CTPWA_HESS_JBYTES overrides it for measurements)
+static long long hess_jbytes() {
+    if (const char* e = std::getenv("CTPWA_HESS_JBYTES")) {
+        long long v = std::atoll(e);
+        if (v >= (1LL << 20)) return v;
+    }
+    return 2LL << 30;
+}
+static long long hess_chunk(const Model_& M, long long maxn) {
+    const HessDims d = hess_dims(M);
+    const long long row = (long long)M.P.npol * 2 * d.C * 8;
+    long long c = std::max<long long>(256, hess_jbytes() / std::max<long long>(1, row) / 256 * 256);
+    return std::max<long long>(1, std::min(maxn, c));
 }
 static long long hess_workspace_bytes(const Model_& M, long long maxn, long long ntot) {
     HessDims d = hess_dims(M);
-    long long chunk = std::min<long long>(maxn, HESS_CHUNK);
+    long long chunk = hess_chunk(M, maxn);
     long long nG = (long long)(2 * d.C) * (2 * d.C + 1) / 2, nUu = (long long)d.NU * (d.NU + 1) / 2;
-    long long maxcol = std::max<long long>(2048, std::max(d.NU, d.NZ));
     long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + 2) + (long long)(d.NZ + d.NU) * ctpwa_hess_rows_waves(chunk) +
-                  HESS_NWG * maxcol + 3 * (nG + nUu + d.NZ + d.NU) +
-                  (long long)d.NU * d.NU;
-    w += hess_mfma_part_doubles(d);
+                  3 * (nG + nUu + d.NZ + d.NU) + (long long)d.NU * d.NU;
+    w += gram_part_doubles(2 * d.C) + gram_part_doubles(d.NU);
     w += (long long)M.customs.size() * M.ncf2() * ntot;   // second-order Custom factor rows
-    return w * 8 + (nG + nUu) * 8 + 12 * 256;    // + the index lists and allocation slack
+    return w * 8 + 64 * 1024;    // + the index tables and allocation slack
+}
+
+static void gram_setup(ctpwa_handle* h, GramTabs& g, int ncol) {
+    g.ncol = ncol;
+    int npp = 0;
+    g.ntp = ctpwa_gram_tables(ncol, nullptr, &npp, nullptr);
+    std::vector<int4> pp((size_t)npp);
+    std::vector<int2> tiles((size_t)g.ntp);
+    ctpwa_gram_tables(ncol, pp.data(), &g.npp, tiles.data());
+    g.nwg = ctpwa_gram_nwg(g.npp);
+    g.pp = h->upload(pp);
+    g.tiles = h->upload(tiles);
+    g.part = h->alloc<double>((size_t)g.nwg * g.ntp * 256);
 }
 
 static void hess_setup(ctpwa_handle* h) {
@@ -901,32 +931,50 @@ static void hess_setup(ctpwa_handle* h) {
     h->hC = hd.C;
     h->hNU = hd.NU;
     h->hNZ = hd.NZ;
-    if (2 * h->hC > 500) throw std::runtime_error("Hessian: more than 250 Jacobian columns is not built");
-    {
-        std::vector<int> mult(P.nfree, 0);
-        for (int a = 0; a < P.namp; ++a)
-            if (++mult[P.amap[a]] > 64)
-                throw std::runtime_error("Hessian: more than 64 amplitudes share one free coupling");
-    }
-    auto klG = upper_triangle(2 * h->hC), klU = upper_triangle(h->hNU);
-    h->hnG = (int)klG.size();
-    h->hnUu = (int)klU.size();
-    h->hklG = h->upload(klG);
-    h->hklU = h->upload(klU);
-    long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
-    h->hchunk = std::min<long long>(maxn, HESS_CHUNK);
-    h->hnwg = HESS_NWG;
-    h->hJ = h->alloc<double>((size_t)h->hchunk * P.npol * 2 * h->hC);
+    h->hnG = 2 * hd.C * (2 * hd.C + 1) / 2;
+    h->hnUu = hd.NU * (hd.NU + 1) / 2;
+    const long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
+    h->hchunk = hess_chunk(M, maxn);
+    const size_t es = M.mixed ? sizeof(float) : sizeof(double);
+    h->hJ = h->alloc<unsigned char>((size_t)h->hchunk * P.npol * 2 * h->hC * es);
     h->hU = h->alloc<double>((size_t)h->hchunk * h->hNU);
     h->hcs_ld = ctpwa_hess_rows_waves(h->hchunk);
     h->hcs = h->alloc<double>((size_t)(h->hNZ + h->hNU) * h->hcs_ld);
     h->hW = h->alloc<double>((size_t)h->hchunk * 2);
-    size_t maxcol = std::max<size_t>(2048, std::max<size_t>(h->hNU, h->hNZ));
-    h->hpart = h->alloc<double>((size_t)h->hnwg * maxcol);
     // totals: per class G, Uu, Z, g
     h->htot = h->alloc<double>((size_t)3 * (h->hnG + h->hnUu + h->hNZ + h->hNU));
     h->hHext = h->alloc<double>((size_t)h->hNU * h->hNU);
-    h->hmpart = h->alloc<double>((size_t)std::max<long long>(1, hess_mfma_part_doubles(hd)));
+    gram_setup(h, h->hgJ, 2 * h->hC);
+    gram_setup(h, h->hgU, h->hNU);
+    // fit parameter -> extended entries: [Re p_f] -> Re v_a, [Im p_f] -> Im v_a with
+    // v_a = r_a p_f(a); theta_j -> theta_j
+    {
+        const int nf = P.nfree, np = 2 * nf + P.ntheta;
+        std::vector<int> ptr(np + 1, 0), xs;
+        std::vector<double> rs;
+        for (int q = 0; q < np; ++q) {
+            if (q >= 2 * nf) {
+                xs.push_back(2 * P.namp + (q - 2 * nf));
+                rs.push_back(1.0);
+            } else {
+                const int f = q < nf ? q : q - nf, off = q < nf ? 0 : P.namp;
+                for (int a = 0; a < P.namp; ++a)
+                    if (P.amap[a] == f) {
+                        xs.push_back(off + a);
+                        rs.push_back(P.aratio[a]);
+                    }
+            }
+            ptr[q + 1] = (int)xs.size();
+        }
+        h->hmap_ptr = h->upload(ptr);
+        h->hmap_x = h->upload(xs);
+        h->hmap_r = h->upload(rs);
+    }
+    std::vector<int> perm;
+    h->hmod = compile_eval(h->device, generate_hess_kernel(M, &perm));
+    if ((int)perm.size() != h->hNZ + h->hNU) throw std::runtime_error("Hessian: summed-column count mismatch");
+    h->hperm = h->upload(perm);
+    HIPCHECK(hipModuleGetFunction(&h->fn_hrows, h->hmod, "ctpwa_hess_rows"));
     if (!M.customs.empty()) {
         h->mod2 = compile_eval(h->device, generate_custom2_kernel(M));
         HIPCHECK(hipModuleGetFunction(&h->fn_custom2, h->mod2, "ctpwa_custom2"));
@@ -944,12 +992,8 @@ static void agram_setup(ctpwa_handle* h) {
     h->gchunk = std::max<long long>(1, std::min<long long>(maxn, 1 << 18));
     h->gX = h->alloc<double>((size_t)h->gchunk * P.npol * n2);
     h->gW = h->upload(std::vector<double>((size_t)h->gchunk, 1.0));
-    auto kl = upper_triangle(n2);
-    h->gnG = (int)kl.size();
-    h->gkl = h->upload(kl);
-    if (ctpwa_mfma_gram_ok(n2)) h->gpart = h->alloc<double>((size_t)ctpwa_mfma_gram_part_doubles(n2, HESS_MFMA_NWG));
-    else h->gpart = h->alloc<double>((size_t)HESS_NWG * 2048);
-    h->gtot = h->alloc<double>((size_t)h->gnG);
+    h->gtot = h->alloc<double>((size_t)n2 * (n2 + 1) / 2);
+    gram_setup(h, h->gg, n2);
     h->agram_ready = true;
 }
 
@@ -965,21 +1009,16 @@ extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, i
         const long long ldx = h->gchunk * P.npol;
         HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s));
         HIPCHECK(launch_custom(h, s));
-        HIPCHECK(ctpwa_launch_zero(h->gtot, h->gnG, s));
-        for (long long s0 = 0; s0 < h->n[cls]; s0 += h->gchunk) {
+        HIPCHECK(ctpwa_launch_zero(h->gtot, (long long)n2 * (n2 + 1) / 2, s));
+        const GramTabs& G = h->gg;
+        int k = 0;
+        for (long long s0 = 0; s0 < h->n[cls]; s0 += h->gchunk, ++k) {
             long long cnt = std::min(h->gchunk, h->n[cls] - s0);
             HIPCHECK(ctpwa_launch_amp_rows(&h->aux, h->off[cls] + s0, cnt, h->gX, ldx, s));
-            if (ctpwa_mfma_gram_ok(n2)) {
-                HIPCHECK(ctpwa_launch_wgram_mfma(h->gX, n2, ldx, cnt * P.npol, h->gW, cnt, h->gkl, h->gnG, h->gpart,
-                                                 HESS_MFMA_NWG, h->gtot, s));
-            } else {
-                for (int e0 = 0; e0 < h->gnG; e0 += 2048) {
-                    int ne = std::min(2048, h->gnG - e0);
-                    HIPCHECK(ctpwa_launch_wgram(h->gX, n2, ldx, cnt * P.npol, h->gW, cnt, h->gkl, e0, ne, h->gpart, HESS_NWG, s));
-                    HIPCHECK(ctpwa_launch_accum(h->gpart, HESS_NWG, ne, h->gtot + e0, s));
-                }
-            }
+            HIPCHECK(ctpwa_launch_gram_panels(h->gX, 0, n2, ldx, cnt * P.npol, h->gW, cnt, G.pp, G.npp, G.ntp, G.part,
+                                              G.nwg, k > 0, s));
         }
+        if (h->n[cls] > 0) HIPCHECK(ctpwa_launch_accum_panels(G.part, G.nwg, G.ntp, G.tiles, n2, h->gtot, s));
         HIPCHECK(ctpwa_launch_gram_to_complex(h->gtot, P.namp, out_dev, s));
         return 0;
     } catch (const std::exception& e) {
@@ -1043,60 +1082,51 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         }
         size_t per_class = (size_t)h->hnG + h->hnUu + h->hNZ + h->hNU;
         HIPCHECK(ctpwa_launch_zero(h->htot, (long long)(3 * per_class), s));
-        HessArgs a{};
-        a.st = h->store(); a.table = h->table;
-        a.off_v = (long long)M.off_v(); a.off_rec = (long long)M.off_rec();
-        a.amps = h->aux.amps; a.blocks = h->aux.blocks; a.dens = h->aux.dens; a.dterms = h->aux.dterms; a.terms = h->aux.terms;
-        a.namp = P.namp; a.nblock = (int)P.blocks.size(); a.npol = P.npol; a.ntheta = P.ntheta;
-        a.J = h->hJ; a.U = h->hU; a.cs = h->hcs; a.cs_ld = h->hcs_ld; a.W = h->hW; a.ld = h->hchunk;
-        a.C = h->hC; a.NU = h->hNU; a.NZ = h->hNZ;
-        a.cfb2 = h->cfb2; a.cld = h->ntot; a.ncf2 = M.customs.empty() ? 0 : M.ncf2(); a.ncv = M.ncv();
-        a.bcp = h->hbcp; a.bcl = h->hbcl;
         HessAsm as{};
+        const long long ldJ = h->hchunk * P.npol;
+        const int f32 = M.mixed;
         for (int c = 0; c < 3; ++c) {
             double* base = h->htot + c * per_class;
             as.G[c] = base;
             as.Uu[c] = base + h->hnG;
             as.Zs[c] = base + h->hnG + h->hnUu;
             as.gs[c] = base + h->hnG + h->hnUu + h->hNZ;
-            a.cls = c;
-            a.wts = (c > 0 && h->pw[c]) ? h->wts : nullptr;
-            a.wconst = h->wc[c];
-            for (long long s0 = 0; s0 < h->n[c]; s0 += h->hchunk) {
+            const double* wts = (c > 0 && h->pw[c]) ? h->wts : nullptr;
+            double wconst = h->wc[c];
+            int cls = c;
+            int k = 0;
+            for (long long s0 = 0; s0 < h->n[c]; s0 += h->hchunk, ++k) {
                 long long cnt = std::min(h->hchunk, h->n[c] - s0);
-                HIPCHECK(ctpwa_launch_hess_rows(&a, h->off[c] + s0, cnt, M.maxloc, s));
-                if (ctpwa_mfma_gram_ok(2 * h->hC)) {
-                    HIPCHECK(ctpwa_launch_wgram_mfma(h->hJ, 2 * h->hC, h->hchunk * P.npol, cnt * P.npol, h->hW, cnt,
-                                                     h->hklG, h->hnG, h->hmpart, HESS_MFMA_NWG, (double*)as.G[c], s));
-                } else {
-                    for (int e0 = 0; e0 < h->hnG; e0 += 2048) {
-                        int ne = std::min(2048, h->hnG - e0);
-                        HIPCHECK(ctpwa_launch_wgram(h->hJ, 2 * h->hC, h->hchunk * P.npol, cnt * P.npol, h->hW, cnt,
-                                                    h->hklG, e0, ne, h->hpart, h->hnwg, s));
-                        HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.G[c] + e0, s));
-                    }
-                }
-                if (c > 0 && ctpwa_mfma_gram_ok(h->hNU)) {
-                    HIPCHECK(ctpwa_launch_wgram_mfma(h->hU, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hklU,
-                                                     h->hnUu, h->hmpart, HESS_MFMA_NWG, (double*)as.Uu[c], s));
-                } else if (c > 0) {
-                    for (int e0 = 0; e0 < h->hnUu; e0 += 2048) {
-                        int ne = std::min(2048, h->hnUu - e0);
-                        HIPCHECK(ctpwa_launch_wgram(h->hU, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hklU,
-                                                    e0, ne, h->hpart, h->hnwg, s));
-                        HIPCHECK(ctpwa_launch_accum(h->hpart, h->hnwg, ne, (double*)as.Uu[c] + e0, s));
-                    }
-                }
+                long long e0 = h->off[c] + s0;
+                // per-event rows of the chunk (generated kernel)
+                void* args[] = {&h->table, (void*)&h->slp, &h->inv, &h->invf, &h->ntot, (void*)&wts, &wconst, &cls, &e0,
+                                &cnt, &h->hJ, (void*)&ldJ, &h->hchunk, &h->hU, &h->hW, &h->hcs, &h->hcs_ld, &h->cfb2,
+                                &h->ntot};
+                const unsigned grid = (unsigned)((cnt + 255) / 256);
+                HIPCHECK(hipModuleLaunchKernel(h->fn_hrows, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+                // weighted Grams on the matrix cores, summed over the class's chunks
+                HIPCHECK(ctpwa_launch_gram_panels(h->hJ, f32, 2 * h->hC, ldJ, cnt * P.npol, h->hW, cnt, h->hgJ.pp,
+                                                  h->hgJ.npp, h->hgJ.ntp, h->hgJ.part, h->hgJ.nwg, k > 0, s));
+                if (c > 0)
+                    HIPCHECK(ctpwa_launch_gram_panels(h->hU, 0, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hgU.pp,
+                                                      h->hgU.npp, h->hgU.ntp, h->hgU.part, h->hgU.nwg, k > 0, s));
                 // Zs[c] and gs[c] are adjacent: one pass adds both from the rows kernel's sums
-                HIPCHECK(ctpwa_launch_accum_cols(h->hcs, h->hcs_ld, cnt, h->hNZ + h->hNU, (double*)as.Zs[c], s));
+                HIPCHECK(ctpwa_launch_accum_cols(h->hcs, h->hcs_ld, cnt, h->hNZ + h->hNU, h->hperm, (double*)as.Zs[c], s));
+            }
+            if (h->n[c] > 0) {
+                HIPCHECK(ctpwa_launch_accum_panels(h->hgJ.part, h->hgJ.nwg, h->hgJ.ntp, h->hgJ.tiles, 2 * h->hC,
+                                                   (double*)as.G[c], s));
+                if (c > 0)
+                    HIPCHECK(ctpwa_launch_accum_panels(h->hgU.part, h->hgU.nwg, h->hgU.ntp, h->hgU.tiles, h->hNU,
+                                                       (double*)as.Uu[c], s));
             }
         }
-        as.klG = h->hklG; as.nG = h->hnG; as.klU = h->hklU; as.nUu = h->hnUu;
         as.C = h->hC; as.NU = h->hNU; as.NZ = h->hNZ;
         as.namp = P.namp; as.nfree = P.nfree; as.ntheta = P.ntheta; as.nblock = (int)P.blocks.size();
         as.amps = h->aux.amps; as.blocks = h->aux.blocks; as.table = h->table; as.off_v = (long long)M.off_v();
         as.Np = (double)h->n[0]; as.Wd = h->Wd; as.Wb = h->Wb;
         as.ngauss = (int)P.g_slot.size(); as.g_slot = h->fin.g_slot; as.g_sig = h->fin.g_sig;
+        as.map_ptr = h->hmap_ptr; as.map_x = h->hmap_x; as.map_r = h->hmap_r;
         as.Hext = h->hHext; as.out = hess_dev;
         HIPCHECK(ctpwa_launch_hess_assemble(&as, s));
         return 0;
@@ -1357,6 +1387,25 @@ extern "C" long long ctpwa_generate_source(const char* plan_text, int precision,
     }
 }
 
+// source of the generated Hessian rows kernel for a plan, without a device
+extern "C" long long ctpwa_generate_hessian_source(const char* plan_text, int precision, char* buf, long long cap) {
+    try {
+        if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
+        Model_ M = derive(parse_plan(plan_text));
+        M.mixed = precision;
+        std::string src = generate_hess_kernel(M, nullptr);
+        if (buf && cap > 0) {
+            long long n = std::min<long long>(cap - 1, (long long)src.size());
+            std::memcpy(buf, src.data(), (size_t)n);
+            buf[n] = '\0';
+        }
+        return (long long)src.size() + 1;
+    } catch (const std::exception& e) {
+        g_err = std::string("ctpwa_generate_hessian_source: ") + e.what();
+        return -1;
+    }
+}
+
 // compile the fused kernel for a plan with hipRTC (no device needed); returns the
 // code-object size in bytes, or -1 with the compiler log in ctpwa_last_error()
 extern "C" long long ctpwa_compile_check(const char* plan_text, int precision) {
@@ -1381,6 +1430,7 @@ extern "C" long long ctpwa_compile_check(const char* plan_text, int precision) {
             return (long long)cs;
         };
         long long cs = compile(generate_eval_kernel(M));
+        cs += compile(generate_hess_kernel(M, nullptr));
         // models with Custom lineshapes: also the Hessian's second-order factor module
         if (!M.customs.empty()) cs += compile(generate_custom2_kernel(M));
         return cs;

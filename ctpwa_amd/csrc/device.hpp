@@ -164,41 +164,6 @@ struct AuxArgs {
     const int* bcl;
 };
 
-// Hessian: per-event rows of one chunk of one class (csrc/hessian.hip)
-struct HessArgs {
-    // model
-    EvStore st;
-    const double* table;
-    long long off_v, off_rec;
-    const DAmp* amps;
-    const DBlock* blocks;
-    const DDen* dens;
-    const DDenTerms* dterms;
-    const DTerm* terms;
-    int namp, nblock, npol, ntheta;
-    // per-event weights of the class
-    const double* wts;       // per-event (global index) or null
-    double wconst;
-    int cls;
-    // outputs, column-major over the chunk's rows (a wave's store of one column is one
-    // contiguous run): row R = p * n_ev + r (polarisation p of chunk event r) for J
-    double* J;               // [2C][ld * npol]
-    double* U;               // [NU = 2 namp + ntheta][ld]
-    double* cs;              // per-wave column sums [NZ + NU][cs_ld]: the alpha-weighted
-    long long cs_ld;         // second-part scalars, then the u columns unweighted
-    double* W;               // [2][ld] alpha, beta
-    long long ld;            // row capacity of a chunk
-    int C, NU, NZ;
-    // Custom factors with first and second derivatives per stored event: factor c's rows
-    // start at c * ncf2 * cld (value, dr[ncv], di[ncv], hr[nh], hi[nh]); block b's
-    // factors are bcl[bcp[b] .. bcp[b+1])
-    const double* cfb2;
-    long long cld;
-    int ncf2, ncv;
-    const int* bcp;
-    const int* bcl;
-};
-
 // Hessian assembly from the per-class sums
 struct HessAsm {
     // per class c: G[c] (upper-triangle list over 2C), Uu[c] (upper triangle over NU),
@@ -207,10 +172,6 @@ struct HessAsm {
     const double* Uu[3];
     const double* Zs[3];
     const double* gs[3];
-    const int2* klG;
-    int nG;
-    const int2* klU;
-    int nUu;
     int C, NU, NZ;
     int namp, nfree, ntheta, nblock;
     const DAmp* amps;
@@ -221,6 +182,11 @@ struct HessAsm {
     int ngauss;
     const int* g_slot;
     const double* g_sig;
+    // fit parameter q = (Re p_f, Im p_f, theta) -> extended entries map_x[map_ptr[q] ..
+    // map_ptr[q + 1]) with factors map_r (v_a = r_a p_f(a))
+    const int* map_ptr;
+    const int* map_x;
+    const double* map_r;
     double* Hext;            // [NU][NU] scratch
     double* out;             // [np][np]
 };

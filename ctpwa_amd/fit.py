@@ -113,12 +113,19 @@ class Fitter:
         final = p.detach().clone()
         nll, g = self.loss_and_grad(final.clone().requires_grad_(True))
         state = opt.state[opt._params[0]]
-        converged = bool(state.get("n_iter", 0) < max_iter)
+        max_eval = opt.param_groups[0]["max_eval"]
+        # L-BFGS converged when it stopped on its own tolerances, not on its iteration or
+        # evaluation budget
+        converged = bool(state.get("n_iter", 0) < max_iter and state.get("func_evals", 0) < max_eval)
         steps = 0
         if polish:
-            p2, nll2, steps = self.polish(final)
-            if nll2 < float(nll.item()):
+            p2, nll2, steps, newton_exit = self.polish(final, with_exit=True)
+            moved = nll2 < float(nll.item())
+            if moved:
                 final, nll = p2, torch.tensor(nll2)
+            # after the polish: converged when it ended on the positive-definite Newton exit,
+            # or when L-BFGS had converged and the polish left the point unchanged
+            converged = bool(newton_exit or (converged and not moved))
         res = FitResult(params=final.cpu().numpy(), nll=float(nll.item()), n_evals=self.n_evals,
                         converged=converged, seconds=0.0, nll_history=hist, polish_steps=steps)
         if hessian:
@@ -126,23 +133,29 @@ class Fitter:
         res.seconds = time.perf_counter() - t0
         return res
 
-    def polish(self, p_init, max_steps: int = 40, tol: float = 1e-6, lam0: float = 1.0):
+    def polish(self, p_init, max_steps: int = 40, tol: float = 1e-6, lam0: float = 1.0, with_exit: bool = False):
         """Damped Newton on the exact Hessian (example/fit.py:308-385).  In the subspace
         without the two fixed parameters, solve (H + lam I) d = -g with lam at least
         -lambda_min(H) + 1e-6, so that d descends where H is not positive definite; the
         step is clamped into the box; an accepted step (NLL lower by more than tol) scales
         lam by 0.3 (not below 1e-6), a rejected one by 10, and lam above 1e10 ends it.  Once
         H is positive definite, one undamped Newton step (kept if it lowers the NLL) ends
-        it.  Returns (params, nll, steps)."""
+        it.  Parameters with an all-zero Hessian row (the likelihood does not depend on
+        them, as errors() treats them) stay out of the subspace and unchanged.  Returns
+        (params, nll, steps), and with ``with_exit`` also whether the positive-definite
+        Newton exit was reached."""
         dev = self.ana.device
         n = self.n
-        mask = torch.ones(self.np, dtype=torch.bool, device=dev)
-        mask[0] = False
-        mask[n] = False
-        nf = int(mask.sum().item())
         best = torch.as_tensor(np.asarray(p_init.cpu() if torch.is_tensor(p_init) else p_init,
                                           dtype=np.float64)).to(dev).clone()
         self._clamp_(best)
+        mask = torch.ones(self.np, dtype=torch.bool, device=dev)
+        mask[0] = False
+        mask[n] = False
+        H_full = self.ana.getHessian(best)
+        mask &= ~torch.all(H_full == 0, dim=1)
+        nf = int(mask.sum().item())
+        newton_exit = False
 
         def nll_at(q):
             self.n_evals += 1
@@ -153,7 +166,10 @@ class Fitter:
         steps = 0
         for step in range(max_steps):
             steps = step + 1
-            H = self.ana.getHessian(best)[mask][:, mask]
+            if H_full is None:
+                H_full = self.ana.getHessian(best)
+            H = H_full[mask][:, mask]
+            H_full = None
             eig_min = float(torch.linalg.eigvalsh(H)[0].item())
             q = best.clone().requires_grad_(True)
             self.n_evals += 1
@@ -166,6 +182,7 @@ class Fitter:
                 nc = nll_at(cand)
                 if nc < nll_best - tol:
                     best, nll_best = cand, nc
+                newton_exit = True
                 break
             lam = max(lam, -eig_min + 1e-6)
             Hd = H + lam * torch.eye(nf, dtype=torch.float64, device=dev)
@@ -184,6 +201,8 @@ class Fitter:
                 lam *= 10.0
                 if lam > 1e10:
                     break
+        if with_exit:
+            return best.detach(), nll_best, steps, newton_exit
         return best.detach(), nll_best, steps
 
     def errors(self, res: FitResult) -> FitResult:

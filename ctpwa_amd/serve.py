@@ -498,12 +498,19 @@ class FitJob:
 
 @dataclass
 class FitRecord:
+    """``status``: "ok" (the minimiser ran to its end), "rejected" (not admitted: its HBM
+    footprint exceeds the service's budget, or its starting point is not finite) or
+    "failed" (its events could not be loaded, its handle could not be built, or its
+    likelihood or Hessian failed or was not finite at its starting point); ``error`` says
+    why.  A rejected or failed fit leaves every other fit's result unchanged."""
     job_id: int
     outcome: Outcome
     n_events: int
     admitted: float
     finished: float
     arrival: float = 0.0
+    status: str = "ok"
+    error: str = ""
 
 
 @dataclass
@@ -517,25 +524,31 @@ class ServiceStats:
     admissions_seconds: float = 0.0
 
     def summary(self) -> Dict[str, float]:
+        """Service figures over the fits that ran ("ok"); rejected and failed fits are
+        counted apart."""
         lat = np.array(self.latencies) if self.latencies else np.zeros(1)
+        ok = [r for r in self.fits if r.status == "ok"]
+        fit_lat = [r.finished - r.arrival for r in ok]
         return {
-            "fits": len(self.fits),
+            "fits": len(ok),
+            "rejected": int(sum(r.status == "rejected" for r in self.fits)),
+            "failed": int(sum(r.status == "failed" for r in self.fits)),
             "evaluations": int(len(self.latencies)),
             "wall_s": self.wall,
-            "fits_per_hour": len(self.fits) / self.wall * 3600.0 if self.wall > 0 else 0.0,
+            "fits_per_hour": len(ok) / self.wall * 3600.0 if self.wall > 0 else 0.0,
             "evals_per_s": len(self.latencies) / self.wall if self.wall > 0 else 0.0,
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3),
             "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_batch": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0,
             "rounds": len(self.batch_sizes),
             "admission_s": self.admissions_seconds,
-            "converged": int(sum(r.outcome.converged for r in self.fits)),
-            "hessians": int(sum(r.outcome.n_hess for r in self.fits)),
-            "evals_per_fit_median": float(np.median([r.outcome.n_evals for r in self.fits])) if self.fits else 0.0,
+            "converged": int(sum(r.outcome.converged for r in ok)),
+            "hessians": int(sum(r.outcome.n_hess for r in ok)),
+            "evals_per_fit_median": float(np.median([r.outcome.n_evals for r in ok])) if ok else 0.0,
             "hessian_round_p50_ms": float(np.percentile(self.hessian_seconds, 50) * 1e3) if self.hessian_seconds else 0.0,
             # arrival -> result of whole fits (admission queueing included)
-            "fit_latency_p50_s": float(np.percentile([r.finished - r.arrival for r in self.fits], 50)) if self.fits else 0.0,
-            "fit_latency_p99_s": float(np.percentile([r.finished - r.arrival for r in self.fits], 99)) if self.fits else 0.0,
+            "fit_latency_p50_s": float(np.percentile(fit_lat, 50)) if ok else 0.0,
+            "fit_latency_p99_s": float(np.percentile(fit_lat, 99)) if ok else 0.0,
         }
 
 
@@ -590,6 +603,8 @@ class FitService:
         t0 = self.clock()
 
         def admit():
+            # a job that cannot be admitted is reported and skipped; only an error of the
+            # service itself (not of one job) ends the run
             try:
                 for job in order:
                     while not mem["stop"]:
@@ -597,12 +612,24 @@ class FitService:
                         if wait <= 0:
                             break
                         time.sleep(min(wait, 0.005))
-                    smp = job.samples() if callable(job.samples) else job.samples
-                    ph, da, bk = smp.get("phsp"), smp.get("data"), smp.get("bkg")
-                    n3 = (len(ph), len(da), 0 if bk is None else len(bk))
-                    need = self.footprint(n3)
+                    if mem["stop"]:
+                        return
+                    n3 = (0, 0, 0)
+                    try:
+                        if not np.all(np.isfinite(np.asarray(job.x0, dtype=np.float64))):
+                            ready.put(("rejected", job, n3, "the starting point is not finite"))
+                            continue
+                        smp = job.samples() if callable(job.samples) else job.samples
+                        ph, da, bk = smp.get("phsp"), smp.get("data"), smp.get("bkg")
+                        n3 = (len(ph), len(da), 0 if bk is None else len(bk))
+                        need = self.footprint(n3)
+                    except Exception as exc:
+                        ready.put(("failed", job, n3, "events: %s" % exc))
+                        continue
                     if need > self.hbm_budget:
-                        raise RuntimeError("fit %d needs %d bytes, more than the HBM budget" % (job.job_id, need))
+                        ready.put(("rejected", job, n3, "needs %d bytes of HBM, more than the service's budget of %d"
+                                   % (need, self.hbm_budget)))
+                        continue
                     with cond:
                         while mem["used"] + need > self.hbm_budget and not mem["stop"]:
                             cond.wait(0.05)
@@ -610,14 +637,28 @@ class FitService:
                             return
                         mem["used"] += need
                     ta = self.clock()
-                    h = _native.Handle(self.plan_text, ph, da, bk, smp.get("phsp_weights"), smp.get("data_weights"),
-                                       smp.get("bkg_weights"), device=self.device, precision=self.precision)
+                    try:
+                        h = _native.Handle(self.plan_text, ph, da, bk, smp.get("phsp_weights"),
+                                           smp.get("data_weights"), smp.get("bkg_weights"), device=self.device,
+                                           precision=self.precision)
+                    except Exception as exc:
+                        with cond:
+                            mem["used"] -= need
+                            cond.notify_all()
+                        ready.put(("failed", job, n3, "handle: %s" % exc))
+                        continue
                     st.admissions_seconds += self.clock() - ta
-                    ready.put((job, h, need, sum(n3), self.clock() - t0))
+                    ready.put(("ok", job, h, need, sum(n3), self.clock() - t0))
             except BaseException as exc:          # surfaces in the main loop
                 ready.put(exc)
                 return
             ready.put(None)
+
+        def refuse(job, n3, status, why):
+            t = self.clock() - t0
+            x0 = np.asarray(job.x0, dtype=np.float64)
+            st.fits.append(FitRecord(job.job_id, Outcome(x0, float("nan"), 0, 0, False), int(sum(n3)), t, t,
+                                     job.arrival, status, why))
 
         worker = threading.Thread(target=admit, daemon=True)
         worker.start()
@@ -639,7 +680,10 @@ class FitService:
                         break
                     if isinstance(item, BaseException):
                         raise item
-                    job, h, need, nev, t_adm = item
+                    if item[0] != "ok":
+                        refuse(item[1], item[2], item[0], item[3])
+                        continue
+                    _, job, h, need, nev, t_adm = item
                     if opt is None:
                         self.np = h.n_params
                         if self.method == "newton":
@@ -654,7 +698,8 @@ class FitService:
                         O_host = torch.zeros((self.max_slots, 1 + self.np), dtype=torch.float64).pin_memory()
                     slot = free_slots.pop()
                     opt.start(slot, job.x0, job.lower, job.upper, job.max_evals)
-                    active[slot] = dict(job=job, h=h, need=need, t_req=self.clock(), admitted=t_adm, n_events=nev)
+                    active[slot] = dict(job=job, h=h, need=need, t_req=self.clock(), admitted=t_adm, n_events=nev,
+                                        first=True)
                 if not active:
                     continue
                 # one round: every active fit's pending point in one batched evaluation
@@ -662,44 +707,75 @@ class FitService:
                 slots = sorted(active)
                 Ph = P_host.numpy()
                 Ph[slots] = opt.xt[slots]
+                failed: Dict[int, str] = {}
                 with torch.cuda.stream(stream):
                     P.copy_(P_host, non_blocking=True)
-                    self.batch.nll_grad([active[s]["h"] for s in slots], [P[s].data_ptr() for s in slots],
-                                        [O[s].data_ptr() for s in slots], stream.cuda_stream)
+                    try:
+                        self.batch.nll_grad([active[s]["h"] for s in slots], [P[s].data_ptr() for s in slots],
+                                            [O[s].data_ptr() for s in slots], stream.cuda_stream)
+                    except RuntimeError:
+                        # find the fits that fail: each evaluated alone (a fit's solo result
+                        # is bit-identical to its batched one)
+                        for s in slots:
+                            try:
+                                active[s]["h"].nll_grad(P[s].data_ptr(), O[s].data_ptr(), stream.cuda_stream)
+                            except RuntimeError as exc:
+                                failed[s] = "likelihood: %s" % exc
                     O_host.copy_(O, non_blocking=True)
                 stream.synchronize()
                 tdone = self.clock()
                 st.round_seconds.append(tdone - tr)
                 st.batch_sizes.append(len(slots))
                 res = O_host.numpy()
+                for s in slots:
+                    # the reference reports a non-finite loss as 1e30 (src/Analysis.cu:1145-1158);
+                    # at a fit's starting point that leaves nothing to minimise
+                    if active[s]["first"] and s not in failed and not (abs(res[s, 0]) < 1e30):
+                        failed[s] = "the likelihood is not finite at the starting point"
+                    active[s]["first"] = False
+                ok = [s for s in slots if s not in failed]
+                done: Dict[int, Outcome] = {}
                 if self.method == "newton":
-                    done, need_h = opt.step(slots, res[slots, 0], res[slots, 1:])
+                    done, need_h = opt.step(ok, res[ok, 0], res[ok, 1:]) if ok else ({}, [])
                     if need_h:
                         # exact Hessians where fits moved (P[s] holds the point just evaluated)
                         th = self.clock()
                         with torch.cuda.stream(stream):
                             for s in need_h:
-                                active[s]["h"].hessian(P[s].data_ptr(), Hd[s].data_ptr(), stream.cuda_stream)
+                                try:
+                                    active[s]["h"].hessian(P[s].data_ptr(), Hd[s].data_ptr(), stream.cuda_stream)
+                                except RuntimeError as exc:
+                                    failed[s] = "Hessian: %s" % exc
                             H_host.copy_(Hd, non_blocking=True)
                         stream.synchronize()
                         st.hessian_seconds.append(self.clock() - th)
-                        done.update(opt.set_hessians(need_h, H_host.numpy()[need_h]))
-                else:
-                    done = opt.step(slots, res[slots, 0], res[slots, 1:])
+                        need_h = [s for s in need_h if s not in failed]
+                        if need_h:
+                            done.update(opt.set_hessians(need_h, H_host.numpy()[need_h]))
+                elif ok:
+                    done = opt.step(ok, res[ok, 0], res[ok, 1:])
                 tnext = self.clock()
-                for s in slots:
+                for s in ok:
                     a = active[s]
                     st.latencies.append(tdone - a["t_req"])
                     a["t_req"] = tnext
-                for s, outcome in done.items():
+
+                def retire(s, outcome, status="ok", why=""):
                     a = active.pop(s)
                     st.fits.append(FitRecord(a["job"].job_id, outcome, a["n_events"], a["admitted"], tnext - t0,
-                                             a["job"].arrival))
+                                             a["job"].arrival, status, why))
                     a["h"].close()
                     with cond:
                         mem["used"] -= a["need"]
                         cond.notify_all()
                     free_slots.append(s)
+
+                for s, outcome in done.items():
+                    if s not in failed:
+                        retire(s, outcome)
+                for s, why in failed.items():
+                    x = np.asarray(active[s]["job"].x0, dtype=np.float64)
+                    retire(s, Outcome(x, float("nan"), 0, 0, False), "failed", why)
         finally:
             with cond:
                 mem["stop"] = True

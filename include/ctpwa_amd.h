@@ -154,9 +154,15 @@ const char* ctpwa_kernel_source(const ctpwa_handle* h);
  * (NUL-terminated) into buf and returns the full size including the NUL, -1 on error */
 long long ctpwa_generate_source(const char* plan_text, int precision, char* buf, long long cap);
 
-/* compile the fused kernel for a plan with hipRTC (no device needed), and for a plan with
- * Custom lineshapes also the Hessian's second-order factor kernel; returns the total
- * code-object size, or -1 with the compiler log in ctpwa_last_error() */
+/* source of the generated Hessian rows kernel for a plan (the per-event part of
+ * analysis::getHessian, src/Analysis.cu:2689-3378), without a device; as
+ * ctpwa_generate_source */
+long long ctpwa_generate_hessian_source(const char* plan_text, int precision, char* buf, long long cap);
+
+/* compile the fused kernel and the Hessian rows kernel for a plan with hipRTC (no device
+ * needed), and for a plan with Custom lineshapes also the Hessian's second-order factor
+ * kernel; returns the total code-object size, or -1 with the compiler log in
+ * ctpwa_last_error() */
 long long ctpwa_compile_check(const char* plan_text, int precision);
 
 const char* ctpwa_last_error(void);

@@ -131,3 +131,23 @@ def test_multistart_checkpoint_and_warm_start(tmp_path):
     # run 0 from the saved couplings: one L-BFGS iteration and the polish reach the minimum
     w = multistart(ana, 1, seed=9, warm_start=cp, max_iter=1, hessian=False)
     assert w[0].nll <= rs[0].nll + 1e-6
+
+
+def test_converged_flag_and_dead_rows_in_polish():
+    """converged is False when L-BFGS stops on its iteration budget and no polish runs; the
+    polish leaves the all-zero Hessian row (a parameter the NLL does not depend on) out of
+    its subspace, so it takes the positive-definite Newton exit at once instead of damping
+    until lam passes 1e10, and the fit then counts as converged."""
+    n, P, mu, A, lo, hi = _problem(5)
+    ana = QuadraticAnalysis(n, P, mu, A, lo, hi)
+    start = mu + 0.4
+    start[-1] = mu[-1] + 0.1
+    raw = Fitter(ana).minimize(start, max_iter=1, hessian=False, polish=False)
+    assert not raw.converged
+    f = Fitter(ana)
+    p2, nll2, steps, newton_exit = f.polish(start.copy(), with_exit=True)
+    assert newton_exit and steps == 1
+    assert float(p2[-1]) == start[-1]           # the dead coordinate is left unchanged
+    assert np.allclose(p2.numpy()[:-1], mu[:-1], atol=1e-9)
+    r = Fitter(ana).minimize(start, max_iter=1, hessian=False)
+    assert r.converged and r.nll < 1e-9

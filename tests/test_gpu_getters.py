@@ -83,3 +83,29 @@ def test_amplitude_getters_with_identical_particles(tmp_path, oracle_lib):
     t = ana.getDataTensor().cpu()
     assert t.numel() == da.shape[0] * plan.npol * base.n_amp
     assert (t - _flat(ref)).abs().max() <= 1e-10 * _flat(ref).abs().max()
+
+
+def test_theta_of_last_evaluation_survives_other_calls(tmp_path, oracle_lib):
+    """getNLL(A), then getIntensity at B: the stored amplitudes, and getFitFractions of a
+    complex vector, stay at A's theta (the theta of the last getNLL / getHessian)."""
+    import ctpwa_amd as ctpwa
+    ph, da, bk = samples(6000, 1500, 0, 35)
+    cfg = _write_cfg(tmp_path, "jpsi_3pi", ph, da, None)
+    print("case theta of the last evaluation", flush=True)
+    ana = ctpwa.analysis(cfg, samples={"phsp_truth": ph})
+    plan = ana.plan
+    O = oracle_lib.Oracle(plan.to_text(), 3, ph, da)
+    p0 = W.nominal_params(plan, 1)
+    pa, pb = perturbed(p0, plan, 5, 0.1), perturbed(p0, plan, 6, 0.1)
+    ana.getNLL(torch.tensor(pa, device="cuda"))
+    ana.getIntensity(torch.tensor(pb, device="cuda"), 1)
+    ref = _flat(O.amplitudes(pa, 1))
+    assert (ana.getDataTensor().cpu() - ref).abs().max() <= 1e-10 * ref.abs().max()
+    nf = plan.n_free
+    v = torch.complex(torch.tensor(pa[:nf]), torch.tensor(pa[nf:2 * nf])).to("cuda")
+    ff_v = ana.getFitFractions(v).cpu()
+    ff_a = ana.getFitFractions(torch.tensor(pa, device="cuda")).cpu()
+    assert torch.allclose(ff_v, ff_a, rtol=1e-12, atol=1e-14)
+    ana.getHessian(torch.tensor(pb, device="cuda"))
+    ref = _flat(O.amplitudes(pb, 1))
+    assert (ana.getDataTensor().cpu() - ref).abs().max() <= 1e-10 * ref.abs().max()
