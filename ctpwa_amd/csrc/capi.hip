@@ -468,6 +468,7 @@ static bool constant_weights(const double* w, long long n, double* c) {
     return true;
 }
 
+
 extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long n_phsp, const double* w_phsp,
                             const double* data, long long n_data, const double* w_data,
                             const double* bkg, long long n_bkg, const double* w_bkg, int precision, int device,
@@ -850,6 +851,7 @@ static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s) {
     return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, h->M.wgs, 1, 1, 0, s, args, nullptr);
 }
 
+
 struct DeviceGuard {
     int prev = 0;
     explicit DeviceGuard(int dev) { (void)hipGetDevice(&prev); (void)hipSetDevice(dev); }
@@ -871,6 +873,7 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     if (e != hipSuccess) return fail(std::string("ctpwa_nll_grad: ") + hipGetErrorString(e));
     return 0;
 }
+
 
 
 // Hessian workspace: Jacobian columns C, u-vector length NU, second-part scalars NZ per event
@@ -898,7 +901,8 @@ static long long hess_chunk(const Model_& M, long long maxn) {
     const HessDims d = hess_dims(M);
     const long long row = (long long)M.P.npol * 2 * d.C * 8;
     long long c = std::max<long long>(256, hess_jbytes() / std::max<long long>(1, row) / 256 * 256);
-    return std::max<long long>(1, std::min(maxn, c));
+    // a multiple of 4 events: the Gram's 16-byte slab loads stay aligned
+    return (std::max<long long>(1, std::min(maxn, c)) + 3) / 4 * 4;
 }
 static long long hess_workspace_bytes(const Model_& M, long long maxn, long long ntot) {
     HessDims d = hess_dims(M);
@@ -933,6 +937,7 @@ static void hess_setup(ctpwa_handle* h) {
     h->hNZ = hd.NZ;
     h->hnG = 2 * hd.C * (2 * hd.C + 1) / 2;
     h->hnUu = hd.NU * (hd.NU + 1) / 2;
+    if (P.ntheta > 64) throw std::runtime_error("Hessian: more than 64 free resonance parameters is not built");
     const long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
     h->hchunk = hess_chunk(M, maxn);
     const size_t es = M.mixed ? sizeof(float) : sizeof(double);
@@ -989,7 +994,7 @@ static void agram_setup(ctpwa_handle* h) {
     const Plan& P = h->M.P;
     const int n2 = 2 * P.namp;
     long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
-    h->gchunk = std::max<long long>(1, std::min<long long>(maxn, 1 << 18));
+    h->gchunk = (std::max<long long>(1, std::min<long long>(maxn, 1 << 18)) + 3) / 4 * 4;
     h->gX = h->alloc<double>((size_t)h->gchunk * P.npol * n2);
     h->gW = h->upload(std::vector<double>((size_t)h->gchunk, 1.0));
     h->gtot = h->alloc<double>((size_t)n2 * (n2 + 1) / 2);

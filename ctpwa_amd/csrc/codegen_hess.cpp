@@ -110,21 +110,40 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
          "  const int wv = threadIdx.x >> 6;\n"
          "  WaveColSums cs_{slab[wv], cs + (long long)blockIdx.x * 4 + wv, cs_ld, (int)(threadIdx.x & 63), 0, 0};\n"
          "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n";
-    // Every phase reads its per-event inputs and coefficients through its own copies of
-    // the event index and the table pointer, passed through an empty asm statement: the
-    // compiler then reloads them where a phase needs them instead of keeping the first
-    // phase's values live in registers through the second.
+    // Every per-event input is loaded once, at the start, before any store: on this
+    // target a wait for a load issued after stores also waits for those stores, so loads
+    // placed among the Jacobian stores would serialise the kernel on store completion.
+    // Each phase reads the wave-uniform coefficients at its own offset of the table
+    // pointer, a zero passed through an empty asm statement: the compiler then reloads
+    // them (scalar loads) where the second phase needs them instead of keeping the first
+    // phase's values live in registers.
     auto phase_vars = [&](int ph) {
-        o << "  long long e_" << ph << " = e0 + r;\n  asm volatile(\"\" : \"+v\"(e_" << ph << "));\n"
-          << "  const double* tb_" << ph << " = tab;\n  asm volatile(\"\" : \"+s\"(tb_" << ph << "));\n";
+        o << "  long long z_" << ph << " = 0;\n  asm volatile(\"\" : \"+s\"(z_" << ph << "));\n"
+          << "  const double* __restrict__ tb_" << ph << " = tab + z_" << ph << ";\n";
     };
-    // expressions, for phase ph
-    auto xv = [&](int ph, int i) {
-        std::ostringstream q;
-        if (imap[i] >= 0) q << "inv[" << imap[i] << "LL * ld + e_" << ph << "]";
-        else q << "(double)invf[" << (-imap[i] - 1) << "LL * ld + e_" << ph << "]";
-        return q.str();
-    };
+    std::vector<char> xused(M.inv.size(), 0);
+    for (const DenSpec& Dn : M.dens) {
+        for (const Term& t : Dn.re) if (t.inv >= 0) xused[t.inv] = 1;
+        for (const Term& t : Dn.im) if (t.inv >= 0) xused[t.inv] = 1;
+    }
+    o << "  const long long e = e0 + r;\n"
+         "  const double wload = (cls != 0 && wts) ? wts[e] : wconst;\n";
+    for (size_t i = 0; i < M.inv.size(); ++i) {
+        if (!xused[i]) continue;
+        if (imap[i] >= 0) o << "  const double x" << i << " = inv[" << imap[i] << "LL * ld + e];\n";
+        else o << "  const double x" << i << " = (double)invf[" << (-imap[i] - 1) << "LL * ld + e];\n";
+    }
+    for (size_t u = 0; u < M.us.size(); ++u)
+        for (int p = 0; p < NP; ++p) {
+            std::string nm = std::to_string(u) + "_" + std::to_string(p);
+            o << "  const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+            o << "  const double s" << nm << "x = (double)g" << nm << ".x, s" << nm << "y = (double)g" << nm << ".y;\n";
+        }
+    // Custom factor rows (value, first and second derivatives) of the event
+    for (size_t c = 0; c < M.customs.size(); ++c)
+        for (int comp = 0; comp < (M.customs.empty() ? 0 : M.ncf2()); ++comp)
+            o << "  const double cf" << c << "_" << comp << " = cfb2[" << (long long)c * M.ncf2() + comp << "LL * cld + e];\n";
+    auto xv = [&](int, int i) { return "x" + std::to_string(i); };
     auto rec = [&](int ph, long long rr, int comp) {
         std::ostringstream q;
         q << "tb_" << ph << "[OFFR + " << rr << "LL * REC + " << comp << "]";
@@ -133,11 +152,8 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
     const int ncf2 = M.customs.empty() ? 0 : M.ncf2();
     const int ncv = M.ncv();
     const int nh = ncv * (ncv + 1) / 2;
-    auto cf = [&](int ph, int c, int comp) {
-        std::ostringstream q;
-        q << "cfb2[" << (long long)c * ncf2 + comp << "LL * cld + e_" << ph << "]";
-        return q.str();
-    };
+    auto cf = [&](int, int c, int comp) { return "cf" + std::to_string(c) + "_" + std::to_string(comp); };
+    (void)ncf2;
     // the propagators of block b (psi_d = pr + i pi) and the block product (br + i bi)
     auto emit_block_value = [&](int ph, int b, const char* ind) {
         const std::vector<int>& ds = M.block_dens[b];
@@ -163,11 +179,7 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
             o << ind << "{ const double tr = br * pr" << d << " - bi * pi" << d << "; bi = br * pi" << d << " + bi * pr" << d
               << "; br = tr; }\n";
     };
-    auto sload = [&](int ph, int u, int p, const char* ind) {
-        std::string nm = std::to_string(u) + "_" + std::to_string(p);
-        o << ind << "const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e_" << ph << "];\n";
-        o << ind << "const double s" << nm << "x = (double)g" << nm << ".x, s" << nm << "y = (double)g" << nm << ".y;\n";
-    };
+    auto sload = [&](int, int, int, const char*) {};
 
     // ---- phase 1: block products, amplitude columns, S, I, weights -------------------
     phase_vars(1);
@@ -202,7 +214,7 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
     }
     o << "  double I = 0.0;\n";
     for (int p = 0; p < NP; ++p) o << "  I += S" << p << "r * S" << p << "r + S" << p << "i * S" << p << "i;\n";
-    o << "  const double we = (cls == 0) ? 1.0 : (wts ? wts[e0 + r] : wconst);\n"
+    o << "  const double we = (cls == 0) ? 1.0 : wload;\n"
          "  double alpha, beta;\n"
          "  if (cls == 0) { alpha = 1.0; beta = 0.0; }\n"
          "  else if (I == 0.0) { alpha = 0.0; beta = 0.0; }\n"

@@ -47,7 +47,7 @@ namespace ctpwa {
 // Lane layouts (16x16x4, one block): A[i][k] in lane 16 k + i, B[k][j] in lane 16 k + j;
 // D[4 r + q][j] (f64) or D[4 q + r][j] (f32) in component r of lane 16 q + j.
 constexpr int GP_WAVES = 4;
-constexpr int GP_MAXPAIR = 16;
+constexpr int GP_PAIRS = 16;                 // tile pairs of two different panels
 template <typename T> struct GP;
 template <> struct GP<double> {
     static constexpr int R = 16;
@@ -61,60 +61,59 @@ template <> struct GP<float> {
 };
 typedef double gp_acc64 __attribute__((ext_vector_type(4)));
 
-// LDS bytes of k_gram_panels: per wave a slab of 128 columns, or the 16 combined tiles
-inline size_t gram_panels_lds(bool f32) {
+// LDS bytes of k_gram_panels<T, NT, FULL>: per wave a slab of the pair's columns, or the
+// combined tile pairs
+inline size_t gram_panels_lds(bool f32, int NT, bool full) {
     const int R = f32 ? 32 : 16;
     const size_t es = f32 ? 4 : 8;
-    size_t slabs = (size_t)GP_WAVES * 128 * (R + 1) * es;
-    size_t comb = (size_t)GP_MAXPAIR * 256 * sizeof(double);
+    const int cols = full ? 128 : NT * 16;
+    const int npair = full ? NT * NT : NT * (NT + 1) / 2;
+    size_t slabs = (size_t)GP_WAVES * cols * (R + 1) * es;
+    size_t comb = (size_t)npair * 256 * sizeof(double);
     return std::max(slabs, comb);
 }
 
-template <typename T>
+// NT: tiles per panel; FULL: all NT x NT tile pairs of the panel pair (a Gram of several
+// panels: a diagonal pair then also computes its tiles below the diagonal, which the
+// accumulation skips), else the NT (NT + 1) / 2 pairs I <= J (a Gram of one panel).  With
+// the pair count fixed at compile time every pair's product is unconditional.
+template <typename T, int NT, bool FULL>
 __global__ void __launch_bounds__(64 * GP_WAVES)
 k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows, const double* __restrict__ Wt,
               long long nev, const int4* __restrict__ pp, int ntp, double* __restrict__ part, int accumulate) {
-    constexpr int R = GP<T>::R, ST = R + 1, LPC = 64 / R, NIT = 128 / LPC;
+    constexpr int R = GP<T>::R, ST = R + 1;
+    constexpr int NPAIR = FULL ? NT * NT : NT * (NT + 1) / 2;
     typedef typename GP<T>::acc_t acc_t;
     extern __shared__ __attribute__((aligned(16))) unsigned char gp_smem[];
     __shared__ double wsh[GP_WAVES][R];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int4 pq = pp[blockIdx.y];                 // panels P, Q; first tile pair; tile pairs
-    const int P = pq.x, Q = pq.y, tp0 = pq.z, npair = pq.w;
+    const int P = pq.x, Q = pq.y, tp0 = pq.z;
     const bool diag = P == Q;
     const int wP = min(64, ncol - 64 * P), wQ = min(64, ncol - 64 * Q);
-    const int TP = (wP + 15) >> 4, TQ = (wQ + 15) >> 4;
     const int ncl = diag ? 64 : 128;
-    T* xs = (T*)gp_smem + (size_t)wave * 128 * ST;
+    constexpr int NCOLS = FULL ? 128 : NT * 16;
+    T* xs = (T*)gp_smem + (size_t)wave * NCOLS * ST;
     double* ws = wsh[wave];
-    // tile pairs of the panel pair, in the order of the global list: LDS offsets of their
-    // A (panel P) and B (panel Q: region 64 unless diagonal) columns for this lane
-    int aoff[GP_MAXPAIR], boff[GP_MAXPAIR];
-    {
-        int q = 0;
-        for (int I = 0; I < 4; ++I)
-            for (int J = 0; J < 4; ++J) {
-                const bool use = I < TP && J < TQ && (!diag || I <= J);
-                if (use && q < GP_MAXPAIR) {
-                    aoff[q] = (I * 16 + (lane & 15)) * ST;
-                    boff[q] = ((diag ? 0 : 64) + J * 16 + (lane & 15)) * ST;
-                }
-                if (use) ++q;
-            }
-    }
+    const int boff0 = diag ? 0 : 64;
     // this wave's rows: the grid's waves split [0, nrows) into ranges of whole slabs
     const long long nw = (long long)gridDim.x * GP_WAVES;
     const long long gw = (long long)blockIdx.x * GP_WAVES + wave;
     const long long per = ((nrows + nw - 1) / nw + R - 1) / R * R;
     const long long r0 = gw * per < nrows ? gw * per : nrows;
     const long long r1 = r0 + per < nrows ? r0 + per : nrows;
-    gp_acc64 acc[GP_MAXPAIR];
+    gp_acc64 acc[NPAIR];
 #pragma unroll
-    for (int q = 0; q < GP_MAXPAIR; ++q) acc[q] = (gp_acc64){0.0, 0.0, 0.0, 0.0};
-    // lane l carries row l mod R of columns LPC * it + l / R
-    const int rr = lane % R, ch = lane / R;
-    T pre[NIT];
+    for (int q = 0; q < NPAIR; ++q) acc[q] = (gp_acc64){0.0, 0.0, 0.0, 0.0};
+    // 16-byte loads: lane l carries rows VW (l mod LPC) .. + VW - 1 of column CPI it + l / LPC
+    // (the chunk's leading dimension is a multiple of 4, so every such run is aligned)
+    constexpr int VW = 16 / (int)sizeof(T), LPC_ = R / VW, CPI = 64 / LPC_;
+    constexpr int NLD = (FULL ? 128 : NT * 16) / CPI;
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    const int rr0 = (lane % LPC_) * VW, ch = lane / LPC_;
+    vec_t pre[NLD];
     double wpre = 0.0;
+    const int rw = lane % R;
     auto gcol = [&](int c) -> int {           // global column of LDS column c, or -1
         if (c < 64) return c < wP ? 64 * P + c : -1;
         return (c - 64) < wQ ? 64 * Q + (c - 64) : -1;
@@ -122,58 +121,74 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
     auto load = [&](long long t0) {
         const int nt = (int)((r1 - t0) < R ? (r1 - t0) : R);
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int c = LPC * it + ch;
+        for (int it = 0; it < NLD; ++it) {
+            const int c = CPI * it + ch;
             const int g = c < ncl ? gcol(c) : -1;
-            pre[it] = (g >= 0 && rr < nt) ? X[(long long)g * ldx + t0 + rr] : (T)0;
+            const T* src = X + (long long)(g < 0 ? 0 : g) * ldx + t0 + rr0;
+            if (g >= 0 && rr0 + VW <= nt) {
+                pre[it] = *(const vec_t*)src;
+            } else {
+#pragma unroll
+                for (int v = 0; v < VW; ++v) pre[it][v] = (g >= 0 && rr0 + v < nt) ? src[v] : (T)0;
+            }
         }
-        wpre = (ch == 0 && rr < nt) ? Wt[(t0 + rr) % nev] : 0.0;
+        wpre = (lane < R && rw < nt) ? Wt[(t0 + rw) % nev] : 0.0;
     };
     if (r0 < r1) load(r0);
     for (long long t0 = r0; t0 < r1; t0 += R) {
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) {
-            const int c = LPC * it + ch;
-            if (c < ncl) xs[c * ST + rr] = pre[it];
+        for (int it = 0; it < NLD; ++it) {
+            const int c = CPI * it + ch;
+            if (c < ncl) {
+#pragma unroll
+                for (int v = 0; v < VW; ++v) xs[c * ST + rr0 + v] = pre[it][v];
+            }
         }
-        if (ch == 0) ws[rr] = wpre;
+        if (lane < R) ws[rw] = wpre;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (t0 + R < r1) load(t0 + R);
+        const int li = lane & 15;
         if constexpr (sizeof(T) == 8) {
 #pragma unroll
             for (int k0 = 0; k0 < R; k0 += 4) {
                 const int kr = k0 + (lane >> 4);
                 const double w = ws[kr];
+                int q = 0;
 #pragma unroll
-                for (int q = 0; q < GP_MAXPAIR; ++q)
-                    if (q < npair)
-                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(w * (double)xs[aoff[q] + kr], (double)xs[boff[q] + kr],
-                                                                      acc[q], 0, 0, 0);
+                for (int I = 0; I < NT; ++I) {
+                    const double av = w * (double)xs[(I * 16 + li) * ST + kr];
+#pragma unroll
+                    for (int J = FULL ? 0 : I; J < NT; ++J, ++q)
+                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, (double)xs[(boff0 + J * 16 + li) * ST + kr], acc[q],
+                                                                      0, 0, 0);
+                }
             }
         } else {
-            acc_t t[GP_MAXPAIR];
+            acc_t t[NPAIR];
 #pragma unroll
-            for (int q = 0; q < GP_MAXPAIR; ++q) t[q] = (acc_t){0.0f, 0.0f, 0.0f, 0.0f};
+            for (int q = 0; q < NPAIR; ++q) t[q] = (acc_t){0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int k0 = 0; k0 < R; k0 += 4) {
                 const int kr = k0 + (lane >> 4);
                 const double w = ws[kr];
+                int q = 0;
 #pragma unroll
-                for (int q = 0; q < GP_MAXPAIR; ++q)
-                    if (q < npair)
-                        t[q] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)(w * (double)xs[aoff[q] + kr]), (float)xs[boff[q] + kr],
-                                                                    t[q], 0, 0, 0);
+                for (int I = 0; I < NT; ++I) {
+                    const float av = (float)(w * (double)xs[(I * 16 + li) * ST + kr]);
+#pragma unroll
+                    for (int J = FULL ? 0 : I; J < NT; ++J, ++q)
+                        t[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, (float)xs[(boff0 + J * 16 + li) * ST + kr], t[q], 0, 0, 0);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < GP_MAXPAIR; ++q)
-                if (q < npair) {
-                    acc[q][0] += (double)t[q][0];
-                    acc[q][1] += (double)t[q][1];
-                    acc[q][2] += (double)t[q][2];
-                    acc[q][3] += (double)t[q][3];
-                }
+            for (int q = 0; q < NPAIR; ++q) {
+                acc[q][0] += (double)t[q][0];
+                acc[q][1] += (double)t[q][1];
+                acc[q][2] += (double)t[q][2];
+                acc[q][3] += (double)t[q][3];
+            }
         }
         // the slab is consumed before the next one overwrites it
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -187,19 +202,18 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
     for (int w = 0; w < GP_WAVES; ++w) {
         if (wave == w) {
 #pragma unroll
-            for (int q = 0; q < GP_MAXPAIR; ++q)
-                if (q < npair) {
+            for (int q = 0; q < NPAIR; ++q) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int e = q * 256 + GP<T>::row(lane, r) * 16 + (lane & 15);
-                        comb[e] = (w == 0) ? acc[q][r] : comb[e] + acc[q][r];
-                    }
+                for (int r = 0; r < 4; ++r) {
+                    const int e = q * 256 + GP<T>::row(lane, r) * 16 + (lane & 15);
+                    comb[e] = (w == 0) ? acc[q][r] : comb[e] + acc[q][r];
                 }
+            }
         }
         __syncthreads();
     }
     double* o = part + ((size_t)blockIdx.x * ntp + tp0) * 256;
-    for (int e = threadIdx.x; e < npair * 256; e += 64 * GP_WAVES) o[e] = accumulate ? o[e] + comb[e] : comb[e];
+    for (int e = threadIdx.x; e < NPAIR * 256; e += 64 * GP_WAVES) o[e] = accumulate ? o[e] + comb[e] : comb[e];
 }
 
 // tot[idx(a, b)] += the sum over the nwg workgroup partials of k_gram_panels, for every
@@ -331,43 +345,74 @@ __global__ void __launch_bounds__(256) k_hess_ext(HessAsm h) {
     h.Hext[q] = H;
 }
 
-__global__ void k_hess_second(HessAsm h) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int n = h.namp, NU = h.NU;
+// z offset of block b in the class sums: the blocks before it with waves
+__device__ inline int hess_zoff(const HessAsm& h, int b) {
+    int z = 0;
+    for (int q = 0; q < b; ++q) {
+        const DBlock& B = h.blocks[q];
+        if (B.nsl > 0) z += B.nsl * 2 * B.nloc + B.nloc * (B.nloc + 1) / 2;
+    }
+    return z;
+}
+
+// second-part terms: every (amplitude, local parameter) entry of the (v, theta) blocks
+// belongs to one thread (the classes added in a fixed order); the (theta, theta) terms
+// of all blocks, which may share parameters, and the Gaussian constraints are gathered
+// in LDS by one thread in block order and then added in parallel
+constexpr int HS_MAXTH = 64;
+__global__ void __launch_bounds__(256) k_hess_second(HessAsm h) {
+    __shared__ double tt[HS_MAXTH * HS_MAXTH];
+    const int n = h.namp, NU = h.NU, P = h.ntheta;
     double cp, cq;
     hess_pf(h, cp, cq);
-    for (int c = 0; c < 3; ++c) {
-        double s = (c == 0) ? cp : ((c == 1) ? -1.0 : 1.0);
-        const double* Z = h.Zs[c];
-        int zq = 0;
-        for (int b = 0; b < h.nblock; ++b) {
-            const DBlock& B = h.blocks[b];
-            if (B.nsl == 0) continue;
-            for (int cc = 0; cc < B.nsl; ++cc) {
-                int am = B.site + cc;
-                for (int i = 0; i < B.nloc; ++i) {
-                    double qr = Z[zq++], qi = Z[zq++];
-                    int j = 2 * n + B.vars[i];
-                    // d2S/dRe v_a dtheta = dA_a/dtheta: 2 Re Q ; d2S/dIm v_a dtheta = i dA: -2 Im Q
-                    double re = 2.0 * s * qr, im = -2.0 * s * qi;
-                    h.Hext[(long long)am * NU + j] += re;
-                    h.Hext[(long long)j * NU + am] += re;
-                    h.Hext[(long long)(n + am) * NU + j] += im;
-                    h.Hext[(long long)j * NU + n + am] += im;
-                }
+    const double sc[3] = {cp, -1.0, 1.0};
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        const int b = h.amps[a].block;
+        const DBlock& B = h.blocks[b];
+        if (B.nsl == 0) continue;
+        const int z0 = hess_zoff(h, b) + (a - B.site) * 2 * B.nloc;
+        for (int i = 0; i < B.nloc; ++i) {
+            double re = 0.0, im = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                // d2S/dRe v_a dtheta = dA_a/dtheta: 2 Re Q ; d2S/dIm v_a dtheta = i dA: -2 Im Q
+                re += 2.0 * sc[c] * h.Zs[c][z0 + 2 * i];
+                im += -2.0 * sc[c] * h.Zs[c][z0 + 2 * i + 1];
             }
-            for (int i = 0; i < B.nloc; ++i)
-                for (int k = i; k < B.nloc; ++k) {
-                    double rv = 2.0 * s * Z[zq++];
-                    int j1 = 2 * n + B.vars[i], j2 = 2 * n + B.vars[k];
-                    h.Hext[(long long)j1 * NU + j2] += rv;
-                    if (j1 != j2) h.Hext[(long long)j2 * NU + j1] += rv;
-                }
+            const int j = 2 * n + B.vars[i];
+            h.Hext[(long long)a * NU + j] += re;
+            h.Hext[(long long)j * NU + a] += re;
+            h.Hext[(long long)(n + a) * NU + j] += im;
+            h.Hext[(long long)j * NU + n + a] += im;
         }
     }
-    for (int g = 0; g < h.ngauss; ++g) {
-        int j = 2 * n + h.g_slot[g];
-        h.Hext[(long long)j * NU + j] += 1.0 / (h.g_sig[g] * h.g_sig[g]);
+    if (P > HS_MAXTH) return;     // rejected on the host
+    for (int q = threadIdx.x; q < P * P; q += blockDim.x) tt[q] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < 3; ++c) {
+            int zq = 0;
+            for (int b = 0; b < h.nblock; ++b) {
+                const DBlock& B = h.blocks[b];
+                if (B.nsl == 0) continue;
+                zq += B.nsl * 2 * B.nloc;
+                for (int i = 0; i < B.nloc; ++i)
+                    for (int k = i; k < B.nloc; ++k) {
+                        const double rv = 2.0 * sc[c] * h.Zs[c][zq++];
+                        const int j1 = B.vars[i], j2 = B.vars[k];
+                        tt[j1 * P + j2] += rv;
+                        if (j1 != j2) tt[j2 * P + j1] += rv;
+                    }
+            }
+        }
+        for (int g = 0; g < h.ngauss; ++g) {
+            const int j = h.g_slot[g];
+            tt[j * P + j] += 1.0 / (h.g_sig[g] * h.g_sig[g]);
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < P * P; q += blockDim.x) {
+        const int j1 = q / P, j2 = q - j1 * P;
+        h.Hext[(long long)(2 * n + j1) * NU + 2 * n + j2] += tt[q];
     }
 }
 
@@ -404,17 +449,20 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_zero(do
 // matrix-core Gram of ncol columns; returns the number of tile pairs
 extern "C" __attribute__((visibility("hidden"))) int ctpwa_gram_tables(int ncol, int4* pp, int* npp, int2* tiles) {
     const int npan = (ncol + 63) / 64;
+    const int T1 = (ncol + 15) / 16;        // tiles of a one-panel Gram
     int q = 0, y = 0;
     for (int P = 0; P < npan; ++P)
         for (int Q = P; Q < npan; ++Q) {
-            const int TP = (std::min(64, ncol - 64 * P) + 15) / 16, TQ = (std::min(64, ncol - 64 * Q) + 15) / 16;
             const int q0 = q;
+            // one panel: the pairs I <= J of its tiles; several: all 4 x 4 pairs of a panel
+            // pair (tiles past the last column give entries the accumulation skips)
             for (int I = 0; I < 4; ++I)
-                for (int J = 0; J < 4; ++J)
-                    if (I < TP && J < TQ && (P != Q || I <= J)) {
-                        if (tiles) tiles[q] = make_int2(4 * P + I, 4 * Q + J);
-                        ++q;
-                    }
+                for (int J = 0; J < 4; ++J) {
+                    const bool use = (npan == 1) ? (I < T1 && J < T1 && I <= J) : true;
+                    if (!use) continue;
+                    if (tiles) tiles[q] = make_int2(4 * P + I, 4 * Q + J);
+                    ++q;
+                }
             if (pp) pp[y] = make_int4(P, Q, q0, q - q0);
             ++y;
         }
@@ -436,13 +484,29 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_gram_pa
                                                                                      double* part, int nwg, int accumulate,
                                                                                      hipStream_t s) {
     if (ncol <= 0 || nrows <= 0) return hipSuccess;
-    const size_t sh = gram_panels_lds(f32 != 0);
-    if (f32)
-        hipLaunchKernelGGL(k_gram_panels<float>, dim3(nwg, npp), dim3(64 * GP_WAVES), sh, s, (const float*)X, ncol, ldx,
-                           nrows, Wt, nev, pp, ntp, part, accumulate);
-    else
-        hipLaunchKernelGGL(k_gram_panels<double>, dim3(nwg, npp), dim3(64 * GP_WAVES), sh, s, (const double*)X, ncol, ldx,
-                           nrows, Wt, nev, pp, ntp, part, accumulate);
+    const dim3 g(nwg, npp), b(64 * GP_WAVES);
+#define GP_LAUNCH(TT, NTT, FF)                                                                                   \
+    hipLaunchKernelGGL((k_gram_panels<TT, NTT, FF>), g, b, gram_panels_lds(f32 != 0, NTT, FF), s, (const TT*)X, ncol, \
+                       ldx, nrows, Wt, nev, pp, ntp, part, accumulate)
+    const int T1 = (ncol + 15) / 16;
+    if (ncol > 64) {
+        if (f32) GP_LAUNCH(float, 4, true); else GP_LAUNCH(double, 4, true);
+    } else if (f32) {
+        switch (T1) {
+        case 1: GP_LAUNCH(float, 1, false); break;
+        case 2: GP_LAUNCH(float, 2, false); break;
+        case 3: GP_LAUNCH(float, 3, false); break;
+        default: GP_LAUNCH(float, 4, false); break;
+        }
+    } else {
+        switch (T1) {
+        case 1: GP_LAUNCH(double, 1, false); break;
+        case 2: GP_LAUNCH(double, 2, false); break;
+        case 3: GP_LAUNCH(double, 3, false); break;
+        default: GP_LAUNCH(double, 4, false); break;
+        }
+    }
+#undef GP_LAUNCH
     return hipGetLastError();
 }
 
@@ -482,7 +546,8 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_gram_to
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_hess_assemble(const HessAsm* h, hipStream_t s) {
     const long long ne = (long long)h->NU * h->NU;
     hipLaunchKernelGGL(k_hess_ext, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, *h);
-    hipLaunchKernelGGL(k_hess_second, dim3(1), dim3(64), 0, s, *h);
+    if (h->ntheta > HS_MAXTH) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_hess_second, dim3(1), dim3(256), 0, s, *h);
     const int np = 2 * h->nfree + h->ntheta;
     const long long nm = (long long)np * np;
     hipLaunchKernelGGL(k_hess_map, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, s, *h);
