@@ -24,6 +24,7 @@ EXPORTS = [
     "ctpwa_compile_check", "ctpwa_last_error", "ctpwa_version",
     "ctpwa_batch_create", "ctpwa_batch_destroy", "ctpwa_batch_nll_grad",
     "ctpwa_amplitude_gram", "ctpwa_partial_intensities", "ctpwa_partial_amplitudes", "ctpwa_last_theta",
+    "ctpwa_nll_grad_noalloc",
 ]
 
 _lib = None
@@ -47,7 +48,7 @@ def lib():
     L.ctpwa_destroy.restype = None
     L.ctpwa_dims.argtypes = [_vp] + [ctypes.POINTER(ctypes.c_int)] * 4
     L.ctpwa_param_info.argtypes = [_vp, _dp, _dp, _dp]
-    for f in ("ctpwa_nll_grad", "ctpwa_hessian"):
+    for f in ("ctpwa_nll_grad", "ctpwa_hessian", "ctpwa_nll_grad_noalloc"):
         getattr(L, f).argtypes = [_vp, _vp, _vp, _vp]
     for f in ("ctpwa_intensity", "ctpwa_amplitudes", "ctpwa_amplitude_gram", "ctpwa_partial_intensities",
               "ctpwa_partial_amplitudes"):
@@ -202,6 +203,10 @@ class Handle:
     def nll_grad(self, params_ptr: int, out_ptr: int, stream: int):
         check(lib().ctpwa_nll_grad(self.h, ctypes.c_void_p(params_ptr), ctypes.c_void_p(out_ptr),
                                    ctypes.c_void_p(stream)), "ctpwa_nll_grad")
+
+    def nll_grad_noalloc(self, params_ptr: int, out_ptr: int, stream: int):
+        check(lib().ctpwa_nll_grad_noalloc(self.h, ctypes.c_void_p(params_ptr), ctypes.c_void_p(out_ptr),
+                                           ctypes.c_void_p(stream)), "ctpwa_nll_grad_noalloc")
 
     def hessian(self, params_ptr: int, out_ptr: int, stream: int):
         check(lib().ctpwa_hessian(self.h, ctypes.c_void_p(params_ptr), ctypes.c_void_p(out_ptr),

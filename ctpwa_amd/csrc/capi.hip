@@ -468,6 +468,38 @@ static bool constant_weights(const double* w, long long n, double* c) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// ctpwa_nll_grad_noalloc: the per-workgroup partial sums and the class totals of an
+// evaluation live in `scratch`, a file-scope device array of NOALLOC_DOUBLES doubles
+// (2 MiB) that every GPU the library runs on holds until the process ends, instead of
+// the handle's buffers.  Host code never passes the array's name to a kernel (it is a
+// host-side stand-in there): each device's own address comes from hipGetSymbolAddress,
+// taken once per device with the device current (at ctpwa_create) and kept in g_scratch.
+// Every evaluation overwrites what it reads (each workgroup writes its partial row, the
+// reduction writes the totals), so nothing needs clearing between calls; calls that
+// overlap on one GPU share the array and race.
+static const size_t NOALLOC_DOUBLES = (size_t)1 << 18;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch;
+
+// the device's copy of `scratch` (call with the device current)
+static double* scratch_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) return it->second;
+    double* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = ptr;
+    return ptr;
+}
+static double* scratch_lookup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    return it == g_scratch.end() ? nullptr : it->second;
+}
 
 extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long n_phsp, const double* w_phsp,
                             const double* data, long long n_data, const double* w_data,
@@ -486,6 +518,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw std::runtime_error("no HIP device available");
         if (device < 0 || device >= ndev) throw std::runtime_error("device index out of range");
         HIPCHECK(hipSetDevice(device));
+        scratch_setup(device);     // this device's address of the no-alloc entry point's scratch
         h = new ctpwa_handle();
         h->device = device;
         if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
@@ -845,8 +878,9 @@ static hipError_t launch_custom(ctpwa_handle* h, hipStream_t s) {
     return hipModuleLaunchKernel(h->fn_custom, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr);
 }
 
-static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s) {
-    void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->wts, &h->part, &h->wg_cls, &h->wg_rng, &h->ntot,
+static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s, double* part = nullptr) {
+    double* pp = part ? part : h->part;
+    void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->wts, &pp, &h->wg_cls, &h->wg_rng, &h->ntot,
                     &h->nwg, &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2], &h->cfb};
     return hipModuleLaunchKernel(h->fn_eval, h->nwg, 1, 1, h->M.wgs, 1, 1, 0, s, args, nullptr);
 }
@@ -874,6 +908,30 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     return 0;
 }
 
+extern "C" int ctpwa_nll_grad_noalloc(ctpwa_handle* h, const double* params_dev, double* out_dev, void* stream) {
+    if (!h || !params_dev || !out_dev) return fail("ctpwa_nll_grad_noalloc: NULL argument");
+    const size_t need = (size_t)h->nacc * (size_t)std::max(1, h->nwg) + 3 * (size_t)h->nacc;
+    if (need > NOALLOC_DOUBLES)
+        return fail("ctpwa_nll_grad_noalloc: the evaluation needs " + std::to_string(need) +
+                    " doubles of scratch, more than the " + std::to_string(NOALLOC_DOUBLES) + " it holds");
+    DeviceGuard g(h->device);
+    double* scr = scratch_lookup(h->device);
+    if (!scr) return fail("ctpwa_nll_grad_noalloc: no scratch address for this device");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    if (e == hipSuccess) e = launch_custom(h, s);
+    if (e == hipSuccess) e = launch_eval(h, s, scr);
+    if (e == hipSuccess) {
+        FinArgs f = h->fin;
+        f.part = scr;
+        f.totals = scr + (size_t)h->nacc * (size_t)std::max(1, h->nwg);
+        f.params = params_dev;
+        f.out = out_dev;
+        e = ctpwa_launch_finalize(&f, s);
+    }
+    if (e != hipSuccess) return fail(std::string("ctpwa_nll_grad_noalloc: ") + hipGetErrorString(e));
+    return 0;
+}
 
 
 // Hessian workspace: Jacobian columns C, u-vector length NU, second-part scalars NZ per event

@@ -73,6 +73,17 @@ int ctpwa_param_info(const ctpwa_handle* h, double* init, double* lower, double*
  * [Re p, Im p, theta]; out_dev: device [1 + 2 n_free + n_theta] = [NLL, gradient]. */
 int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double* out_dev, void* stream);
 
+/* ctpwa_nll_grad that allocates nothing: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it).  The evaluation's per-workgroup partial sums and class
+ * totals live in a library-owned static device array of 2 MiB per GPU (held until the
+ * process ends) instead of the handle's buffers; the result is bit-identical to
+ * ctpwa_nll_grad.  A model whose partials exceed the array (n_acc x n_workgroups +
+ * 3 n_acc > 262144 doubles, n_acc = 1 + 2 n_amp + theta-gradient terms, at most 1024
+ * workgroups) is rejected with an error.  All calls on one GPU share the
+ * array: calls that overlap on one GPU race. */
+int ctpwa_nll_grad_noalloc(ctpwa_handle* h, const double* params_dev, double* out_dev, void* stream);
+
 /* Hessian in the same layout: device [(2 n_free + n_theta)^2], row-major. */
 int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, void* stream);
 
