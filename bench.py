@@ -189,6 +189,75 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
             "sync": sync}
 
 
+def hessian_flops(plan, n_data, n_phsp, n_bkg):
+    """Algorithmic flops of one Hessian (the reference's formula: per data and background
+    event the (2n+P)^2 outer product, per event of every class the complex n x n
+    amplitude Gram over the polarisations)."""
+    n, P = plan.n_free, plan.n_theta
+    m = 2 * n + P
+    return (n_data + n_bkg) * m * (m + 1) + 8 * plan.npol * (n_data + n_bkg + n_phsp) * n * (n + 1) // 2
+
+
+def run_hessian_wide(args, W, torch, precisions):
+    """getHessian on the two-family four-body cascade (149 amplitudes, 24 free resonance
+    parameters, 322 x 322): 10M phase-space events and 2M "data" events, both uniform in
+    phase space (the Hessian's cost does not depend on how the data are distributed, and
+    accept-reject on this model's narrow phi takes minutes), seed 4; one first call
+    (which compiles the generated Hessian kernel), then the mean of --hessian-wide-reps
+    calls, per form."""
+    from ctpwa_amd import analysis
+    name = "jpsi_kkpipi_wide"
+    spec = W.WORKLOADS[name]
+    t0 = time.time()
+    smp = {"phsp": W.phase_space(spec["n_phsp"], seed=1004, masses=spec["masses"], workers=8),
+           "data": W.phase_space(spec["n_data"], seed=3004, masses=spec["masses"], workers=8)}
+    t_gen = time.time() - t0
+    log("[bench] %s samples generated in %.1f s" % (name, t_gen))
+    d = tempfile.mkdtemp(prefix="ctpwa_wide_")
+    cfg = os.path.join(d, "config.yml")
+    with open(cfg, "w") as f:
+        f.write(spec["config"])
+    res = {"workload": "%s: J/psi -> K+ K- pi+ pi-, K1(1270), K1(1400), K1(1650) -> K*(892) pi and K rho, "
+                       "K* K*bar and K* K*2(1430)bar over three K* states, phi or f2'(1525) with pi pi "
+                       "resonances; %%d amplitudes, %%d free resonance "
+                       "parameters; %d data (uniform in phase space) / %d phase space, seed 4" % (
+                           name, spec["n_data"], spec["n_phsp"]),
+           "call": "analysis.getHessian, mean of %d after one first call" % args.hessian_wide_reps,
+           "sample_generation_s": round(t_gen, 1), "ms": {}, "first_call_s": {}}
+    plan = None
+    for prec in precisions:
+        t = time.time()
+        ana = analysis(cfg, samples={"phsp": smp["phsp"], "data": smp["data"]}, precision=prec)
+        torch.cuda.synchronize()
+        t_init = time.time() - t
+        plan = ana.plan
+        p = torch.tensor(W.nominal_params(plan, 4), dtype=torch.float64, device="cuda")
+        log("[bench] %s %s: handle ready in %.1f s, first getHessian" % (name, prec, t_init))
+        t = time.time()
+        ana.getHessian(p)
+        torch.cuda.synchronize()
+        res["first_call_s"][prec] = round(time.time() - t, 2)
+        t = time.perf_counter()
+        for _ in range(args.hessian_wide_reps):
+            H = ana.getHessian(p)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.hessian_wide_reps * 1e3
+        res["ms"][prec] = round(ms, 2)
+        sym = float((H - H.T).abs().max() / H.abs().max())
+        log("[bench] %s %s: init %.1f s, first getHessian %.1f s, getHessian %.2f ms, |H - H^T|/|H| %.2e"
+            % (name, prec, t_init, res["first_call_s"][prec], ms, sym))
+        del ana, H
+        torch.cuda.empty_cache()
+    res["workload"] = res["workload"] % (plan.n_amp, plan.n_theta)
+    fl = hessian_flops(plan, spec["n_data"], spec["n_phsp"], 0)
+    res["algorithmic_flops"] = fl
+    res["mfma_fp64_peak_tflops"] = 78.6
+    res["frac_of_fp64_mfma_peak"] = {k: round(fl / (v * 1e-3) / 78.6e12, 4) for k, v in res["ms"].items()}
+    import shutil
+    shutil.rmtree(d, ignore_errors=True)
+    return res
+
+
 def run_serving(args, samples, plan, spec, W, precisions):
     """Toy-MC ensemble through the fit service: data counts log-uniform in [5k, 500k],
     phase space 5x data, Poisson arrivals; every fit is a bounded Newton minimisation with
@@ -241,6 +310,9 @@ def main():
     ap.add_argument("--serving-slots", type=int, default=64, help="fits evaluated together at most")
     ap.add_argument("--hessian-reps", type=int, default=10,
                     help="getHessian calls timed per form on the same workload (0: skip)")
+    ap.add_argument("--hessian-wide-reps", type=int, default=3,
+                    help="getHessian calls timed per form on the 149-amplitude four-body cascade at "
+                         "2M data / 10M phase space (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
                          "known size that calibrates the HBM byte counters of a --pmc profile)")
@@ -295,6 +367,11 @@ def main():
     if args.serving_jobs > 0:
         serving = run_serving(args, samples, plan, spec, W, precisions)
 
+    wide = None
+    if args.hessian_wide_reps > 0:
+        del samples
+        wide = run_hessian_wide(args, W, torch, precisions)
+
     if args.copy_probe_mb > 0:
         nb = args.copy_probe_mb << 20
         src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
@@ -333,9 +410,14 @@ def main():
     if compare is not None:
         out["fp64"] = compare
     if args.hessian_reps > 0:
+        fl = hessian_flops(plan, n_data, n_phsp, n_bkg)
+        hms = {p: round(results[p]["hessian_ms"], 4) for p in precisions}
         out["hessian"] = {"call": "analysis.getHessian on the same workload, mean of %d after one warm-up"
                                   % args.hessian_reps,
-                          "ms": {p: round(results[p]["hessian_ms"], 4) for p in precisions}}
+                          "ms": hms, "algorithmic_flops": fl, "mfma_fp64_peak_tflops": 78.6,
+                          "frac_of_fp64_mfma_peak": {k: round(fl / (v * 1e-3) / 78.6e12, 4) for k, v in hms.items()}}
+    if wide is not None:
+        out["hessian_wide"] = wide
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out), flush=True)

@@ -89,3 +89,15 @@ def test_package_exposes_analysis_class():
     assert ctpwa.analysis.__name__ == "analysis"
     for m in ("getNLL", "getHessian", "getNVector", "getNFreeTheta"):
         assert callable(getattr(ctpwa.analysis, m))
+
+
+def test_wide_cascade_plan_generates():
+    """The 149-amplitude four-body cascade passes the native layout checks (node tensors
+    of at most 64 entries, thresholds) and yields both generated kernels' sources (its
+    hipRTC compile takes tens of seconds, so the GPU tests cover that)."""
+    from ctpwa_amd import _native
+    plan = plan_of("jpsi_kkpipi_wide")
+    assert (plan.n_amp, plan.n_theta) == (149, 24)
+    src = _native.generate_source(plan.to_text(), "fp64")
+    hsrc = _native.generate_hessian_source(plan.to_text(), "mixed")
+    assert "ctpwa_eval" in src and "ctpwa_hess_rows" in hsrc and "float* __restrict__ J" in hsrc
