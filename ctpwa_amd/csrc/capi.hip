@@ -31,6 +31,8 @@ extern "C" hipError_t ctpwa_launch_init(const double*, long long, int, const ctp
                                          hipStream_t);
 extern "C" hipError_t ctpwa_launch_to_mixed(const double2*, const double*, long long, int, int, const int*, float2*,
                                              double*, float*, long long, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_polsym(const double2*, long long, long long, long long, int, unsigned long long*,
+                                          hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep_batch(const double* const*, double* const*, const ctpwa::PrepArgs*, int, int,
@@ -314,7 +316,8 @@ struct ctpwa_handle {
     float* invf = nullptr;
     int* d_imap = nullptr;
     const void* slp = nullptr;      // the SL store the fused kernel reads
-    EvStore store() const { return EvStore{sl, slf, inv, invf, d_imap, ntot, mixed}; }
+    int* d_psign = nullptr;         // [NU] signs of the mirrored polarisation (folded store)
+    EvStore store() const { return EvStore{sl, slf, inv, invf, d_imap, ntot, mixed, M.npstore(), d_psign}; }
 
     template <class T>
     T* alloc(size_t count) {
@@ -501,6 +504,38 @@ static double* scratch_lookup(int device) {
     return it == g_scratch.end() ? nullptr : it->second;
 }
 
+// After the SL tensors are built: when both polarisations of every SL tensor are mirror
+// images over all events (polarisation 1 = sign_u conj(polarisation 0) to 1e-12 of the
+// tensor's largest component; spin-0 final states of a J/psi with M = +1, -1), keep only
+// polarisation 0 ([NU][1][ld]) and the signs: the kernels then read half the SL bytes.
+static void fold_polarisations(ctpwa_handle* h, const unsigned long long* d_sym, int nu, hipStream_t s) {
+    std::vector<unsigned long long> raw((size_t)4 * nu);
+    HIPCHECK(hipMemcpy(raw.data(), d_sym, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    std::vector<int> sign((size_t)nu, 1);
+    for (int u = 0; u < nu; ++u) {
+        double dp, dm, m0;
+        std::memcpy(&dp, &raw[4 * u + 0], 8);
+        std::memcpy(&dm, &raw[4 * u + 1], 8);
+        std::memcpy(&m0, &raw[4 * u + 2], 8);
+        if (!(std::min(dp, dm) <= 1e-12 * m0) && !(dp == 0.0 || dm == 0.0)) return;   // not mirror images
+        sign[u] = (dp <= dm) ? 1 : -1;
+    }
+    const size_t es = h->mixed ? sizeof(float2) : sizeof(double2);
+    void* old = h->mixed ? (void*)h->slf : (void*)h->sl;
+    void* neu = h->alloc<unsigned char>((size_t)nu * h->ntot * es);
+    for (int u = 0; u < nu; ++u)
+        HIPCHECK(hipMemcpyAsync((char*)neu + (size_t)u * h->ntot * es, (const char*)old + (size_t)2 * u * h->ntot * es,
+                                (size_t)h->ntot * es, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipFree(old));
+    h->allocs.erase(std::remove(h->allocs.begin(), h->allocs.end(), old), h->allocs.end());
+    if (h->mixed) h->slf = (float2*)neu; else h->sl = (double2*)neu;
+    h->slp = neu;
+    h->M.nps = 1;
+    h->M.psign = sign;
+    h->d_psign = h->upload(sign);
+}
+
 extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long n_phsp, const double* w_phsp,
                             const double* data, long long n_data, const double* w_data,
                             const double* bkg, long long n_bkg, const double* w_bkg, int precision, int device,
@@ -605,6 +640,12 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             const size_t ev_bytes = (size_t)P.nfinal * 4 * sizeof(double);
             const long long chunk = std::max<long long>(1, (long long)(STAGE_BYTES / ev_bytes));
             if (h->mixed) S->reserve_mixed((size_t)std::max(1, nu) * P.npol * chunk, (size_t)std::max(1, ninv) * chunk);
+            // two polarisations: check over every event whether they are mirror images
+            unsigned long long* d_sym = nullptr;
+            if (P.npol == 2 && nu > 0) {
+                d_sym = h->alloc<unsigned long long>((size_t)4 * nu);
+                HIPCHECK(hipMemsetAsync(d_sym, 0, (size_t)4 * nu * sizeof(unsigned long long), S->s));
+            }
             int b = 0;
             for (int c = 0; c < 3; ++c) {
                 for (long long s0 = 0; s0 < ns[c]; s0 += chunk) {
@@ -617,17 +658,20 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
                         HIPCHECK(ctpwa_launch_init(S->dev[b], cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
                                                    (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv, d_tabs,
                                                    S->sl64, S->inv64, cnt, 0, S->s));
+                        if (d_sym) HIPCHECK(ctpwa_launch_polsym(S->sl64, cnt, 0, cnt, nu, d_sym, S->s));
                         HIPCHECK(ctpwa_launch_to_mixed(S->sl64, S->inv64, cnt, nu * P.npol, ninv, h->d_imap, h->slf,
                                                        h->inv, h->invf, h->ntot, h->off[c] + s0, S->s));
                     } else {
                         HIPCHECK(ctpwa_launch_init(S->dev[b], cnt, P.nfinal, d_topos, (int)topos.size(), d_nodes, d_kins,
                                                    (int)kins.size(), d_us, nu, d_cg, d_pol, P.npol, d_invs, ninv, d_tabs,
                                                    h->sl, h->inv, h->ntot, h->off[c] + s0, S->s));
+                        if (d_sym) HIPCHECK(ctpwa_launch_polsym(h->sl, h->ntot, h->off[c] + s0, cnt, nu, d_sym, S->s));
                     }
                     b ^= 1;
                 }
             }
             HIPCHECK(hipStreamSynchronize(S->s));
+            if (d_sym) fold_polarisations(h, d_sym, nu, S->s);
         }
 
         // model tables for prep / finalize / aux
@@ -1375,7 +1419,7 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
 // per-event bytes of the SL tensors and the invariants a model stores at a precision
 static long long event_bytes(const Model_& M, const std::vector<int>* used) {
     std::vector<int> imap = M.imap();
-    long long b = (long long)M.us.size() * M.P.npol * (M.mixed ? 8 : 16);
+    long long b = (long long)M.us.size() * M.npstore() * (M.mixed ? 8 : 16);
     for (size_t i = 0; i < M.inv.size(); ++i)
         if (!used || (*used)[i]) b += (imap[i] >= 0) ? 8 : 4;
     b += (long long)M.customs.size() * M.ncf() * 8;      // Custom factor rows

@@ -28,6 +28,8 @@
 
 namespace ctpwa {
 
+static std::string D(double x);
+static std::string DH_(double x) { return x < 0 ? "(-1.0)" : "1.0"; }
 static std::string D(double x) {
     std::ostringstream s;
     s.precision(17);
@@ -593,6 +595,17 @@ std::string generate_eval_kernel(const Model_& M) {
     auto sl_conv = [&](const std::string& x) {
         return M.mixed ? "make_double2((double)" + x + ".x, (double)" + x + ".y)" : x;
     };
+    // stored polarisations per SL tensor; a polarisation past them is the signed
+    // conjugate of its mirror image
+    const int NPS = M.npstore();
+    auto mirror = [&](int u, int p, const char* ind) {
+        std::ostringstream q;
+        const int m = NP - 1 - p;
+        const double g = (double)M.psign[u];
+        q << ind << "const double2 s" << u << "_" << p << " = make_double2(" << DH_(g) << " * s" << u << "_" << m << ".x, "
+          << DH_(-g) << " * s" << u << "_" << m << ".y);\n";
+        return q.str();
+    };
     o << "  const int cls = wg_cls[wg];\n"
          "  const long long e0 = wg_rng[2 * wg], e1 = wg_rng[2 * wg + 1];\n"
          "  const double wc = (cls == 1) ? w1 : w2;\n  const int pw = (cls == 1) ? pw1 : ((cls == 2) ? pw2 : 0);\n"
@@ -627,8 +640,12 @@ std::string generate_eval_kernel(const Model_& M) {
         for (int u = 0; u < NU; ++u)
             for (int p = 0; p < NP; ++p) {
                 std::string nm = std::to_string(u) + "_" + std::to_string(p);
+                if (p >= NPS) {
+                    rows.push_back(Row{std::string(), 0, tb, mirror(u, p, "    ")});
+                    continue;
+                }
                 std::ostringstream src;
-                src << "(const unsigned char*)sl + " << (long long)u * NP + p << "LL * ld * " << esl;
+                src << "(const unsigned char*)sl + " << (long long)u * NPS + p << "LL * ld * " << esl;
                 std::ostringstream rd;
                 rd << "    const " << SLT << " g" << nm << " = ((const " << SLT << "*)(B + " << tb << "))[threadIdx.x];\n"
                    << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
@@ -656,6 +673,7 @@ std::string generate_eval_kernel(const Model_& M) {
         o << "  auto issue = [&](unsigned char* dst, const long long ts) {\n"
              "    const int t = threadIdx.x;\n";
         for (const Row& r : rows) {
+            if (r.es == 0) continue;             // a mirrored polarisation: no copy
             int nl = WGS * r.es / 16;
             o << "    if (t < " << nl << " && ts + (long long)(t * " << 16 / r.es << ") < e1)\n"
               << "      __builtin_amdgcn_global_load_lds((const void*)(" << r.src << " + ts * " << r.es << " + t * 16),\n"
@@ -679,25 +697,30 @@ std::string generate_eval_kernel(const Model_& M) {
             // (registers), so their latency overlaps the current event's arithmetic
             o << "  long long e = e0 + threadIdx.x;\n  const long long ep = (e < e1) ? e : e0;\n";
             for (int u = 0; u < NU; ++u)
-                for (int p = 0; p < NP; ++p)
-                    o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + ep];\n";
+                for (int p = 0; p < NPS; ++p)
+                    o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + ep];\n";
             o << "  for (; e < e1; e += " << WGS << ") {\n";
             for (int u = 0; u < NU; ++u)
                 for (int p = 0; p < NP; ++p) {
                     std::string nm = std::to_string(u) + "_" + std::to_string(p);
-                    o << "    const double2 s" << nm << " = " << sl_conv("n" + nm) << ";\n";
+                    if (p >= NPS) o << mirror(u, p, "    ");
+                    else o << "    const double2 s" << nm << " = " << sl_conv("n" + nm) << ";\n";
                 }
             o << "    { const long long en = (e + " << WGS << " < e1) ? e + " << WGS << " : e;\n";
             for (int u = 0; u < NU; ++u)
-                for (int p = 0; p < NP; ++p)
-                    o << "      n" << u << "_" << p << " = sl[" << (long long)u * NP + p << "LL * ld + en];\n";
+                for (int p = 0; p < NPS; ++p)
+                    o << "      n" << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + en];\n";
             o << "    }\n";
         } else {
             o << "  for (long long e = e0 + threadIdx.x; e < e1; e += " << WGS << ") {\n";
             for (int u = 0; u < NU; ++u)
                 for (int p = 0; p < NP; ++p) {
                     std::string nm = std::to_string(u) + "_" + std::to_string(p);
-                    o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+                    if (p >= NPS) {
+                        o << mirror(u, p, "    ");
+                        continue;
+                    }
+                    o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NPS + p << "LL * ld + e];\n";
                     o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
                 }
         }
@@ -918,7 +941,11 @@ std::string generate_eval_kernel(const Model_& M) {
         for (int u = 0; u < NU; ++u)
             for (int p = 0; p < NP; ++p) {
                 std::string nm = std::to_string(u) + "_" + std::to_string(p);
-                o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+                if (p >= NPS) {
+                    o << mirror(u, p, "    ");
+                    continue;
+                }
+                o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NPS + p << "LL * ld + e];\n";
                 o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
             }
         for (size_t i = 0; i < M.inv.size(); ++i)

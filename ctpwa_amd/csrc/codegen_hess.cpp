@@ -133,10 +133,20 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
         if (imap[i] >= 0) o << "  const double x" << i << " = inv[" << imap[i] << "LL * ld + e];\n";
         else o << "  const double x" << i << " = (double)invf[" << (-imap[i] - 1) << "LL * ld + e];\n";
     }
+    // stored polarisations per SL tensor; a polarisation past them is the signed
+    // conjugate of its mirror image
+    const int NPS = M.npstore();
     for (size_t u = 0; u < M.us.size(); ++u)
         for (int p = 0; p < NP; ++p) {
             std::string nm = std::to_string(u) + "_" + std::to_string(p);
-            o << "  const " << SLT << " g" << nm << " = sl[" << (long long)u * NP + p << "LL * ld + e];\n";
+            if (p >= NPS) {
+                const std::string mm = std::to_string(u) + "_" + std::to_string(NP - 1 - p);
+                const bool neg = M.psign[u] < 0;
+                o << "  const double s" << nm << "x = " << (neg ? "-" : "") << "s" << mm << "x, s" << nm << "y = " << (neg ? "" : "-")
+                  << "s" << mm << "y;\n";
+                continue;
+            }
+            o << "  const " << SLT << " g" << nm << " = sl[" << (long long)u * NPS + p << "LL * ld + e];\n";
             o << "  const double s" << nm << "x = (double)g" << nm << ".x, s" << nm << "y = (double)g" << nm << ".y;\n";
         }
     // Custom factor rows (value, first and second derivatives) of the event

@@ -118,9 +118,11 @@ inline size_t assemble_lds_bytes(const FinArgs& f) {
            (size_t)(f.ncpe + f.nthe + f.nfree + f.ntheta + 2) * sizeof(int);
 }
 
-// Per-event inputs of a handle: SL tensors [NU][npol][ld] and invariants [K][ld].
+// Per-event inputs of a handle: SL tensors [NU][nps][ld] and invariants [K][ld].
 // precision 0: everything FP64.  precision 1 (mixed): SL tensors FP32, invariants FP32
 // except the squared masses s, which stay FP64 (m0^2 - s cancels near a pole).
+// nps < npol (mirror-image polarisations): polarisation p >= nps of SL tensor u is
+// psign[u] conj(polarisation npol - 1 - p).
 struct EvStore {
     const double2* sl;       // precision 0
     const float2* slf;       // precision 1
@@ -129,13 +131,22 @@ struct EvStore {
     const int* imap;         // precision 1: invariant i -> row in inv (>= 0) or -(row in invf) - 1
     long long ld;
     int mixed;
+    int nps;                 // stored polarisations per SL tensor
+    const int* psign;        // [NU] sign of the mirrored polarisation (nps < npol)
 };
-__device__ inline double2 ev_sl(const EvStore& s, long long row, long long e) {
+__device__ inline double2 ev_sl_row(const EvStore& s, long long row, long long e) {
     if (s.mixed) {
         float2 f = s.slf[row * s.ld + e];
         return make_double2((double)f.x, (double)f.y);
     }
     return s.sl[row * s.ld + e];
+}
+// polarisation p of SL tensor u for event e (npol: the model's polarisation count)
+__device__ inline double2 ev_sl(const EvStore& s, int u, int p, int npol, long long e) {
+    if (p < s.nps) return ev_sl_row(s, (long long)u * s.nps + p, e);
+    const double2 m = ev_sl_row(s, (long long)u * s.nps + (npol - 1 - p), e);
+    const double g = (double)s.psign[u];
+    return make_double2(g * m.x, -g * m.y);
 }
 __device__ inline double ev_inv(const EvStore& s, int i, long long e) {
     if (s.mixed) {

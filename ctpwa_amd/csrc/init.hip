@@ -15,6 +15,8 @@
 // Angles are handled algebraically (direction cosines and unit phasors) instead of
 // acos/atan2 round trips; Clebsch-Gordan products and Y_lm normalisations come from a
 // host-built table.
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace ctpwa {
@@ -513,5 +515,47 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_init(co
     long long nb = (N + bs - 1) / bs;
     hipLaunchKernelGGL(ctpwa::k_init_events, dim3((unsigned)nb), dim3(bs), 0, stream, mom, N, nfinal, topos, ntopo,
                        nodes, kins, nkin, us, nu, cgtab, polmap, npol, invs, ninv, tabs, sl, inv, ld, e_off);
+    return hipGetLastError();
+}
+
+namespace ctpwa {
+
+// mirror-image check of the two polarisations of every SL tensor over events
+// [e_off, e_off + n) of a [nu][2][ld] FP64 store: acc[4 u + 0] = max |sl1 - conj(sl0)|,
+// acc[4 u + 1] = max |sl1 + conj(sl0)|, acc[4 u + 2] = max |sl0| (componentwise maxima,
+// non-negative doubles kept as their bit patterns, so an atomic integer maximum orders
+// them; any order of the atomics gives the same maxima)
+__global__ void __launch_bounds__(256) k_polsym(const double2* __restrict__ sl, long long ld, long long e_off, long long n,
+                                                int nu, unsigned long long* __restrict__ acc) {
+    const int u = blockIdx.y;
+    double dp = 0.0, dm = 0.0, m0 = 0.0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const double2 a = sl[((long long)u * 2 + 0) * ld + e_off + i];
+        const double2 b = sl[((long long)u * 2 + 1) * ld + e_off + i];
+        dp = fmax(dp, fmax(fabs(b.x - a.x), fabs(b.y + a.y)));
+        dm = fmax(dm, fmax(fabs(b.x + a.x), fabs(b.y - a.y)));
+        m0 = fmax(m0, fmax(fabs(a.x), fabs(a.y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        dp = fmax(dp, __shfl_xor(dp, off, 64));
+        dm = fmax(dm, __shfl_xor(dm, off, 64));
+        m0 = fmax(m0, __shfl_xor(m0, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&acc[4 * u + 0], (unsigned long long)__double_as_longlong(dp));
+        atomicMax(&acc[4 * u + 1], (unsigned long long)__double_as_longlong(dm));
+        atomicMax(&acc[4 * u + 2], (unsigned long long)__double_as_longlong(m0));
+    }
+}
+
+}  // namespace ctpwa
+
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_polsym(const double2* sl, long long ld,
+                                                                                 long long e_off, long long n, int nu,
+                                                                                 unsigned long long* acc,
+                                                                                 hipStream_t stream) {
+    if (n <= 0 || nu <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)std::min<long long>(256, (n + 255) / 256);
+    hipLaunchKernelGGL(ctpwa::k_polsym, dim3(nb, (unsigned)nu), dim3(256), 0, stream, sl, ld, e_off, n, nu, acc);
     return hipGetLastError();
 }

@@ -236,6 +236,12 @@ struct Model_ {
     // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
     // invariants, FP64 squared masses and Custom-expression m, q)
     int mixed = 0;
+    // polarisations stored per SL tensor: npol, or 1 when the two polarisations of every SL
+    // tensor are mirror images, sl_u,1 = psign[u] conj(sl_u,0) (set at handle creation
+    // after a check over all events; spin-0 final states of a J = M, -M initial state)
+    int nps = 0;
+    std::vector<int> psign;
+    int npstore() const { return nps ? nps : P.npol; }
     // storage row of invariant i: >= 0 row of the FP64 store, < 0: -(row of the FP32 store) - 1
     std::vector<int> imap() const {
         std::vector<int> m(inv.size());

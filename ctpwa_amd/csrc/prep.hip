@@ -519,7 +519,7 @@ __global__ void k_amplitudes(AuxArgs a, long long e0, long long n, double* __res
         double2 psi = block_psi(a, A.block, e);
         double K = a.table[a.off_rec + (long long)A.rec * REC];
         for (int p = 0; p < a.npol; ++p) {
-            double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
+            double2 s = ev_sl(a.st, A.u, p, a.npol, e);
             double r = K * (psi.x * s.x - psi.y * s.y), im = K * (psi.x * s.y + psi.y * s.x);
             long long o = ((i * a.npol + p) * a.namp + am) * 2;
             out[o] = r;
@@ -540,7 +540,7 @@ __global__ void k_amp_rows(AuxArgs a, long long e0, long long n, double* __restr
         double2 psi = block_psi(a, A.block, e);
         double K = a.table[a.off_rec + (long long)A.rec * REC];
         for (int p = 0; p < a.npol; ++p) {
-            double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
+            double2 s = ev_sl(a.st, A.u, p, a.npol, e);
             X[(long long)am * ldx + p * n + i] = K * (psi.x * s.x - psi.y * s.y);
             X[(long long)(a.namp + am) * ldx + p * n + i] = K * (psi.x * s.y + psi.y * s.x);
         }
@@ -571,7 +571,7 @@ __global__ void k_partial_int(AuxArgs a, const int* __restrict__ wptr, const int
                 for (int c = 0; c < B.nsl; ++c) {
                     const int am = B.site + c;
                     const double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
-                    double2 s = ev_sl(a.st, (long long)a.amps[am].u * a.npol + p, e);
+                    double2 s = ev_sl(a.st, a.amps[am].u, p, a.npol, e);
                     yr += cr * s.x - ci * s.y;
                     yi += cr * s.y + ci * s.x;
                 }
@@ -610,7 +610,7 @@ __global__ void k_partial_amp(AuxArgs a, const int* __restrict__ wptr, const int
                 for (int c = 0; c < B.nsl; ++c) {
                     const int am = B.site + c;
                     const double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
-                    double2 s = ev_sl(a.st, (long long)a.amps[am].u * a.npol + p, e);
+                    double2 s = ev_sl(a.st, a.amps[am].u, p, a.npol, e);
                     yr += cr * s.x - ci * s.y;
                     yi += cr * s.y + ci * s.x;
                 }
@@ -634,7 +634,7 @@ __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __rest
         double cr = a.table[a.off_c + 2 * am], ci = a.table[a.off_c + 2 * am + 1];
         double yr = cr * psi.x - ci * psi.y, yi = cr * psi.y + ci * psi.x;
         for (int p = 0; p < a.npol; ++p) {
-            double2 s = ev_sl(a.st, (long long)A.u * a.npol + p, e);
+            double2 s = ev_sl(a.st, A.u, p, a.npol, e);
             Sr[p] += yr * s.x - yi * s.y;
             Si[p] += yr * s.y + yi * s.x;
         }
