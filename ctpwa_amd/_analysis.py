@@ -124,11 +124,10 @@ class analysis:
         self._weights = w
         self._truth = None
         # theta of the stored amplitudes: a host copy (initial values, reCalcAmp) or, after
-        # getNLL / getHessian / writeResult, a device copy of the theta they were called at
-        # (taken on the stream without a host wait, read back only on demand).  Other
-        # calls (intensities, amplitudes, Grams) do not move it.
+        # getNLL / getHessian / writeResult, the theta the device evaluated last (the
+        # handle keeps it in its own slot, which intensities, amplitudes and Grams leave;
+        # read back only on demand, so evaluations never wait for the host)
         self._theta_host = np.asarray([t.init for t in self.plan.theta], dtype=np.float64)
-        self._theta_dev: Optional[torch.Tensor] = None
         self._theta_on_host = True
         self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
                                  w["phsp"], w["data"], w["bkg"], device=device, precision=precision)
@@ -291,7 +290,10 @@ class analysis:
             if not 0 <= w < nw:
                 raise RuntimeError("waves index out of range: %d (partials = %d)" % (w, nw))
         p = params.detach().contiguous()
-        self._remember_theta(p)
+        # writeResult moves the stored amplitudes to its theta (host copy: the Grams and
+        # per-event outputs below do not store a last-evaluated theta)
+        self._theta_host = p[2 * self.plan.n_free:].cpu().numpy().astype(np.float64)
+        self._theta_on_host = True
         Q = self._gram(self._h, p, 0)
         X, total = R.interference_matrix(Q, self.plan, p.cpu().numpy())
         nd = float(np.sum(self._weights["data"])) if self._weights["data"] is not None else float(self.n_events["data"])
@@ -409,14 +411,13 @@ class analysis:
         return A.reshape(-1)
 
     def _remember_theta(self, p: torch.Tensor):
-        # an asynchronous device copy of p's theta (no host synchronisation)
-        self._theta_dev = p[2 * self.plan.n_free:].clone()
+        # the native call that follows stores p's theta as the last evaluated one
         self._theta_on_host = False
 
     @property
     def _theta(self) -> np.ndarray:
         if not self._theta_on_host:
-            self._theta_host = self._theta_dev.cpu().numpy().astype(np.float64)
+            self._theta_host = self._h.last_theta(torch.cuda.current_stream(self.device).cuda_stream)
             self._theta_on_host = True
         return self._theta_host
 

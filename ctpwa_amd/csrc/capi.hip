@@ -34,7 +34,7 @@ extern "C" hipError_t ctpwa_launch_to_mixed(const double2*, const double*, long 
 extern "C" hipError_t ctpwa_launch_polsym(const double2*, long long, long long, long long, int, unsigned long long*,
                                           hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
-extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, unsigned int*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep_batch(const double* const*, double* const*, const ctpwa::PrepArgs*, int, int,
                                               hipStream_t);
 extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, int, size_t, hipStream_t);
@@ -317,6 +317,7 @@ struct ctpwa_handle {
     int* d_imap = nullptr;
     const void* slp = nullptr;      // the SL store the fused kernel reads
     int* d_psign = nullptr;         // [NU] signs of the mirrored polarisation (folded store)
+    unsigned int* fin_done = nullptr;   // finishing-workgroup counter of the final assembly (zero between calls)
     EvStore store() const { return EvStore{sl, slf, inv, invf, d_imap, ntot, mixed, M.npstore(), d_psign}; }
 
     template <class T>
@@ -745,11 +746,13 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         setup_partition(h);
         h->nacc = M.nacc();
         h->part = h->alloc<double>((size_t)h->nacc * std::max(1, h->nwg));
+        h->fin_done = h->alloc<unsigned int>(1);
+        HIPCHECK(hipMemset(h->fin_done, 0, sizeof(unsigned int)));
 
         h->prep = PrepArgs{d_blocks, d_res, d_nodes2, d_topos2, d_slcL, d_slc0, d_dens, d_amps,
                            (int)dens.size(), P.namp, P.nfree, P.ntheta,
                            (long long)M.off_v(), (long long)M.off_c(), (long long)M.off_rec(),
-                           (long long)M.off_th(), nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                           (long long)M.off_th(), (long long)M.off_lth(), 0, nullptr, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         {
             std::vector<char> blob;
             auto put = [&](const void* p, size_t n) -> int {
@@ -940,13 +943,13 @@ extern "C" int ctpwa_nll_grad(ctpwa_handle* h, const double* params_dev, double*
     if (!h || !params_dev || !out_dev) return fail("ctpwa_nll_grad: NULL argument");
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 2, s);
     if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess) e = launch_eval(h, s);
     if (e == hipSuccess) {
         h->fin.params = params_dev;
         h->fin.out = out_dev;
-        e = ctpwa_launch_finalize(&h->fin, s);
+        e = ctpwa_launch_finalize(&h->fin, h->fin_done, s);
     }
     if (e != hipSuccess) return fail(std::string("ctpwa_nll_grad: ") + hipGetErrorString(e));
     return 0;
@@ -962,7 +965,7 @@ extern "C" int ctpwa_nll_grad_noalloc(ctpwa_handle* h, const double* params_dev,
     double* scr = scratch_lookup(h->device);
     if (!scr) return fail("ctpwa_nll_grad_noalloc: no scratch address for this device");
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s);
+    hipError_t e = ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 2, s);
     if (e == hipSuccess) e = launch_custom(h, s);
     if (e == hipSuccess) e = launch_eval(h, s, scr);
     if (e == hipSuccess) {
@@ -971,7 +974,7 @@ extern "C" int ctpwa_nll_grad_noalloc(ctpwa_handle* h, const double* params_dev,
         f.totals = scr + (size_t)h->nacc * (size_t)std::max(1, h->nwg);
         f.params = params_dev;
         f.out = out_dev;
-        e = ctpwa_launch_finalize(&f, s);
+        e = ctpwa_launch_finalize(&f, h->fin_done, s);
     }
     if (e != hipSuccess) return fail(std::string("ctpwa_nll_grad_noalloc: ") + hipGetErrorString(e));
     return 0;
@@ -1166,7 +1169,7 @@ extern "C" int ctpwa_last_theta(ctpwa_handle* h, double* theta_host, void* strea
     if (h->M.P.ntheta == 0) return 0;
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = hipMemcpyAsync(theta_host, h->table + h->M.off_th(), (size_t)h->M.P.ntheta * sizeof(double),
+    hipError_t e = hipMemcpyAsync(theta_host, h->table + h->M.off_lth(), (size_t)h->M.P.ntheta * sizeof(double),
                                   hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(std::string("ctpwa_last_theta: ") + hipGetErrorString(e));
@@ -1181,7 +1184,7 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         if (!h->hess_ready) hess_setup(h);
         const Model_& M = h->M;
         const Plan& P = M.P;
-        HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, M.maxloc, 1, s));
+        HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, M.maxloc, 3, s));
         if (h->fn_custom2) {
             void* args[] = {&h->table, &h->inv, &h->invf, &h->ntot, &h->cfb2};
             unsigned grid = (unsigned)std::min<long long>(16384, (h->ntot + 63) / 64);

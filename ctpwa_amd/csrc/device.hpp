@@ -73,6 +73,8 @@ struct PrepArgs {
     int nden, namp, nfree, ntheta;
     long long off_v, off_c, off_rec;
     long long off_th;        // theta values (read by Custom lineshapes in the fused kernel)
+    long long off_lth;       // theta of the last NLL / Hessian evaluation (ctpwa_last_theta)
+    int remember;            // this launch writes theta to off_lth as well
     // the same tables packed in one blob (16-byte aligned sections) that the prep kernel
     // copies to LDS before use; meta_bytes == 0: read the tables from global memory
     const void* meta;

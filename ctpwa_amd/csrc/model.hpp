@@ -264,7 +264,7 @@ struct Model_ {
     size_t off_v() const { return 0; }
     size_t off_c() const { return 2 * (size_t)P.namp; }
     size_t off_rec() const { return 4 * (size_t)P.namp; }
-    size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC + (size_t)P.ntheta; }
+    size_t table_size() const { return off_rec() + (size_t)n_coef_rec * REC + 2 * (size_t)P.ntheta; }
     int nacc() const { return 1 + 2 * P.namp + (int)thterms.size() + (int)cterms.size(); }
     // Custom factors: derivative directions (the largest block among Custom blocks) and
     // per-event rows of one factor in the factor buffer (value, then dr[], di[])
@@ -277,6 +277,7 @@ struct Model_ {
     // the Hessian's second-order rows of one factor: value, dr[], di[], hr[], hi[] (packed k <= l)
     int ncf2() const { return 2 + 2 * ncv() + ncv() * (ncv() + 1); }
     size_t off_th() const { return off_rec() + (size_t)n_coef_rec * REC; }   // theta values (for Custom)
+    size_t off_lth() const { return off_th() + (size_t)P.ntheta; }          // theta of the last NLL / Hessian
 };
 
 // final-state subset of a momentum reference

@@ -224,7 +224,10 @@ __device__ __forceinline__ void prep_body(const double* __restrict__ params, dou
     int item = blockIdx.x;
     const double* theta = params + 2 * a.nfree;
     if (blockIdx.x == 0)
-        for (int j = threadIdx.x; j < a.ntheta; j += blockDim.x) table[a.off_th + j] = theta[j];
+        for (int j = threadIdx.x; j < a.ntheta; j += blockDim.x) {
+            table[a.off_th + j] = theta[j];
+            if (a.remember) table[a.off_lth + j] = theta[j];
+        }
     if (threadIdx.x != 0) return;
     if (item < a.nden) {
         const DDen& D = a.dens[item];
@@ -469,6 +472,37 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
 
 __global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) { assemble_body(f); }
 
+// k_reduce and k_assemble in one launch: every workgroup sums one (class, accumulator)
+// over the class's workgroup partials (the order of k_reduce), and the workgroup that
+// finishes last, found by a device-scope counter, runs the assembly on the totals and
+// resets the counter for the next evaluation.  Results are those of the two kernels.
+static_assert(RED_THREADS == ASM_THREADS, "one workgroup size for both parts");
+__global__ void __launch_bounds__(ASM_THREADS) k_finalize(FinArgs f, unsigned int* __restrict__ done) {
+    __shared__ double sh[RED_THREADS];
+    __shared__ int last;
+    const int pair = blockIdx.x;
+    const int c = pair / f.nacc, col = pair - c * f.nacc;
+    const double* src = f.part + (long long)col * f.nwg;
+    double s = 0.0;
+    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        f.totals[pair] = sh[0];
+        __threadfence();                                   // the total is visible before the count
+        last = atomicAdd(done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();                                       // every total is visible to this workgroup
+    assemble_body(f);
+    if (threadIdx.x == 0) *done = 0u;
+}
+
 __global__ void __launch_bounds__(ASM_THREADS) k_assemble_batch(const FinArgs* __restrict__ fins) {
     assemble_body(fins[blockIdx.x]);
 }
@@ -647,28 +681,36 @@ __global__ void k_intensity(AuxArgs a, long long e0, long long n, double* __rest
 }  // namespace ctpwa
 
 template <int NV, bool H>
-static hipError_t launch_prep_t(const double* params, double* table, const ctpwa::PrepArgs* a, hipStream_t stream) {
+static hipError_t launch_prep_t(const double* params, double* table, const ctpwa::PrepArgs* a, int remember,
+                                hipStream_t stream) {
     int items = a->nden + a->namp;
-    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3(items), dim3(64), (size_t)a->meta_bytes, stream, params, table, *a);
+    ctpwa::PrepArgs b = *a;
+    b.remember = remember;
+    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3(items), dim3(64), (size_t)a->meta_bytes, stream, params, table, b);
     return hipGetLastError();
 }
 
-// nv: largest number of free parameters in one block; hess: also second derivatives
+// nv: largest number of free parameters in one block; flags: bit 0 also second
+// derivatives (Hessian), bit 1 theta is also stored as the last evaluated theta (the
+// NLL and Hessian entry points; the per-event outputs leave it)
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep(const double* params, double* table,
                                                                               const ctpwa::PrepArgs* a, int nv,
-                                                                              int hess, hipStream_t stream) {
-    if (nv <= 1) return hess ? launch_prep_t<1, true>(params, table, a, stream) : launch_prep_t<1, false>(params, table, a, stream);
-    if (nv <= 2) return hess ? launch_prep_t<2, true>(params, table, a, stream) : launch_prep_t<2, false>(params, table, a, stream);
-    if (nv <= 4) return hess ? launch_prep_t<4, true>(params, table, a, stream) : launch_prep_t<4, false>(params, table, a, stream);
-    return hess ? launch_prep_t<8, true>(params, table, a, stream) : launch_prep_t<8, false>(params, table, a, stream);
+                                                                              int flags, hipStream_t stream) {
+    const int hess = flags & 1, rem = (flags >> 1) & 1;
+    if (nv <= 1) return hess ? launch_prep_t<1, true>(params, table, a, rem, stream) : launch_prep_t<1, false>(params, table, a, rem, stream);
+    if (nv <= 2) return hess ? launch_prep_t<2, true>(params, table, a, rem, stream) : launch_prep_t<2, false>(params, table, a, rem, stream);
+    if (nv <= 4) return hess ? launch_prep_t<4, true>(params, table, a, rem, stream) : launch_prep_t<4, false>(params, table, a, rem, stream);
+    return hess ? launch_prep_t<8, true>(params, table, a, rem, stream) : launch_prep_t<8, false>(params, table, a, rem, stream);
 }
 
 template <int NV, bool H>
 static hipError_t launch_prep_batch_t(const double* const* params, double* const* tables, const ctpwa::PrepArgs* a,
                                       int njobs, hipStream_t stream) {
     int items = a->nden + a->namp;
+    ctpwa::PrepArgs b = *a;
+    b.remember = 1;                 // NLL evaluations: each table keeps its theta as the last evaluated one
     hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3(items, njobs), dim3(64), (size_t)a->meta_bytes, stream, params,
-                       tables, *a);
+                       tables, b);
     return hipGetLastError();
 }
 
@@ -695,11 +737,11 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finaliz
     return hipGetLastError();
 }
 
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs* f, hipStream_t stream) {
-    hipLaunchKernelGGL(ctpwa::k_reduce, dim3(3 * f->nacc), dim3(ctpwa::RED_THREADS), 0, stream, *f);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ctpwa::k_assemble, dim3(1), dim3(ctpwa::ASM_THREADS), ctpwa::assemble_lds_bytes(*f), stream, *f);
+// done: a device counter that is zero between evaluations (reset by the kernel itself)
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs* f, unsigned int* done,
+                                                                                  hipStream_t stream) {
+    hipLaunchKernelGGL(ctpwa::k_finalize, dim3(3 * f->nacc), dim3(ctpwa::ASM_THREADS), ctpwa::assemble_lds_bytes(*f), stream,
+                       *f, done);
     return hipGetLastError();
 }
 

@@ -124,11 +124,12 @@ int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, dou
  * Fails for models with Custom lineshapes */
 int ctpwa_partial_intensities(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream);
 
-/* theta (the last ntheta parameters) of the handle's most recent evaluation of any kind,
- * copied to `theta_host` [ntheta] after the work queued on `stream`: the resonance
- * parameters at which the reference's stored amplitudes stand after getNLL / getHessian
- * / writeResult (src/AmpGen.cu:2344-2726), for the post-fit methods that use them.
- * Undefined before the first evaluation */
+/* theta (the last ntheta parameters) of the handle's most recent ctpwa_nll_grad,
+ * ctpwa_nll_grad_noalloc, batched evaluation or ctpwa_hessian (the per-event outputs and
+ * Grams leave it), copied to `theta_host` [ntheta] after the work queued on `stream`:
+ * the resonance parameters at which the reference's stored amplitudes stand after
+ * getNLL / getHessian / writeResult (src/AmpGen.cu:2344-2726), for the post-fit methods
+ * that use them.  Undefined before the first evaluation */
 int ctpwa_last_theta(ctpwa_handle* h, double* theta_host, void* stream);
 
 /* per-event coherent partial-wave amplitudes of class `cls` (couplings applied): device
