@@ -101,8 +101,9 @@ def cpu_baseline(cfg_path, frac, threads, hessian=True):
     return out
 
 
-def run_form(ana, args, spec, torch, W, prec, sizes):
+def run_form(ana, args, spec, torch, W, prec, sizes, label=None, hessian=True):
     """Time K NLL+gradient evaluations at perturbed points in one precision form."""
+    label = label or prec
     plan = ana.plan
     p0 = W.nominal_params(plan, None if args.workload == "jpsi_rho" else spec["seed"])
     rng = np.random.default_rng(123)
@@ -131,7 +132,7 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
     ms_per_step = dt / args.steps * 1e3
     value = args.steps / dt
     nll_last = float(last[0].item())
-    log("[bench] %s: %d steps in %.4f s: %.4f ms/step, NLL %.6f" % (prec, args.steps, dt, ms_per_step, nll_last))
+    log("[bench] %s: %d steps in %.4f s: %.4f ms/step, NLL %.6f" % (label, args.steps, dt, ms_per_step, nll_last))
     # the rate a fit loop sees: the same steps with the loss read on the host every step
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -141,7 +142,7 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
     dt_sync = time.perf_counter() - t
     sync = {"value": round(args.steps / dt_sync, 3), "ms_per_step": round(dt_sync / args.steps * 1e3, 4),
             "def": "the same steps with the loss copied to the host (nll.item()) after every step"}
-    log("[bench] %s synchronous: %.4f ms/step" % (prec, sync["ms_per_step"]))
+    log("[bench] %s synchronous: %.4f ms/step" % (label, sync["ms_per_step"]))
 
     # dominant kernel: the fused per-event kernel, timed with HIP events on the stream it runs on
     lay = ana.layout()
@@ -156,7 +157,7 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
     traffic = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                          "traffic_ctpwa_eval_%s_%s.txt" % (args.workload, prec))
-    if os.path.exists(tfile) and sizes == (spec["n_data"], spec["n_phsp"], spec["n_bkg"]):
+    if label == prec and os.path.exists(tfile) and sizes == (spec["n_data"], spec["n_phsp"], spec["n_bkg"]):
         with open(tfile) as f:
             for line in f:
                 if line.startswith("bytes_per_launch"):
@@ -172,9 +173,9 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
             "kernel": "ctpwa_eval", "kernel_ms": round(kern_ms, 5),
             "algorithmic_bytes": lay["bytes_per_eval"], "algorithmic_bytes_def": bdef,
             "per_amplitude_bytes_formula": survey_bytes}
-    log("[bench] %s fused kernel %.4f ms, %.1f GB/s of %d B" % (prec, kern_ms, achieved, lay["bytes_per_eval"]))
+    log("[bench] %s fused kernel %.4f ms, %.1f GB/s of %d B" % (label, kern_ms, achieved, lay["bytes_per_eval"]))
     hess_ms = None
-    if args.hessian_reps > 0:
+    if hessian and args.hessian_reps > 0:
         # getHessian on the same workload (not part of `value`): one warm-up, then the mean
         ph = pts[0].detach()
         ana.getHessian(ph)
@@ -187,6 +188,34 @@ def run_form(ana, args, spec, torch, W, prec, sizes):
         log("[bench] %s getHessian %.3f ms" % (prec, hess_ms))
     return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof, "hessian_ms": hess_ms,
             "sync": sync}
+
+
+def run_fixed_theta(args, cfg, samples, spec, torch, W, precisions, sizes):
+    """The same workload and events with every resonance mass and width fixed at its
+    configured value: no free resonance parameter, so the phase-space term of each
+    evaluation is the quadratic form of the phase-space amplitude Gram built once when the
+    analysis is created, and the fused kernel reads the data and background only."""
+    from ctpwa_amd import analysis
+    fixed_cfg = os.path.join(os.path.dirname(cfg), "config_fixed_theta.yml")
+    with open(cfg) as f:
+        txt = f.read()
+    with open(fixed_cfg, "w") as f:
+        f.write(W.fixed_theta_config(txt))
+    res = {"config": "the headline workload's events with every resonance parameter fixed (no free theta): "
+                     "phase-space term from the amplitude Gram stored at creation"}
+    for prec in precisions:
+        t0 = time.time()
+        ana = analysis(fixed_cfg, samples=samples, precision=prec)
+        torch.cuda.synchronize()
+        init_s = time.time() - t0
+        r = run_form(ana, args, spec, torch, W, prec, sizes, label=prec + " fixed-theta", hessian=False)
+        rf = r["roofline"]
+        res[prec] = {"value": round(r["value"], 3), "ms_per_step": round(r["ms_per_step"], 4), "sync": r["sync"],
+                     "kernel_ms": rf["kernel_ms"], "bytes_per_eval": rf["algorithmic_bytes"],
+                     "kernel_frac_of_hbm": rf["frac"], "init_s": round(init_s, 2), "nll": r["nll"]}
+        del ana
+        torch.cuda.empty_cache()
+    return res
 
 
 def hessian_flops(plan, n_data, n_phsp, n_bkg):
@@ -313,6 +342,8 @@ def main():
     ap.add_argument("--hessian-wide-reps", type=int, default=3,
                     help="getHessian calls timed per form on the 149-amplitude four-body cascade at "
                          "2M data / 10M phase space (0: skip)")
+    ap.add_argument("--no-fixed-theta", dest="fixed_theta", action="store_false",
+                    help="skip the leg that times the workload with every resonance parameter fixed")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (a plain copy kernel of "
                          "known size that calibrates the HBM byte counters of a --pmc profile)")
@@ -362,6 +393,10 @@ def main():
                    "speedup_mixed_vs_fp64": round(r["value"] / other["value"], 3),
                    "delta_nll_mixed_minus_fp64": r["nll"] - other["nll"],
                    "delta_nll_points": "NLL at the same last timed parameter point in both forms"}
+
+    fixed = None
+    if args.fixed_theta:
+        fixed = run_fixed_theta(args, cfg, samples, spec, torch, W, precisions, (n_data, n_phsp, n_bkg))
 
     serving = None
     if args.serving_jobs > 0:
@@ -416,6 +451,8 @@ def main():
                                   % args.hessian_reps,
                           "ms": hms, "algorithmic_flops": fl, "mfma_fp64_peak_tflops": 78.6,
                           "frac_of_fp64_mfma_peak": {k: round(fl / (v * 1e-3) / 78.6e12, 4) for k, v in hms.items()}}
+    if fixed is not None:
+        out["fixed_theta"] = fixed
     if wide is not None:
         out["hessian_wide"] = wide
     if serving is not None:

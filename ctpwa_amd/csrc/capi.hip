@@ -49,7 +49,7 @@ extern "C" int ctpwa_gram_nwg(int);
 extern "C" hipError_t ctpwa_launch_gram_panels(const void*, int, int, long long, long long, const double*, long long,
                                                 const int4*, int, int, double*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_accum_panels(const double*, int, int, const int2*, int, double*, hipStream_t);
-extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, hipStream_t);
+extern "C" hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs*, long long, long long, double*, long long, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_partial_int(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
                                                double*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_partial_amp(const ctpwa::AuxArgs*, const int*, const int*, int, long long, long long,
@@ -309,6 +309,10 @@ struct ctpwa_handle {
     long long gchunk = 0;
     double *gX = nullptr, *gW = nullptr, *gtot = nullptr;
     GramTabs gg;
+    // no free resonance parameter: the phase-space amplitude Gram without couplings or
+    // theta coefficients, built once at creation; the evaluations then scan data and
+    // background only (fin.gram points here)
+    double2* d_gram = nullptr;
     std::vector<int> used_inv;
     // storage precision (0 FP64, 1 mixed) and the FP32 stores of the mixed form
     int mixed = 0;
@@ -439,14 +443,18 @@ static void check_q0_masses(const Model_& M) {
     }
 }
 
-static void setup_partition(ctpwa_handle* h) {
-    const long long nev = h->n[0] + h->n[1] + h->n[2];
-    long long target = std::min<long long>(1024, std::max<long long>(1, (nev + 1023) / 1024));
+// workgroups of the fused kernel: about 1024 (four 128-thread workgroups per CU), at least
+// 256 events each, shared out among the classes by event count, each a whole number of
+// 256-event steps of one class.  scan_phsp == false (phase-space term from the stored
+// Gram): the phase space gets none.
+static void setup_partition(ctpwa_handle* h, bool scan_phsp) {
+    const long long nev = (scan_phsp ? h->n[0] : 0) + h->n[1] + h->n[2];
+    long long target = std::min<long long>(1024, std::max<long long>(1, (nev + 255) / 256));
     std::vector<int> cls;
     std::vector<long long> rng;
     for (int c = 0; c < 3; ++c) {
         h->g0[c] = (int)cls.size();
-        long long n = h->n[c];
+        long long n = (c == 0 && !scan_phsp) ? 0 : h->n[c];
         if (n > 0) {
             long long want = std::max<long long>(1, (long long)std::llround((double)target * n / nev));
             long long chunk = (n + want - 1) / want;
@@ -536,6 +544,9 @@ static void fold_polarisations(ctpwa_handle* h, const unsigned long long* d_sym,
     h->M.psign = sign;
     h->d_psign = h->upload(sign);
 }
+
+static bool fixed_gram_applies(const ctpwa_handle* h);
+static void build_fixed_gram(ctpwa_handle* h);
 
 extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long n_phsp, const double* w_phsp,
                             const double* data, long long n_data, const double* w_data,
@@ -743,8 +754,9 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->table = h->alloc<double>(M.table_size());
         HIPCHECK(hipMemset(h->table, 0, M.table_size() * sizeof(double)));
 
-        setup_partition(h);
         h->nacc = M.nacc();
+        const bool use_gram = fixed_gram_applies(h);
+        setup_partition(h, !use_gram);
         h->part = h->alloc<double>((size_t)h->nacc * std::max(1, h->nwg));
         h->fin_done = h->alloc<unsigned int>(1);
         HIPCHECK(hipMemset(h->fin_done, 0, sizeof(unsigned int)));
@@ -884,6 +896,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
             for (const Term& t : D.re) if (t.inv >= 0) h->used_inv[t.inv] = 1;
             for (const Term& t : D.im) if (t.inv >= 0) h->used_inv[t.inv] = 1;
         }
+        if (use_gram) build_fixed_gram(h);
         HIPCHECK(hipStreamSynchronize(nullptr));
         (void)hipSetDevice(prev);
         *out = h;
@@ -926,6 +939,7 @@ static hipError_t launch_custom(ctpwa_handle* h, hipStream_t s) {
 }
 
 static hipError_t launch_eval(ctpwa_handle* h, hipStream_t s, double* part = nullptr) {
+    if (h->nwg <= 0) return hipSuccess;       // stored phase-space Gram and no data events
     double* pp = part ? part : h->part;
     void* args[] = {&h->table, &h->slp, &h->inv, &h->invf, &h->wts, &pp, &h->wg_cls, &h->wg_rng, &h->ntot,
                     &h->nwg, &h->wc[1], &h->wc[2], &h->pw[1], &h->pw[2], &h->cfb};
@@ -1095,16 +1109,74 @@ static void hess_setup(ctpwa_handle* h) {
 
 // amplitude Gram of one class: Q[l][k] = sum_e sum_p conj(A_l) A_k (amplitudes without
 // couplings), the integrals behind fit fractions and the interference matrix
+// events per chunk of amplitude rows: at most 2^18, and rows of at most 256 MiB
+static long long agram_chunk(const Plan& P, long long maxn) {
+    const long long per_event = (long long)P.npol * 2 * P.namp * 8;
+    const long long cap = std::max<long long>(4, (256LL << 20) / per_event);
+    return (std::max<long long>(1, std::min<long long>(maxn, std::min<long long>(cap, 1 << 18))) + 3) / 4 * 4;
+}
+static long long agram_workspace_bytes(const Plan& P, long long maxn) {
+    const long long n2 = 2 * P.namp, ch = agram_chunk(P, maxn);
+    return ch * P.npol * n2 * 8 + ch * 8 + n2 * (n2 + 1) / 2 * 8 + gram_part_doubles((int)n2) * 8;
+}
 static void agram_setup(ctpwa_handle* h) {
     const Plan& P = h->M.P;
     const int n2 = 2 * P.namp;
     long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
-    h->gchunk = (std::max<long long>(1, std::min<long long>(maxn, 1 << 18)) + 3) / 4 * 4;
+    h->gchunk = agram_chunk(P, maxn);
     h->gX = h->alloc<double>((size_t)h->gchunk * P.npol * n2);
     h->gW = h->upload(std::vector<double>((size_t)h->gchunk, 1.0));
     h->gtot = h->alloc<double>((size_t)n2 * (n2 + 1) / 2);
     gram_setup(h, h->gg, n2);
     h->agram_ready = true;
+}
+
+// Gram of class cls's amplitude rows at the parameter point the table holds, to out
+// [namp][namp][2]: with the theta coefficients K_a (withK = 1) or without them (0)
+static void class_gram(ctpwa_handle* h, int cls, int withK, double* out, hipStream_t s) {
+    if (!h->agram_ready) agram_setup(h);
+    const Plan& P = h->M.P;
+    const int n2 = 2 * P.namp;
+    const long long ldx = h->gchunk * P.npol;
+    HIPCHECK(ctpwa_launch_zero(h->gtot, (long long)n2 * (n2 + 1) / 2, s));
+    const GramTabs& G = h->gg;
+    int k = 0;
+    for (long long s0 = 0; s0 < h->n[cls]; s0 += h->gchunk, ++k) {
+        long long cnt = std::min(h->gchunk, h->n[cls] - s0);
+        HIPCHECK(ctpwa_launch_amp_rows(&h->aux, h->off[cls] + s0, cnt, h->gX, ldx, withK, s));
+        HIPCHECK(ctpwa_launch_gram_panels(h->gX, 0, n2, ldx, cnt * P.npol, h->gW, cnt, G.pp, G.npp, G.ntp, G.part,
+                                          G.nwg, k > 0, s));
+    }
+    if (h->n[cls] > 0) HIPCHECK(ctpwa_launch_accum_panels(G.part, G.nwg, G.ntp, G.tiles, n2, h->gtot, s));
+    HIPCHECK(ctpwa_launch_gram_to_complex(h->gtot, P.namp, out, s));
+}
+
+// No free resonance parameter: the amplitudes without couplings, A'_a = Psi_b sl_u, do
+// not change between evaluations, so the phase-space sum of I = sum_p |sum_a c_a A'_a|^2
+// is the quadratic form c^H G c of their Gram G, and the phase-space gradient sums
+// sum_e conj(A'_a) S are (G c)_a (the reference's stored phase-space matrix and
+// quadratic form, src/Analysis.cu:808-826, src/ComputeNLL.cu:575-597).  G is built once
+// here in FP64 on the matrix cores; the final reduction forms the phase-space totals from
+// it and the fused kernel scans data and background only.  Environment CTPWA_FIXED_GRAM=0
+// keeps the per-event phase-space scan (for comparisons).
+static bool fixed_gram_applies(const ctpwa_handle* h) {
+    const Plan& P = h->M.P;
+    if (P.ntheta != 0 || h->nacc != 1 + 2 * P.namp || h->n[0] <= 0) return false;
+    const char* e = std::getenv("CTPWA_FIXED_GRAM");
+    return !(e && std::string(e) == "0");
+}
+static void build_fixed_gram(ctpwa_handle* h) {
+    const Plan& P = h->M.P;
+    hipStream_t s = nullptr;
+    // the table's coefficient records depend on theta only (none free): any couplings do
+    double* p0 = h->upload(std::vector<double>((size_t)std::max(1, 2 * P.nfree), 0.0));
+    double2* G = h->alloc<double2>((size_t)P.namp * P.namp);
+    HIPCHECK(ctpwa_launch_prep(p0, h->table, &h->prep, h->M.maxloc, 0, s));
+    HIPCHECK(launch_custom(h, s));
+    class_gram(h, 0, 0, (double*)G, s);
+    h->d_gram = G;
+    h->fin.gram = G;
+    h->fin.off_c = (long long)h->M.off_c();
 }
 
 extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, int cls, double* out_dev, void* stream) {
@@ -1113,23 +1185,9 @@ extern "C" int ctpwa_amplitude_gram(ctpwa_handle* h, const double* params_dev, i
     DeviceGuard g(h->device);
     hipStream_t s = (hipStream_t)stream;
     try {
-        if (!h->agram_ready) agram_setup(h);
-        const Plan& P = h->M.P;
-        const int n2 = 2 * P.namp;
-        const long long ldx = h->gchunk * P.npol;
         HIPCHECK(ctpwa_launch_prep(params_dev, h->table, &h->prep, h->M.maxloc, 0, s));
         HIPCHECK(launch_custom(h, s));
-        HIPCHECK(ctpwa_launch_zero(h->gtot, (long long)n2 * (n2 + 1) / 2, s));
-        const GramTabs& G = h->gg;
-        int k = 0;
-        for (long long s0 = 0; s0 < h->n[cls]; s0 += h->gchunk, ++k) {
-            long long cnt = std::min(h->gchunk, h->n[cls] - s0);
-            HIPCHECK(ctpwa_launch_amp_rows(&h->aux, h->off[cls] + s0, cnt, h->gX, ldx, s));
-            HIPCHECK(ctpwa_launch_gram_panels(h->gX, 0, n2, ldx, cnt * P.npol, h->gW, cnt, G.pp, G.npp, G.ntp, G.part,
-                                              G.nwg, k > 0, s));
-        }
-        if (h->n[cls] > 0) HIPCHECK(ctpwa_launch_accum_panels(G.part, G.nwg, G.ntp, G.tiles, n2, h->gtot, s));
-        HIPCHECK(ctpwa_launch_gram_to_complex(h->gtot, P.namp, out_dev, s));
+        class_gram(h, cls, 1, out_dev, s);
         return 0;
     } catch (const std::exception& e) {
         return fail(std::string("ctpwa_amplitude_gram: ") + e.what());
@@ -1385,6 +1443,13 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
             max_nacc = std::max(max_nacc, h->nacc);
             max_lds = std::max(max_lds, assemble_lds_bytes(h->fin));
         }
+        {
+            // a handle holds one coefficient table and one set of partial sums
+            std::vector<const ctpwa_handle*> seen(hs, hs + n);
+            std::sort(seen.begin(), seen.end());
+            if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+                throw std::runtime_error("a handle appears twice in one batch (each job needs a handle of its own)");
+        }
         // the staging buffer is reused: wait for the previous upload from it
         if (b->pending) HIPCHECK(hipEventSynchronize(b->copied));
         EvalJob* jobs = (EvalJob*)(b->host + b->o_jobs);
@@ -1411,7 +1476,7 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
         for (int j = 0; j < n; ++j) HIPCHECK(launch_custom(hs[j], s));
         const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
         void* args[] = {&d_jobs, &n};
-        HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
+        if (wg > 0) HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
         HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, max_nacc, max_lds, s));
         return 0;
     } catch (const std::exception& e) {
@@ -1439,6 +1504,10 @@ extern "C" long long ctpwa_footprint_bytes(const char* plan_text, long long n_ph
         long long b = nt * event_bytes(M, nullptr);
         b += (long long)M.table_size() * 8 + (long long)(1 + 2 * M.P.namp + M.P.ntheta) * 1024 * 8;
         b += nt * (long long)M.customs.size() * M.ncf() * 8;     // Custom factor rows
+        // no free resonance parameter: the stored phase-space Gram and its build workspace
+        if (M.P.ntheta == 0)
+            b += (long long)M.P.namp * M.P.namp * 16 +
+                 agram_workspace_bytes(M.P, std::max(n_phsp, std::max(n_data, n_bkg)));
         // (the upload staging is one pool per device for the process, not per handle)
         return b;
     } catch (const std::exception& e) {
@@ -1465,7 +1534,8 @@ extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long lo
     int used = 0;
     for (int u : h->used_inv) used += u;
     if (bytes_per_eval) {
-        long long b = (h->n[0] + h->n[1] + h->n[2]) * event_bytes(h->M, &h->used_inv);
+        // with the stored phase-space Gram an evaluation reads data and background only
+        long long b = ((h->d_gram ? 0 : h->n[0]) + h->n[1] + h->n[2]) * event_bytes(h->M, &h->used_inv);
         for (int c = 1; c < 3; ++c) if (h->pw[c]) b += h->n[c] * 8;
         *bytes_per_eval = b;
     }

@@ -111,6 +111,11 @@ struct FinArgs {
     const double* params;
     double* out;             // [1 + 2 nfree + ntheta]
     double* totals;          // [3][nacc] scratch (kept for inspection)
+    // no free resonance parameter: the phase-space totals come from the stored amplitude
+    // Gram G[a][b] = sum_e sum_p conj(A'_a) A'_b (amplitudes without couplings or theta
+    // coefficients) and the table's c_a (offset off_c) instead of per-event partials
+    const double2* gram;     // [namp][namp], or NULL
+    long long off_c;
 };
 
 // LDS of the assembly kernel: C[nacc] | wcp[ncpe] | wth, vre, vim [nthe] (doubles), then

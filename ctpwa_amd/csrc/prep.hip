@@ -346,15 +346,48 @@ __global__ void __launch_bounds__(64) k_prep_batch(const double* const* __restri
 constexpr int RED_THREADS = 256;
 constexpr int ASM_THREADS = 256;
 
+// this thread's share of class c's total of accumulator column col: the class's
+// workgroup partials in a fixed order, or, for the phase space with a stored Gram (no free
+// resonance parameter), the column from G and c_a = v_a K_a:
+//   col 0: sum_a Re(conj(c_a) (G c)_a) = sum_e I(e);  col 1 + 2a / 2 + 2a: Re / Im (G c)_a
+// (sum_e conj(Psi_a sl_a) S, the fused kernel's accX_a); a tree sum over the workgroup
+// follows, so the result does not depend on timing
+__device__ __forceinline__ double class_share(const FinArgs& f, int c, int col) {
+    double s = 0.0;
+    if (c == 0 && f.gram) {
+        const int n = f.namp;
+        const double2* cv = (const double2*)(f.table + f.off_c);
+        if (col == 0) {
+            for (int a = threadIdx.x; a < n; a += RED_THREADS) {
+                double gr = 0.0, gi = 0.0;
+                for (int b = 0; b < n; ++b) {
+                    const double2 G = f.gram[(long long)a * n + b], cb = cv[b];
+                    gr += G.x * cb.x - G.y * cb.y;
+                    gi += G.x * cb.y + G.y * cb.x;
+                }
+                const double2 ca = cv[a];
+                s += ca.x * gr + ca.y * gi;
+            }
+        } else {
+            const int a = (col - 1) >> 1, im = (col - 1) & 1;
+            for (int b = threadIdx.x; b < n; b += RED_THREADS) {
+                const double2 G = f.gram[(long long)a * n + b], cb = cv[b];
+                s += im ? (G.x * cb.y + G.y * cb.x) : (G.x * cb.x - G.y * cb.y);
+            }
+        }
+        return s;
+    }
+    const double* src = f.part + (long long)col * f.nwg;
+    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
+    return s;
+}
+
 // stage 1: one workgroup per (class, accumulator column); fixed summation order
 __global__ void __launch_bounds__(RED_THREADS) k_reduce(FinArgs f) {
     __shared__ double sh[RED_THREADS];
     int pair = blockIdx.x;
     int c = pair / f.nacc, col = pair - c * f.nacc;
-    const double* src = f.part + (long long)col * f.nwg;
-    double s = 0.0;
-    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
-    sh[threadIdx.x] = s;
+    sh[threadIdx.x] = class_share(f, c, col);
     __syncthreads();
     for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
@@ -369,10 +402,7 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __r
     __shared__ double sh[RED_THREADS];
     int pair = blockIdx.x;
     int c = pair / f.nacc, col = pair - c * f.nacc;
-    const double* src = f.part + (long long)col * f.nwg;
-    double s = 0.0;
-    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
-    sh[threadIdx.x] = s;
+    sh[threadIdx.x] = class_share(f, c, col);
     __syncthreads();
     for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
@@ -482,10 +512,7 @@ __global__ void __launch_bounds__(ASM_THREADS) k_finalize(FinArgs f, unsigned in
     __shared__ int last;
     const int pair = blockIdx.x;
     const int c = pair / f.nacc, col = pair - c * f.nacc;
-    const double* src = f.part + (long long)col * f.nwg;
-    double s = 0.0;
-    for (int g = f.g0[c] + threadIdx.x; g < f.g1[c]; g += RED_THREADS) s += src[g];
-    sh[threadIdx.x] = s;
+    sh[threadIdx.x] = class_share(f, c, col);
     __syncthreads();
     for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
         if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
@@ -564,15 +591,15 @@ __global__ void k_amplitudes(AuxArgs a, long long e0, long long n, double* __res
 
 // amplitude rows for Grams of the amplitudes (fit fractions, interference): column-major
 // X[col][R], R = p * n + i (polarisation p of event e0 + i), columns Re A_a (a < namp)
-// then Im A_a; A_a = K_a Psi_b sl_u,p without couplings
-__global__ void k_amp_rows(AuxArgs a, long long e0, long long n, double* __restrict__ X, long long ldx) {
+// then Im A_a; A_a = K_a Psi_b sl_u,p without couplings (withK == 0: Psi_b sl_u,p)
+__global__ void k_amp_rows(AuxArgs a, long long e0, long long n, double* __restrict__ X, long long ldx, int withK) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     long long e = e0 + i;
     for (int am = 0; am < a.namp; ++am) {
         const DAmp& A = a.amps[am];
         double2 psi = block_psi(a, A.block, e);
-        double K = a.table[a.off_rec + (long long)A.rec * REC];
+        double K = withK ? a.table[a.off_rec + (long long)A.rec * REC] : 1.0;
         for (int p = 0; p < a.npol; ++p) {
             double2 s = ev_sl(a.st, A.u, p, a.npol, e);
             X[(long long)am * ldx + p * n + i] = K * (psi.x * s.x - psi.y * s.y);
@@ -747,9 +774,10 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finaliz
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_amp_rows(const ctpwa::AuxArgs* a, long long e0,
                                                                                   long long n, double* X, long long ldx,
-                                                                                  hipStream_t stream) {
+                                                                                  int withK, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ctpwa::k_amp_rows, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, e0, n, X, ldx);
+    hipLaunchKernelGGL(ctpwa::k_amp_rows, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, *a, e0, n, X, ldx,
+                       withK);
     return hipGetLastError();
 }
 

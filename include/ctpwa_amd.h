@@ -94,7 +94,9 @@ int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* hess_dev, v
  * its events, same summation order).  Replaces the per-fit NLLFunction::forward/backward
  * calls (src/Analysis.cu:625-1261) when a service runs many fits at once; the reference
  * has no batched form.  A batch object owns pinned and device descriptor space for up to
- * `max_jobs` likelihoods; calls on one batch object must not overlap. */
+ * `max_jobs` likelihoods; calls on one batch object must not overlap.  Each likelihood
+ * of a call has a handle of its own (a handle holds one coefficient table and one set of
+ * partial sums); a handle named twice fails the call. */
 typedef struct ctpwa_batch ctpwa_batch;
 int ctpwa_batch_create(int device, int max_jobs, ctpwa_batch** out);
 void ctpwa_batch_destroy(ctpwa_batch* b);
@@ -155,7 +157,11 @@ long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long lo
 long long ctpwa_hessian_workspace_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg);
 
 /* number of events of a class, and algorithmic bytes the fused kernel must read per
- * evaluation (SL tensors + invariants + per-event weights) */
+ * evaluation (SL tensors + invariants + per-event weights), and its workgroup count.  With
+ * no free resonance parameter the phase-space term of every evaluation is the quadratic
+ * form of the phase-space amplitude Gram built at ctpwa_create (the reference's stored
+ * phase-space matrix, src/Analysis.cu:808-826, src/ComputeNLL.cu:575-597): the fused
+ * kernel then reads the data and background events only, and so do these counts */
 int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long long* bytes_per_eval,
                  int* n_sl_tensors, int* n_invariants, int* n_workgroups);
 
