@@ -80,6 +80,7 @@ def cpu_baseline(cfg_path, frac, threads, hessian=True):
         O.nll_grad(p)
         if k >= 2:
             ts.append(time.perf_counter() - t)
+        log("[bench] cpu baseline evaluation %d: %.2f s" % (k, time.perf_counter() - t))
     t_sample = float(np.median(ts))
     t_full = t_sample * full / n
     try:
@@ -94,10 +95,12 @@ def cpu_baseline(cfg_path, frac, threads, hessian=True):
                      % (threads, "the whole workload" if frac >= 1.0 else "the first %.0f%% of every sample" % (100 * frac),
                         n, full, t_sample, "" if frac >= 1.0 else ", scaled linearly in events to the full workload")}
     if hessian:
+        log("[bench] cpu baseline Hessian")
         t = time.perf_counter()
         O.hessian(pts[-1])
         th = (time.perf_counter() - t) * full / n
         out["hessian_ms"] = round(th * 1e3, 1)
+        log("[bench] cpu baseline Hessian: %.2f s" % (time.perf_counter() - t))
     return out
 
 
@@ -162,12 +165,15 @@ def run_form(ana, args, spec, torch, W, prec, sizes, label=None, hessian=True):
             for line in f:
                 if line.startswith("bytes_per_launch"):
                     traffic = int(line.split("=")[1])
+    pols = "%d stored polarisation%s%s" % (lay["n_pol_stored"], "" if lay["n_pol_stored"] == 1 else "s",
+                                           " (the other is its mirror image)" if lay["n_pol_stored"] < plan.npol else "")
+    scope = "" if label == prec else "; data and background events only (phase space from the stored Gram)"
     if prec == "mixed":
-        bdef = ("per event: %d SL tensors x %d polarisations x 8 B (FP32 complex) + squared masses x 8 B "
-                "+ other invariants x 4 B" % (lay["n_sl_tensors"], plan.npol))
+        bdef = ("per event: %d SL tensors x %s x 8 B (FP32 complex) + squared masses x 8 B "
+                "+ other invariants x 4 B%s" % (lay["n_sl_tensors"], pols, scope))
     else:
-        bdef = ("per event: %d SL tensors x %d polarisations x 16 B + %d invariants x 8 B"
-                % (lay["n_sl_tensors"], plan.npol, lay["n_invariants"]))
+        bdef = ("per event: %d SL tensors x %s x 16 B + %d invariants x 8 B%s"
+                % (lay["n_sl_tensors"], pols, lay["n_invariants"], scope))
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "ctpwa_eval", "kernel_ms": round(kern_ms, 5),
@@ -287,30 +293,45 @@ def run_hessian_wide(args, W, torch, precisions):
     return res
 
 
+def _serve_once(plan, samples, x_nom, prec, method, rate, max_evals, args):
+    from ctpwa_amd.serve import FitService, toy_ensemble
+    jobs = toy_ensemble(plan, samples["phsp"], samples["data"], args.serving_jobs, seed=3, rate=rate,
+                        x_nominal=x_nom, max_evals=max_evals)
+    svc = FitService(plan.to_text(), precision=prec, max_slots=args.serving_slots, method=method)
+    st = svc.run(jobs)
+    svc.close()
+    sm = st.summary()
+    sm["arrival_rate_per_s"] = rate
+    sm["arrival_span_s"] = max(j.arrival for j in jobs)
+    log("[bench] serving %s %s at %.1f fits/s: %d fits (%d converged, median %d evaluations, %d Hessians) in %.2f s, "
+        "%.0f fits/h, request p50 %.2f ms p99 %.2f ms, fit p50 %.2f s p99 %.2f s, mean batch %.1f"
+        % (method, prec, rate, sm["fits"], sm["converged"], sm["evals_per_fit_median"], sm["hessians"], sm["wall_s"],
+           sm["fits_per_hour"], sm["latency_p50_ms"], sm["latency_p99_ms"], sm["fit_latency_p50_s"],
+           sm["fit_latency_p99_s"], sm["mean_batch"]))
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sm.items()}
+
+
 def run_serving(args, samples, plan, spec, W, precisions):
     """Toy-MC ensemble through the fit service: data counts log-uniform in [5k, 500k],
-    phase space 5x data, Poisson arrivals; every fit is a bounded Newton minimisation with
-    the GPU Hessian (or L-BFGS) whose NLL+gradient requests the service batches across the
-    active fits, run until EDM < 1e-5 or 400 evaluations."""
-    from ctpwa_amd.serve import FitService, toy_ensemble
+    phase space 5x data, Poisson arrivals (seed 3).  Each minimiser runs twice per form: at
+    a burst rate above the service's capacity (fits/hour is then the capacity) and at half
+    the capacity that burst measured (a stream the service keeps up with).  L-BFGS fits
+    issue NLL+gradient requests until the projected gradient is below 1e-5 or their request
+    budget (200) is spent; Newton fits use the GPU Hessian and stop at EDM < 1e-5."""
     x_nom = W.nominal_params(plan, spec["seed"])
-    res = {"jobs": args.serving_jobs, "arrival_rate_per_s": args.serving_rate, "slots": args.serving_slots,
-           "method": args.serving_method, "stop": "EDM < 1e-5 (Newton) or 400 evaluations",
-           "seed": 3, "data_per_fit": "log-uniform [5000, 500000]", "phsp_per_fit": "5 x data",
-           "events": "windows of the benchmark's data and phase-space samples"}
-    for prec in precisions:
-        jobs = toy_ensemble(plan, samples["phsp"], samples["data"], args.serving_jobs, seed=3,
-                            rate=args.serving_rate, x_nominal=x_nom)
-        svc = FitService(plan.to_text(), precision=prec, max_slots=args.serving_slots, method=args.serving_method)
-        st = svc.run(jobs)
-        svc.close()
-        sm = st.summary()
-        res[prec] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in sm.items()}
-        log("[bench] serving %s: %d fits (%d converged, median %d evaluations, %d Hessians) in %.1f s, %.0f fits/h, "
-            "evaluation p50 %.2f ms p99 %.2f ms, fit p50 %.2f s p99 %.2f s, mean batch %.1f"
-            % (prec, sm["fits"], sm["converged"], sm["evals_per_fit_median"], sm["hessians"], sm["wall_s"],
-               sm["fits_per_hour"], sm["latency_p50_ms"], sm["latency_p99_ms"], sm["fit_latency_p50_s"],
-               sm["fit_latency_p99_s"], sm["mean_batch"]))
+    res = {"jobs": args.serving_jobs, "slots": args.serving_slots, "seed": 3,
+           "data_per_fit": "log-uniform [5000, 500000]", "phsp_per_fit": "5 x data",
+           "events": "windows of the benchmark's data and phase-space samples",
+           "latency_def": "per NLL+gradient request: from the fit issuing its next point to the result on the host",
+           "rates": "above: burst at --serving-rate; below: half the fits/s the burst measured"}
+    for method in [m for m in args.serving_methods.split(",") if m]:
+        max_evals = args.serving_lbfgs_evals if method == "lbfgs" else 400
+        res[method] = {"max_evals": max_evals}
+        for prec in precisions:
+            above = _serve_once(plan, samples, x_nom, prec, method, args.serving_rate, max_evals, args)
+            cap = above["fits_per_hour"] / 3600.0
+            below = _serve_once(plan, samples, x_nom, prec, method, max(0.5 * cap, 1e-3), max_evals, args)
+            res[method][prec] = {"above_capacity": above, "below_capacity": below}
     return res
 
 
@@ -332,10 +353,12 @@ def main():
     ap.add_argument("--serving-jobs", type=int, default=256,
                     help="toy fits run through the fit service after the timing (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=1000.0,
-                    help="Poisson arrival rate of the toy fits per second (above the service's capacity, so "
-                         "fits/hour measures the service)")
-    ap.add_argument("--serving-method", choices=["auto", "newton", "lbfgs"], default="auto",
-                    help="minimiser of the toy fits (auto: Newton with the GPU Hessian)")
+                    help="Poisson arrival rate of the burst run (above the service's capacity, so fits/hour "
+                         "measures the service); a second run streams at half the measured capacity")
+    ap.add_argument("--serving-methods", default="lbfgs,newton",
+                    help="minimisers of the toy fits, comma-separated (lbfgs, newton)")
+    ap.add_argument("--serving-lbfgs-evals", type=int, default=200,
+                    help="NLL+gradient request budget of an L-BFGS toy fit")
     ap.add_argument("--serving-slots", type=int, default=64, help="fits evaluated together at most")
     ap.add_argument("--hessian-reps", type=int, default=10,
                     help="getHessian calls timed per form on the same workload (0: skip)")

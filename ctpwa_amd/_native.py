@@ -60,7 +60,7 @@ def lib():
     L.ctpwa_hessian_workspace_bytes.restype = _ll
     L.ctpwa_hessian_workspace_bytes.argtypes = [ctypes.c_char_p, _ll, _ll, _ll]
     L.ctpwa_layout.argtypes = [_vp, ctypes.POINTER(_ll), ctypes.POINTER(_ll), ctypes.POINTER(ctypes.c_int),
-                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.ctpwa_kernel_source.restype = ctypes.c_char_p
     L.ctpwa_kernel_source.argtypes = [_vp]
     L.ctpwa_generate_source.restype = _ll
@@ -251,11 +251,11 @@ class Handle:
     def layout(self):
         n3 = (_ll * 3)()
         b = _ll()
-        nu, ni, nw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nu, ni, nw, nps = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().ctpwa_layout(self.h, n3, ctypes.byref(b), ctypes.byref(nu), ctypes.byref(ni),
-                                 ctypes.byref(nw)), "ctpwa_layout")
+                                 ctypes.byref(nw), ctypes.byref(nps)), "ctpwa_layout")
         return dict(n_events=list(n3), bytes_per_eval=b.value, n_sl_tensors=nu.value,
-                    n_invariants=ni.value, n_workgroups=nw.value)
+                    n_invariants=ni.value, n_workgroups=nw.value, n_pol_stored=nps.value)
 
     def kernel_source(self) -> str:
         return lib().ctpwa_kernel_source(self.h).decode()

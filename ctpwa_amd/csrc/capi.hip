@@ -5,6 +5,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,9 +36,9 @@ extern "C" hipError_t ctpwa_launch_polsym(const double2*, long long, long long, 
                                           hipStream_t);
 extern "C" hipError_t ctpwa_launch_prep(const double*, double*, const ctpwa::PrepArgs*, int, int, hipStream_t);
 extern "C" hipError_t ctpwa_launch_finalize(const ctpwa::FinArgs*, unsigned int*, hipStream_t);
-extern "C" hipError_t ctpwa_launch_prep_batch(const double* const*, double* const*, const ctpwa::PrepArgs*, int, int,
-                                              hipStream_t);
-extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, int, size_t, hipStream_t);
+extern "C" hipError_t ctpwa_launch_prep_batch(const double* const*, double* const*, const ctpwa::PrepArgs*, int,
+                                              const ctpwa::PrepArgs*, int, const int*, hipStream_t);
+extern "C" hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs*, int, int, size_t, const int*, hipStream_t);
 extern "C" hipError_t ctpwa_launch_aux(const ctpwa::AuxArgs*, long long, long long, double*, int, hipStream_t);
 
 
@@ -254,8 +255,13 @@ static long long gram_part_doubles(int ncol) {
     return (long long)ctpwa_gram_nwg(npp) * ntp * 256;
 }
 
+// serial numbers of handles: a handle created after another was destroyed may reuse its
+// address, never its serial
+static std::atomic<unsigned long long> g_handle_serial{0};
+
 struct ctpwa_handle {
     int device = 0;
+    const unsigned long long serial = ++g_handle_serial;
     Model_ M;
     long long n[3] = {0, 0, 0}, off[3] = {0, 0, 0}, ntot = 0;
     double Wd = 0, Wb = 0;
@@ -277,6 +283,7 @@ struct ctpwa_handle {
     hipModule_t mod = nullptr;
     hipFunction_t fn_eval = nullptr;       // one likelihood
     hipFunction_t fn_group = nullptr;      // many likelihoods of this model in one launch
+    hipFunction_t fn_group_p = nullptr;    // the same, persistent form (graph replays)
     hipFunction_t fn_aux = nullptr;        // amplitudes / intensities (models with Custom lineshapes)
     hipFunction_t fn_custom = nullptr;     // Custom factors and derivatives per event (models with Custom lineshapes)
     double* cfb = nullptr;                 // their per-event rows, [ncustom * ncf][ntot]
@@ -885,6 +892,7 @@ extern "C" int ctpwa_create(const char* plan_text, const double* phsp, long long
         h->mod = compile_eval(device, h->src);
         HIPCHECK(hipModuleGetFunction(&h->fn_eval, h->mod, "ctpwa_eval"));
         HIPCHECK(hipModuleGetFunction(&h->fn_group, h->mod, "ctpwa_eval_group"));
+        HIPCHECK(hipModuleGetFunction(&h->fn_group_p, h->mod, "ctpwa_eval_group_persist"));
         if (!M.customs.empty()) {
             HIPCHECK(hipModuleGetFunction(&h->fn_aux, h->mod, "ctpwa_aux"));
             HIPCHECK(hipModuleGetFunction(&h->fn_custom, h->mod, "ctpwa_custom"));
@@ -1378,10 +1386,28 @@ struct ctpwa_batch {
     int cap = 0;
     size_t bytes = 0;
     char* host = nullptr;          // pinned staging of the descriptors
-    char* dev = nullptr;           // device copy: [EvalJob cap][FinArgs cap][params ptr cap][table ptr cap]
+    // device copy: [header: job count, task count | task counters][EvalJob cap][FinArgs cap]
+    // [params ptr cap][table ptr cap][PrepArgs cap]
+    char* dev = nullptr;
     hipEvent_t copied = nullptr;   // the staging buffer may be rewritten once this completes
     bool pending = false;
-    size_t o_jobs = 0, o_fins = 0, o_params = 0, o_tables = 0;
+    size_t o_hdr = 0, o_ctr = 0, o_jobs = 0, o_fins = 0, o_params = 0, o_tables = 0, o_preps = 0;
+    int max_nacc = 0;              // of the uploaded list
+    size_t max_lds = 0;
+    // Graph replays (opt-in, environment CTPWA_BATCH_GRAPH=1): the descriptors on the device
+    // belong to `key`, a list of (handle serial, params, out); a call with the same list
+    // reuses them without an upload (serials, not handle addresses: a new handle may take a
+    // destroyed one's address) and replays a captured graph of the launches, one per
+    // power-of-two row bucket per model module.  Every argument a captured launch holds is
+    // in the descriptors or is a constant of the model (each job's prep arguments included:
+    // they point into that job's own handle).
+    bool use_graphs = false;
+    std::vector<unsigned long long> key;
+    int rows = 0;
+    hipStream_t cs = nullptr;      // capture stream
+    hipModule_t gmod = nullptr;
+    std::map<int, hipGraphExec_t> execs;
+    int persist = 0;               // workgroups of the persistent grouped kernel
 };
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -1395,33 +1421,65 @@ extern "C" int ctpwa_batch_create(int device, int max_jobs, ctpwa_batch** out) {
     try {
         b->device = device;
         b->cap = max_jobs;
-        b->o_jobs = 0;
-        b->o_fins = align256(sizeof(EvalJob) * max_jobs);
+        b->o_hdr = 0;
+        b->o_ctr = 256;
+        b->o_jobs = 512;
+        b->o_fins = b->o_jobs + align256(sizeof(EvalJob) * max_jobs);
         b->o_params = b->o_fins + align256(sizeof(FinArgs) * max_jobs);
         b->o_tables = b->o_params + align256(sizeof(void*) * max_jobs);
-        b->bytes = b->o_tables + align256(sizeof(void*) * max_jobs);
+        b->o_preps = b->o_tables + align256(sizeof(void*) * max_jobs);
+        b->bytes = b->o_preps + align256(sizeof(PrepArgs) * max_jobs);
+        const char* e = std::getenv("CTPWA_BATCH_GRAPH");
+        b->use_graphs = e && std::string(e) == "1";
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipHostMalloc((void**)&b->host, b->bytes, hipHostMallocDefault));
+        std::memset(b->host, 0, b->bytes);
         HIPCHECK(hipMalloc((void**)&b->dev, b->bytes));
+        // the task counters start at zero; the persistent grouped kernel leaves them at zero
+        HIPCHECK(hipMemset(b->dev, 0, 512));
         HIPCHECK(hipEventCreateWithFlags(&b->copied, hipEventDisableTiming));
+        if (b->use_graphs) HIPCHECK(hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking));
         *out = b;
         return 0;
     } catch (const std::exception& e) {
         if (b->host) (void)hipHostFree(b->host);
         if (b->dev) (void)hipFree(b->dev);
+        if (b->copied) (void)hipEventDestroy(b->copied);
         delete b;
         return fail(std::string("ctpwa_batch_create: ") + e.what());
     }
 }
 
+static void batch_drop_graphs(ctpwa_batch* b) {
+    for (auto& kv : b->execs) (void)hipGraphExecDestroy(kv.second);
+    b->execs.clear();
+}
+
 extern "C" void ctpwa_batch_destroy(ctpwa_batch* b) {
     if (!b) return;
     DeviceGuard g(b->device);
-    if (b->pending) (void)hipEventSynchronize(b->copied);
+    if (!b->execs.empty()) (void)hipDeviceSynchronize();   // no replay still reads the buffers
+    else if (b->pending) (void)hipEventSynchronize(b->copied);
+    batch_drop_graphs(b);
+    if (b->cs) (void)hipStreamDestroy(b->cs);
     (void)hipEventDestroy(b->copied);
     (void)hipHostFree(b->host);
     (void)hipFree(b->dev);
     delete b;
+}
+
+// graph replays: the launches over the uploaded descriptors with the job and task counts
+// read on the device, `rows` (>= the job count) workgroup rows of prep and reduction, the
+// persistent grouped kernel, each job's own prep arguments
+static void batch_launches_graph(ctpwa_batch* b, const ctpwa_handle* h0, int rows, hipStream_t s) {
+    const int* d_hdr = (const int*)(b->dev + b->o_hdr);
+    unsigned int* d_ctr = (unsigned int*)(b->dev + b->o_ctr);
+    HIPCHECK(ctpwa_launch_prep_batch((const double* const*)(b->dev + b->o_params), (double* const*)(b->dev + b->o_tables),
+                                     &h0->prep, h0->M.maxloc, (const PrepArgs*)(b->dev + b->o_preps), rows, d_hdr, s));
+    const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
+    void* args[] = {&d_jobs, &d_hdr, &d_ctr};
+    HIPCHECK(hipModuleLaunchKernel(h0->fn_group_p, b->persist, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
+    HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), rows, b->max_nacc, b->max_lds, d_hdr, s));
 }
 
 extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, const double* const* params_dev,
@@ -1433,53 +1491,119 @@ extern "C" int ctpwa_batch_nll_grad(ctpwa_batch* b, ctpwa_handle* const* hs, con
     hipStream_t s = (hipStream_t)stream;
     try {
         const ctpwa_handle* h0 = hs[0];
-        int max_nacc = 0;
-        size_t max_lds = 0;
         for (int j = 0; j < n; ++j) {
             const ctpwa_handle* h = hs[j];
             if (!h || !params_dev[j] || !out_dev[j]) throw std::runtime_error("NULL handle, parameter or output pointer");
             if (h->device != b->device) throw std::runtime_error("a handle lives on another device than the batch");
             if (h->mod != h0->mod) throw std::runtime_error("the handles of one batch must share one model and precision");
-            max_nacc = std::max(max_nacc, h->nacc);
-            max_lds = std::max(max_lds, assemble_lds_bytes(h->fin));
         }
-        {
-            // a handle holds one coefficient table and one set of partial sums
-            std::vector<const ctpwa_handle*> seen(hs, hs + n);
-            std::sort(seen.begin(), seen.end());
-            if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
-                throw std::runtime_error("a handle appears twice in one batch (each job needs a handle of its own)");
+        // graph replays only for models without Custom lineshapes (those launch one factor
+        // kernel per job)
+        const bool graph = b->use_graphs && h0->M.customs.empty();
+        std::vector<unsigned long long> key;
+        if (graph) {
+            key.resize((size_t)3 * n);
+            for (int j = 0; j < n; ++j) {
+                key[3 * j] = hs[j]->serial;
+                key[3 * j + 1] = (unsigned long long)(uintptr_t)params_dev[j];
+                key[3 * j + 2] = (unsigned long long)(uintptr_t)out_dev[j];
+            }
         }
-        // the staging buffer is reused: wait for the previous upload from it
-        if (b->pending) HIPCHECK(hipEventSynchronize(b->copied));
-        EvalJob* jobs = (EvalJob*)(b->host + b->o_jobs);
-        FinArgs* fins = (FinArgs*)(b->host + b->o_fins);
-        const double** pp = (const double**)(b->host + b->o_params);
-        double** tp = (double**)(b->host + b->o_tables);
         int wg = 0;
-        for (int j = 0; j < n; ++j) {
-            ctpwa_handle* h = hs[j];
-            jobs[j] = EvalJob{h->table, h->slp, h->inv, h->invf, h->wts, h->part, h->wg_cls, h->wg_rng, h->cfb, h->ntot,
-                              h->wc[1], h->wc[2], h->nwg, h->pw[1], h->pw[2], wg};
-            wg += h->nwg;
-            fins[j] = h->fin;
-            fins[j].params = params_dev[j];
-            fins[j].out = out_dev[j];
-            pp[j] = params_dev[j];
-            tp[j] = h->table;
+        if (!graph || key != b->key) {
+            {
+                // a handle holds one coefficient table and one set of partial sums
+                std::vector<const ctpwa_handle*> seen(hs, hs + n);
+                std::sort(seen.begin(), seen.end());
+                if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+                    throw std::runtime_error("a handle appears twice in one batch (each job needs a handle of its own)");
+            }
+            b->key.clear();
+            // the staging buffer is reused: wait for the previous upload from it
+            if (b->pending) HIPCHECK(hipEventSynchronize(b->copied));
+            int* hdr = (int*)(b->host + b->o_hdr);
+            EvalJob* jobs = (EvalJob*)(b->host + b->o_jobs);
+            FinArgs* fins = (FinArgs*)(b->host + b->o_fins);
+            const double** pp = (const double**)(b->host + b->o_params);
+            double** tp = (double**)(b->host + b->o_tables);
+            PrepArgs* pa = (PrepArgs*)(b->host + b->o_preps);
+            int max_nacc = 0;
+            size_t max_lds = 0;
+            for (int j = 0; j < n; ++j) {
+                ctpwa_handle* h = hs[j];
+                max_nacc = std::max(max_nacc, h->nacc);
+                max_lds = std::max(max_lds, assemble_lds_bytes(h->fin));
+                jobs[j] = EvalJob{h->table, h->slp, h->inv, h->invf, h->wts, h->part, h->wg_cls, h->wg_rng, h->cfb,
+                                  h->ntot, h->wc[1], h->wc[2], h->nwg, h->pw[1], h->pw[2], wg};
+                wg += h->nwg;
+                fins[j] = h->fin;
+                fins[j].params = params_dev[j];
+                fins[j].out = out_dev[j];
+                pp[j] = params_dev[j];
+                tp[j] = h->table;
+                pa[j] = h->prep;
+            }
+            hdr[0] = n;
+            hdr[1] = wg;
+            // header and descriptors (the task counters stay as the kernel left them)
+            HIPCHECK(hipMemcpyAsync(b->dev + b->o_hdr, b->host + b->o_hdr, 2 * sizeof(int), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(b->dev + b->o_jobs, b->host + b->o_jobs, b->bytes - b->o_jobs, hipMemcpyHostToDevice, s));
+            HIPCHECK(hipEventRecord(b->copied, s));
+            b->pending = true;
+            b->max_nacc = max_nacc;
+            b->max_lds = max_lds;
+            if (graph) {
+                b->key.swap(key);
+                int rows = 1;
+                while (rows < n) rows <<= 1;
+                b->rows = std::min(rows, b->cap);
+            }
         }
-        HIPCHECK(hipMemcpyAsync(b->dev, b->host, b->bytes, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipEventRecord(b->copied, s));
-        b->pending = true;
-        HIPCHECK(ctpwa_launch_prep_batch((const double* const*)(b->dev + b->o_params), (double* const*)(b->dev + b->o_tables),
-                                         &h0->prep, h0->M.maxloc, n, s));
-        for (int j = 0; j < n; ++j) HIPCHECK(launch_custom(hs[j], s));
-        const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
-        void* args[] = {&d_jobs, &n};
-        if (wg > 0) HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
-        HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, max_nacc, max_lds, s));
+        if (!graph) {
+            // one batched prep launch, each job's Custom factors, one grouped launch whose
+            // workgroups are shared out among the jobs, one batched reduction and assembly
+            HIPCHECK(ctpwa_launch_prep_batch((const double* const*)(b->dev + b->o_params),
+                                             (double* const*)(b->dev + b->o_tables), &h0->prep, h0->M.maxloc, nullptr, n,
+                                             nullptr, s));
+            for (int j = 0; j < n; ++j) HIPCHECK(launch_custom(hs[j], s));
+            const EvalJob* d_jobs = (const EvalJob*)(b->dev + b->o_jobs);
+            void* args[] = {&d_jobs, &n};
+            if (wg > 0) HIPCHECK(hipModuleLaunchKernel(h0->fn_group, wg, 1, 1, h0->M.wgs, 1, 1, 0, s, args, nullptr));
+            HIPCHECK(ctpwa_launch_finalize_batch((const FinArgs*)(b->dev + b->o_fins), n, b->max_nacc, b->max_lds, nullptr,
+                                                 s));
+            return 0;
+        }
+        if (b->gmod != h0->mod) {
+            // a new model: its grouped kernel's persistent grid, and no graph of another model
+            batch_drop_graphs(b);
+            int per_cu = 0, ncu = 0;
+            HIPCHECK(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, h0->fn_group_p, h0->M.wgs, 0));
+            HIPCHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+            b->persist = std::max(1, per_cu) * std::max(1, ncu);
+            b->gmod = h0->mod;
+        }
+        auto it = b->execs.find(b->rows);
+        if (it == b->execs.end()) {
+            hipGraph_t graph_ = nullptr;
+            hipGraphExec_t exec = nullptr;
+            HIPCHECK(hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal));
+            try {
+                batch_launches_graph(b, h0, b->rows, b->cs);
+            } catch (...) {
+                (void)hipStreamEndCapture(b->cs, &graph_);
+                if (graph_) (void)hipGraphDestroy(graph_);
+                throw;
+            }
+            HIPCHECK(hipStreamEndCapture(b->cs, &graph_));
+            hipError_t e = hipGraphInstantiate(&exec, graph_, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph_);
+            HIPCHECK(e);
+            it = b->execs.emplace(b->rows, exec).first;
+        }
+        HIPCHECK(hipGraphLaunch(it->second, s));
         return 0;
     } catch (const std::exception& e) {
+        b->key.clear();                    // upload again on the next call
         return fail(std::string("ctpwa_batch_nll_grad: ") + e.what());
     }
 }
@@ -1528,7 +1652,7 @@ extern "C" long long ctpwa_hessian_workspace_bytes(const char* plan_text, long l
 }
 
 extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long long* bytes_per_eval, int* n_sl, int* n_inv,
-                            int* n_wg) {
+                            int* n_wg, int* n_pol_stored) {
     if (!h) return fail("ctpwa_layout: NULL handle");
     if (n_events3) for (int c = 0; c < 3; ++c) n_events3[c] = h->n[c];
     int used = 0;
@@ -1542,6 +1666,7 @@ extern "C" int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long lo
     if (n_sl) *n_sl = (int)h->M.us.size();
     if (n_inv) *n_inv = used;
     if (n_wg) *n_wg = h->nwg;
+    if (n_pol_stored) *n_pol_stored = h->M.npstore();
     return 0;
 }
 
@@ -1574,6 +1699,8 @@ extern "C" long long ctpwa_generate_hessian_source(const char* plan_text, int pr
         Model_ M = derive(parse_plan(plan_text));
         M.mixed = precision;
         std::string src = generate_hess_kernel(M, nullptr);
+        // models with Custom lineshapes: the second-order factor module follows
+        if (!M.customs.empty()) src += "\n// ---- module: ctpwa_custom2 ----\n" + generate_custom2_kernel(M);
         if (buf && cap > 0) {
             long long n = std::min<long long>(cap - 1, (long long)src.size());
             std::memcpy(buf, src.data(), (size_t)n);

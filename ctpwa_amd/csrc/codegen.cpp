@@ -573,10 +573,21 @@ std::string generate_eval_kernel(const Model_& M) {
          "  double e = __builtin_fma(-x, r, 1.0); r = __builtin_fma(r, e, r);\n"
          "  e = __builtin_fma(-x, r, 1.0); return __builtin_fma(r, e, r);\n}\n";
     o << "#define OFFC " << M.off_c() << "LL\n#define OFFR " << M.off_rec() << "LL\n#define REC " << Model_::REC << "\n";
+    // A wave-uniform double read as two 32-bit halves through readfirstlane: the value is
+    // known to be uniform and lives in scalar registers.  A load through a pointer that is
+    // a kernel argument becomes a scalar load by itself; one through a pointer read from
+    // memory (a job descriptor) does not, and the 40-odd coefficients then took vector
+    // registers and halved the grouped kernel's occupancy.
+    o << "__device__ __forceinline__ double uld(const double* p) {\n"
+         "  const unsigned long long b = *(const unsigned long long*)p;\n"
+         "  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);\n"
+         "  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));\n"
+         "  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);\n}\n";
     // one workgroup's share of the events: the body is shared by the single-likelihood
     // kernel and the grouped kernel that evaluates many likelihoods of this model in one
-    // launch, so a likelihood's sums do not depend on how it was launched
-    o << "__device__ __forceinline__ void eval_body(\n"
+    // launch, so a likelihood's sums do not depend on how it was launched (UL: how the
+    // coefficients are read, which does not change their values)
+    o << "template <bool UL>\n__device__ __forceinline__ void eval_body(\n"
          "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
@@ -612,14 +623,22 @@ std::string generate_eval_kernel(const Model_& M) {
          "  const double LOG_FLOOR = log(1e-10);\n";
     // theta-only coefficients (wave-uniform)
     o << "  const double* tb = tab;\n";
-    auto emit_coefs = [&](const char* ind) {
+    // ul: inside eval_body, whose UL instantiation (the grouped kernels, where the table
+    // pointer comes from a job descriptor in memory rather than a kernel argument) reads
+    // the coefficients through uld() so that they stay in scalar registers
+    auto emit_coefs = [&](const char* ind, bool ul) {
+        auto ld = [&](const std::string& x) {
+            return ul ? "(UL ? uld(tb + (" + x + ")) : tb[" + x + "])" : "tb[" + x + "]";
+        };
         for (int a = 0; a < NA; ++a)
-            o << ind << "const double c" << a << "r = tb[OFFC + " << 2 * a << "], c" << a << "i = tb[OFFC + " << 2 * a + 1 << "];\n";
+            o << ind << "const double c" << a << "r = " << ld("OFFC + " + std::to_string(2 * a)) << ", c" << a
+              << "i = " << ld("OFFC + " + std::to_string(2 * a + 1)) << ";\n";
         for (size_t d = 0; d < M.dens.size(); ++d)
             for (int c = 0; c < M.dens[d].ncoef; ++c)
-                o << ind << "const double k" << d << "_" << c << " = tb[OFFR + " << (long long)(M.dens[d].coef_base + c) << "LL * REC];\n";
+                o << ind << "const double k" << d << "_" << c << " = "
+                  << ld("OFFR + " + std::to_string((long long)(M.dens[d].coef_base + c)) + "LL * REC") << ";\n";
     };
-    emit_coefs("  ");
+    emit_coefs("  ", true);
     o << "  double av = 0.0;\n";
     for (int a = 0; a < NA; ++a) o << "  double ax" << a << "r = 0.0, ax" << a << "i = 0.0;\n";
     for (int t = 0; t < NTH; ++t) o << "  double at" << t << " = 0.0;\n";
@@ -888,7 +907,7 @@ std::string generate_eval_kernel(const Model_& M) {
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
          "    const int* __restrict__ wg_cls, const long long* __restrict__ wg_rng, long long ld, int nwg,\n"
          "    double w1, double w2, int pw1, int pw2, const double* __restrict__ cfb)\n{\n" << decl.str() <<
-         "  eval_body(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red, lbuf, cfb);\n}\n";
+         "  eval_body<false>(tab, slv, inv, invf, wts, part, wg_cls, wg_rng, ld, nwg, w1, w2, pw1, pw2, blockIdx.x, red, lbuf, cfb);\n}\n";
     // grouped launch: jobs[j] owns workgroups [wg_begin_j, wg_begin_{j+1}); the layout of
     // this struct is mirrored by EvalJob in capi.hip
     o << "struct EvalJob {\n"
@@ -900,8 +919,34 @@ std::string generate_eval_kernel(const Model_& M) {
          "  int lo = 0, hi = njobs - 1;\n"
          "  while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (jobs[mid].wg_begin <= (int)blockIdx.x) lo = mid; else hi = mid - 1; }\n"
          "  const EvalJob J = jobs[lo];\n"
-         "  eval_body(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
+         "  eval_body<true>(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
          "            (int)blockIdx.x - J.wg_begin, red, lbuf, J.cf);\n}\n";
+    // persistent form (graph replays): hdr = {job count, total workgroup tasks}; each
+    // workgroup takes tasks from the counter ctr[0] until none is left, and the last
+    // workgroup to leave (ctr[1]) clears both counters for the next launch.  A task computes
+    // exactly what the workgroup of that index computes in ctpwa_eval, so results do not
+    // depend on which workgroup runs it; the job set can change without changing the launch.
+    o << "extern \"C\" __global__ void " << lb << " ctpwa_eval_group_persist(const EvalJob* __restrict__ jobs,\n"
+         "    const int* __restrict__ hdr, unsigned int* __restrict__ ctr)\n{\n"
+      << decl.str() <<
+         "  __shared__ int task;\n"
+         "  const int njobs = hdr[0], total = hdr[1];\n"
+         "  for (;;) {\n"
+         "    __syncthreads();                  // the previous task's reads of red / lbuf are done\n"
+         "    if (threadIdx.x == 0) task = (int)atomicAdd(&ctr[0], 1u);\n"
+         "    __syncthreads();\n"
+         "    const int t = task;\n"
+         "    if (t >= total) break;\n"
+         "    int lo = 0, hi = njobs - 1;\n"
+         "    while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (jobs[mid].wg_begin <= t) lo = mid; else hi = mid - 1; }\n"
+         "    const EvalJob J = jobs[lo];\n"
+         "    eval_body<true>(J.tab, J.sl, J.inv, J.invf, J.wts, J.part, J.wg_cls, J.wg_rng, J.ld, J.nwg, J.w1, J.w2, J.pw1, J.pw2,\n"
+         "              t - J.wg_begin, red, lbuf, J.cf);\n"
+         "  }\n"
+         "  if (threadIdx.x == 0) {\n"
+         "    __threadfence();\n"
+         "    if (atomicAdd(&ctr[1], 1u) == gridDim.x - 1) { atomicExch(&ctr[0], 0u); atomicExch(&ctr[1], 0u); }\n"
+         "  }\n}\n";
     if (!M.customs.empty()) {
         // Custom factors and their derivatives for every stored event at the current
         // parameter point, rows [c * NCF + (r, i, dr[0..NCV), di[0..NCV))] of cfb.  A kernel
@@ -935,7 +980,7 @@ std::string generate_eval_kernel(const Model_& M) {
              "    const float* __restrict__ invf, long long ld, long long e0, long long n, double* __restrict__ out,\n"
              "    int intensity, const double* __restrict__ cfb)\n{\n";
         o << "  const " << SLT << "* __restrict__ sl = (const " << SLT << "*)slv;\n  const double* tb = tab;\n";
-        emit_coefs("  ");
+        emit_coefs("  ", false);
         o << "  const long long ie = (long long)blockIdx.x * 256 + threadIdx.x;\n  if (ie >= n) return;\n"
              "  const long long e = e0 + ie;\n  {\n";
         for (int u = 0; u < NU; ++u)

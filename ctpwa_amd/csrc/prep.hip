@@ -334,8 +334,14 @@ __global__ void __launch_bounds__(64) k_prep(const double* __restrict__ params, 
 // many likelihoods of one model: blockIdx.y selects the likelihood (its parameters and
 // coefficient table); the model tables in `a` are the same for all of them
 template <int NV, bool H>
+// one workgroup row per job.  hdr (graph replays; NULL otherwise): hdr[0] is the number
+// of jobs, rows at or beyond it leave at once, and each job's prep arguments come from
+// args[row] (its own handle's model tables) instead of `a`
 __global__ void __launch_bounds__(64) k_prep_batch(const double* const* __restrict__ params, double* const* __restrict__ tables,
-                                                   PrepArgs a) {
+                                                   PrepArgs a, const PrepArgs* __restrict__ args, const int* __restrict__ hdr) {
+    if (hdr && (int)blockIdx.y >= hdr[0]) return;
+    if (args) a = args[blockIdx.y];
+    a.remember = 1;                 // NLL evaluations: each table keeps its theta as the last evaluated one
     prep_body<NV, H>(params[blockIdx.y], tables[blockIdx.y], a);
 }
 
@@ -396,7 +402,8 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(FinArgs f) {
     if (threadIdx.x == 0) f.totals[pair] = sh[0];
 }
 
-__global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __restrict__ fins) {
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __restrict__ fins, const int* __restrict__ hdr) {
+    if (hdr && (int)blockIdx.y >= hdr[0]) return;
     const FinArgs& f = fins[blockIdx.y];
     if ((int)blockIdx.x >= 3 * f.nacc) return;      // the whole workgroup leaves together
     __shared__ double sh[RED_THREADS];
@@ -530,7 +537,8 @@ __global__ void __launch_bounds__(ASM_THREADS) k_finalize(FinArgs f, unsigned in
     if (threadIdx.x == 0) *done = 0u;
 }
 
-__global__ void __launch_bounds__(ASM_THREADS) k_assemble_batch(const FinArgs* __restrict__ fins) {
+__global__ void __launch_bounds__(ASM_THREADS) k_assemble_batch(const FinArgs* __restrict__ fins, const int* __restrict__ hdr) {
+    if (hdr && (int)blockIdx.x >= hdr[0]) return;
     assemble_body(fins[blockIdx.x]);
 }
 
@@ -732,35 +740,38 @@ extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep(co
 
 template <int NV, bool H>
 static hipError_t launch_prep_batch_t(const double* const* params, double* const* tables, const ctpwa::PrepArgs* a,
-                                      int njobs, hipStream_t stream) {
+                                      const ctpwa::PrepArgs* args, int rows, const int* hdr, hipStream_t stream) {
     int items = a->nden + a->namp;
-    ctpwa::PrepArgs b = *a;
-    b.remember = 1;                 // NLL evaluations: each table keeps its theta as the last evaluated one
-    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3(items, njobs), dim3(64), (size_t)a->meta_bytes, stream, params,
-                       tables, b);
+    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3(items, rows), dim3(64), (size_t)a->meta_bytes, stream, params,
+                       tables, *a, args, hdr);
     return hipGetLastError();
 }
 
+// rows: workgroup rows launched (the job count, or with hdr a bound on the job count hdr[0]
+// on the device, extra rows leave); args: NULL, or the jobs' own prep arguments on the
+// device (a then only gives the launch shape)
 extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_prep_batch(const double* const* params,
                                                                                     double* const* tables,
                                                                                     const ctpwa::PrepArgs* a, int nv,
-                                                                                    int njobs, hipStream_t stream) {
-    if (njobs <= 0) return hipSuccess;
-    if (nv <= 1) return launch_prep_batch_t<1, false>(params, tables, a, njobs, stream);
-    if (nv <= 2) return launch_prep_batch_t<2, false>(params, tables, a, njobs, stream);
-    if (nv <= 4) return launch_prep_batch_t<4, false>(params, tables, a, njobs, stream);
-    return launch_prep_batch_t<8, false>(params, tables, a, njobs, stream);
+                                                                                    const ctpwa::PrepArgs* args, int rows,
+                                                                                    const int* hdr, hipStream_t stream) {
+    if (rows <= 0) return hipSuccess;
+    if (nv <= 1) return launch_prep_batch_t<1, false>(params, tables, a, args, rows, hdr, stream);
+    if (nv <= 2) return launch_prep_batch_t<2, false>(params, tables, a, args, rows, hdr, stream);
+    if (nv <= 4) return launch_prep_batch_t<4, false>(params, tables, a, args, rows, hdr, stream);
+    return launch_prep_batch_t<8, false>(params, tables, a, args, rows, hdr, stream);
 }
 
-// fins: device array [njobs]; max_nacc: the largest accumulator count among them
-extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs* fins, int njobs,
+// fins: device array of the jobs' arguments; rows and hdr as above; max_nacc: the largest
+// accumulator count among the jobs
+extern "C" __attribute__((visibility("hidden"))) hipError_t ctpwa_launch_finalize_batch(const ctpwa::FinArgs* fins, int rows,
                                                                                         int max_nacc, size_t lds_bytes,
-                                                                                        hipStream_t stream) {
-    if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ctpwa::k_reduce_batch, dim3(3 * max_nacc, njobs), dim3(ctpwa::RED_THREADS), 0, stream, fins);
+                                                                                        const int* hdr, hipStream_t stream) {
+    if (rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ctpwa::k_reduce_batch, dim3(3 * max_nacc, rows), dim3(ctpwa::RED_THREADS), 0, stream, fins, hdr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(ctpwa::k_assemble_batch, dim3(njobs), dim3(ctpwa::ASM_THREADS), lds_bytes, stream, fins);
+    hipLaunchKernelGGL(ctpwa::k_assemble_batch, dim3(rows), dim3(ctpwa::ASM_THREADS), lds_bytes, stream, fins, hdr);
     return hipGetLastError();
 }
 

@@ -157,13 +157,15 @@ long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long lo
 long long ctpwa_hessian_workspace_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg);
 
 /* number of events of a class, and algorithmic bytes the fused kernel must read per
- * evaluation (SL tensors + invariants + per-event weights), and its workgroup count.  With
+ * evaluation (SL tensors + invariants + per-event weights), its workgroup count, and the
+ * polarisations stored per SL tensor (1 when the two polarisations of every SL tensor are
+ * mirror images, found at ctpwa_create; else the model's count).  With
  * no free resonance parameter the phase-space term of every evaluation is the quadratic
  * form of the phase-space amplitude Gram built at ctpwa_create (the reference's stored
  * phase-space matrix, src/Analysis.cu:808-826, src/ComputeNLL.cu:575-597): the fused
  * kernel then reads the data and background events only, and so do these counts */
 int ctpwa_layout(const ctpwa_handle* h, long long* n_events3, long long* bytes_per_eval,
-                 int* n_sl_tensors, int* n_invariants, int* n_workgroups);
+                 int* n_sl_tensors, int* n_invariants, int* n_workgroups, int* n_pol_stored);
 
 /* generated source of the fused kernel (NUL-terminated, owned by the handle) */
 const char* ctpwa_kernel_source(const ctpwa_handle* h);
@@ -174,7 +176,9 @@ long long ctpwa_generate_source(const char* plan_text, int precision, char* buf,
 
 /* source of the generated Hessian rows kernel for a plan (the per-event part of
  * analysis::getHessian, src/Analysis.cu:2689-3378), without a device; as
- * ctpwa_generate_source */
+ * ctpwa_generate_source.  For a model with Custom lineshapes the source of the
+ * second-order factor module (ctpwa_custom2) follows a line
+ * "// ---- module: ctpwa_custom2 ----" */
 long long ctpwa_generate_hessian_source(const char* plan_text, int precision, char* buf, long long cap);
 
 /* compile the fused kernel and the Hessian rows kernel for a plan with hipRTC (no device

@@ -182,6 +182,7 @@ def test_mirror_polarisations_are_folded(tmp_path, oracle_lib, precision, per_ev
     ana = ctpwa.analysis(cfg, precision=precision)
     lay = ana.layout()
     assert lay["bytes_per_eval"] == sum(lay["n_events"]) * per_event
+    assert lay["n_pol_stored"] == 1
     src = ana._h.kernel_source()
     assert "s0_1 = make_double2(" in src and "sl[1LL * ld" in src and "sl[4LL * ld" not in src
     plan = ana.plan
