@@ -712,12 +712,21 @@ std::string generate_eval_kernel(const Model_& M) {
         tile_bytes = tb;
     } else {
         if (M.prefetch_sl) {
-            // the SL values of the thread's next event are loaded one iteration ahead
-            // (registers), so their latency overlaps the current event's arithmetic
+            // the SL values and invariants of the thread's next event are loaded one
+            // iteration ahead (registers, in their stored types), so their latency overlaps
+            // the current event's arithmetic
+            auto inv_raw = [&](size_t i, const std::string& ev) {
+                std::ostringstream q;
+                if (imap[i] >= 0) q << "inv[" << imap[i] << "LL * ld + " << ev << "]";
+                else q << "invf[" << (-imap[i] - 1) << "LL * ld + " << ev << "]";
+                return q.str();
+            };
             o << "  long long e = e0 + threadIdx.x;\n  const long long ep = (e < e1) ? e : e0;\n";
             for (int u = 0; u < NU; ++u)
                 for (int p = 0; p < NPS; ++p)
                     o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + ep];\n";
+            for (size_t i = 0; i < M.inv.size(); ++i)
+                if (used[i]) o << "  " << (imap[i] >= 0 ? "double" : "float") << " nx" << i << " = " << inv_raw(i, "ep") << ";\n";
             o << "  for (; e < e1; e += " << WGS << ") {\n";
             for (int u = 0; u < NU; ++u)
                 for (int p = 0; p < NP; ++p) {
@@ -725,10 +734,14 @@ std::string generate_eval_kernel(const Model_& M) {
                     if (p >= NPS) o << mirror(u, p, "    ");
                     else o << "    const double2 s" << nm << " = " << sl_conv("n" + nm) << ";\n";
                 }
+            for (size_t i = 0; i < M.inv.size(); ++i)
+                if (used[i]) o << "    const double x" << i << " = (double)nx" << i << ";\n";
             o << "    { const long long en = (e + " << WGS << " < e1) ? e + " << WGS << " : e;\n";
             for (int u = 0; u < NU; ++u)
                 for (int p = 0; p < NPS; ++p)
                     o << "      n" << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + en];\n";
+            for (size_t i = 0; i < M.inv.size(); ++i)
+                if (used[i]) o << "      nx" << i << " = " << inv_raw(i, "en") << ";\n";
             o << "    }\n";
         } else {
             o << "  for (long long e = e0 + threadIdx.x; e < e1; e += " << WGS << ") {\n";
@@ -742,9 +755,9 @@ std::string generate_eval_kernel(const Model_& M) {
                     o << "    const " << SLT << " g" << nm << " = sl[" << (long long)u * NPS + p << "LL * ld + e];\n";
                     o << "    const double2 s" << nm << " = " << sl_conv("g" + nm) << ";\n";
                 }
+            for (size_t i = 0; i < M.inv.size(); ++i)
+                if (used[i]) o << "    const double x" << i << " = " << inv_load(i, "e") << ";\n";
         }
-        for (size_t i = 0; i < M.inv.size(); ++i)
-            if (used[i]) o << "    const double x" << i << " = " << inv_load(i, "e") << ";\n";
     }
     // per-event core: denominators, Custom factors, block products (shared by the fused
     // kernel and the amplitude/intensity kernel of models with Custom lineshapes)
@@ -754,16 +767,50 @@ std::string generate_eval_kernel(const Model_& M) {
             if (t.inv < 0) return coef;
             return coef + " * x" + std::to_string(t.inv);
         };
+        // a sum starts from its first term (0.0 + x is an instruction: it is not x for x = -0.0)
+        auto term_sum = [&](const char* nm, int d, const std::vector<Term>& ts) {
+            if (ts.empty()) { o << "    double " << nm << d << " = 0.0;\n"; return; }
+            o << "    double " << nm << d << " = " << term_expr(d, ts[0]) << ";\n";
+            for (size_t k = 1; k < ts.size(); ++k) o << "    " << nm << d << " += " << term_expr(d, ts[k]) << ";\n";
+        };
+        // 1 / |D_d|^2 for groups of up to 8 propagators from one reciprocal: prefix products
+        // a_k = m_0 ... m_k of m_d = |D_d|^2, r = 1 / a_last, then backwards q_k = r a_(k-1),
+        // r *= m_k (Montgomery's simultaneous inversion).  One v_rcp_f64 with its Newton
+        // steps per group instead of per propagator; each q_d is within a few ulp of 1/m_d, and
+        // a group of 8 squared moduli stays far inside the FP64 range.
+        std::vector<int> recip;
         for (size_t d = 0; d < M.dens.size(); ++d) {
             const DenSpec& Dn = M.dens[d];
-            o << "    double dr" << d << " = 0.0, di" << d << " = 0.0;\n";
-            for (const Term& t : Dn.re) o << "    dr" << d << " += " << term_expr((int)d, t) << ";\n";
-            for (const Term& t : Dn.im) o << "    di" << d << " += " << term_expr((int)d, t) << ";\n";
-            if (Dn.stat) {   // table lineshape: the factor itself
+            term_sum("dr", (int)d, Dn.re);
+            term_sum("di", (int)d, Dn.im);
+            if (Dn.stat) continue;   // table lineshape: the factor itself
+            recip.push_back((int)d);
+            o << "    const double m" << d << " = dr" << d << " * dr" << d << " + di" << d << " * di" << d << ";\n";
+        }
+        for (size_t g0 = 0; g0 < recip.size(); g0 += 8) {
+            const size_t g1 = std::min(recip.size(), g0 + 8);
+            if (g1 - g0 == 1) {
+                const int d = recip[g0];
+                o << "    const double q" << d << " = frcp(m" << d << ");\n";
+                continue;
+            }
+            for (size_t k = g0 + 1; k < g1; ++k) {
+                o << "    const double a" << recip[k] << " = " << (k == g0 + 1 ? "m" : "a") << recip[k - 1] << " * m"
+                  << recip[k] << ";\n";
+            }
+            o << "    double r" << recip[g0] << " = frcp(a" << recip[g1 - 1] << ");\n";
+            for (size_t k = g1 - 1; k > g0; --k) {
+                const int d = recip[k], prev = recip[k - 1];
+                o << "    const double q" << d << " = r" << recip[g0] << " * " << (k - 1 == g0 ? "m" : "a") << prev << "; r"
+                  << recip[g0] << " *= m" << d << ";\n";
+            }
+            o << "    const double q" << recip[g0] << " = r" << recip[g0] << ";\n";
+        }
+        for (size_t d = 0; d < M.dens.size(); ++d) {
+            if (M.dens[d].stat) {
                 o << "    const double pr" << d << " = dr" << d << ", pi" << d << " = di" << d << ";\n";
                 continue;
             }
-            o << "    const double q" << d << " = frcp(dr" << d << " * dr" << d << " + di" << d << " * di" << d << ");\n";
             o << "    const double pr" << d << " = dr" << d << " * q" << d << ", pi" << d << " = di" << d << " * q" << d << ";\n";
         }
         // Custom lineshape factors as written for this parameter point by ctpwa_custom (their
@@ -824,38 +871,68 @@ std::string generate_eval_kernel(const Model_& M) {
             o << "    t" << u << "r += s" << u << "_" << p << ".x * w" << p << "r; t" << u << "r += s" << u << "_" << p << ".y * w" << p
               << "i; t" << u << "i += s" << u << "_" << p << ".x * w" << p << "i; t" << u << "i -= s" << u << "_" << p << ".y * w" << p << "r;\n";
     }
-    for (int a = 0; a < NA; ++a) {
-        int b = M.amps[a].block, u = M.amps[a].u;
-        o << "    ax" << a << "r += br" << b << " * t" << u << "r; ax" << a << "r += bi" << b << " * t" << u << "i; ax" << a << "i += br" << b
-          << " * t" << u << "i; ax" << a << "i -= bi" << b << " * t" << u << "r;\n";
-    }
-    // theta terms: R_d = Z_b Psi_b Psi_d
+    // per block: the coupling sums P_a = conj(Psi_b) T_u (X_a += P_a), then the theta
+    // terms R_d = Z_b Psi_b Psi_d with Z_b Psi_b = sum_{a in b} c_a conj(P_a)
     std::vector<std::vector<int>> den_terms(M.dens.size());
     for (int t = 0; t < NTH; ++t) den_terms[M.thterms[t].den].push_back(t);
     std::vector<std::vector<int>> custom_terms(M.customs.size());
     for (int t = 0; t < NCT; ++t) custom_terms[M.cterms[t].custom].push_back(t);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const PBlock& B = P.blocks[b];
+        for (int c = 0; c < B.nsl; ++c) {
+            int a = B.site + c, u = M.amps[a].u;
+            o << "    const double P" << a << "r = br" << b << " * t" << u << "r + bi" << b << " * t" << u << "i, P" << a
+              << "i = br" << b << " * t" << u << "i - bi" << b << " * t" << u << "r;\n";
+            o << "    ax" << a << "r += P" << a << "r; ax" << a << "i += P" << a << "i;\n";
+        }
         bool any = false;
         for (int d : M.block_dens[b]) if (!den_terms[d].empty()) any = true;
         for (int c : M.block_customs[b]) if (!custom_terms[c].empty()) any = true;
         if (B.nsl == 0 || !any) continue;
-        o << "    {\n      double zr = 0.0, zi = 0.0;\n";
-        for (int c = 0; c < B.nsl; ++c) {
-            int a = B.site + c, u = M.amps[a].u;
-            o << "      zr += c" << a << "r * t" << u << "r; zr += c" << a << "i * t" << u << "i; zi += c" << a << "i * t" << u
-              << "r; zi -= c" << a << "r * t" << u << "i;\n";
+        o << "    {\n";
+        const bool with_custom = !M.block_customs[b].empty();
+        if (with_custom) {
+            // the derivatives through Custom factors need Z_b itself
+            o << "      double zr = 0.0, zi = 0.0;\n";
+            for (int c = 0; c < B.nsl; ++c) {
+                int a = B.site + c, u = M.amps[a].u;
+                o << "      zr += c" << a << "r * t" << u << "r; zr += c" << a << "i * t" << u << "i; zi += c" << a << "i * t" << u
+                  << "r; zi -= c" << a << "r * t" << u << "i;\n";
+            }
+            o << "      const double zbr = zr * br" << b << " - zi * bi" << b << ", zbi = zr * bi" << b << " + zi * br" << b << ";\n";
+        } else {
+            o << "      double zbr = 0.0, zbi = 0.0;\n";
+            for (int c = 0; c < B.nsl; ++c) {
+                int a = B.site + c;
+                o << "      zbr += c" << a << "r * P" << a << "r; zbr += c" << a << "i * P" << a << "i; zbi += c" << a
+                  << "i * P" << a << "r; zbi -= c" << a << "r * P" << a << "i;\n";
+            }
         }
-        o << "      const double zbr = zr * br" << b << " - zi * bi" << b << ", zbi = zr * bi" << b << " + zi * br" << b << ";\n";
         for (int d : M.block_dens[b]) {
             if (den_terms[d].empty()) continue;
-            o << "      {\n        const double rr = zbr * pr" << d << " - zbi * pi" << d << ", ri = zbr * pi" << d
-              << " + zbi * pr" << d << ";\n";
+            // a component with one term and no invariant is accumulated by two FMAs
+            int nre = 0, nim = 0;
+            bool bare_re = false, bare_im = false;
             for (int t : den_terms[d]) {
                 const ThTerm& T = M.thterms[t];
-                const char* comp = T.im ? "ri" : "rr";
-                if (T.inv < 0) o << "        at" << t << " += " << comp << ";\n";
-                else o << "        at" << t << " += x" << T.inv << " * " << comp << ";\n";
+                (T.im ? nim : nre)++;
+                if (T.inv < 0) (T.im ? bare_im : bare_re) = true;
+            }
+            const bool fre = nre == 1 && bare_re, fim = nim == 1 && bare_im;
+            o << "      {\n";
+            if (nre && !fre) o << "        const double rr = zbr * pr" << d << " - zbi * pi" << d << ";\n";
+            if (nim && !fim) o << "        const double ri = zbr * pi" << d << " + zbi * pr" << d << ";\n";
+            for (int t : den_terms[d]) {
+                const ThTerm& T = M.thterms[t];
+                if (!T.im && fre) {
+                    o << "        at" << t << " += zbr * pr" << d << "; at" << t << " -= zbi * pi" << d << ";\n";
+                } else if (T.im && fim) {
+                    o << "        at" << t << " += zbr * pi" << d << "; at" << t << " += zbi * pr" << d << ";\n";
+                } else {
+                    const char* comp = T.im ? "ri" : "rr";
+                    if (T.inv < 0) o << "        at" << t << " += " << comp << ";\n";
+                    else o << "        at" << t << " += x" << T.inv << " * " << comp << ";\n";
+                }
             }
             o << "      }\n";
         }
