@@ -259,6 +259,16 @@ static long long gram_part_doubles(int ncol) {
 // address, never its serial
 static std::atomic<unsigned long long> g_handle_serial{0};
 
+// one set of Hessian buffers: a chunk of Jacobian rows (FP64, or FP32 in the mixed
+// form), u rows, weights, the rows kernel's per-wave column sums, and the matrix-core
+// Gram partials of the Jacobian (2C columns) and of u (NU columns)
+struct HessWS {
+    long long chunk = 0, cs_ld = 0;
+    void* J = nullptr;
+    double *U = nullptr, *W = nullptr, *cs = nullptr;
+    GramTabs gJ, gU;
+};
+
 struct ctpwa_handle {
     int device = 0;
     const unsigned long long serial = ++g_handle_serial;
@@ -294,14 +304,16 @@ struct ctpwa_handle {
     // kernel, one chunk of Jacobian rows (FP64, or FP32 in the mixed form), u rows,
     // weights, per-wave column sums, and the matrix-core Gram tables and partials of the
     // Jacobian (2C columns) and of u (NU columns)
+    // Buffer set 0 serves the phase space on the caller's stream; set 1 serves data and
+    // background on a stream of the handle, so the two run concurrently.
     bool hess_ready = false;
-    long long hchunk = 0, hcs_ld = 0;
     int hC = 0, hNU = 0, hNZ = 0, hnG = 0, hnUu = 0;
     hipModule_t hmod = nullptr;
     hipFunction_t fn_hrows = nullptr;
-    void* hJ = nullptr;
-    double *hU = nullptr, *hcs = nullptr, *hW = nullptr, *htot = nullptr, *hHext = nullptr;
-    GramTabs hgJ, hgU;
+    HessWS hws[2];
+    hipStream_t hs2 = nullptr;                  // default priority, non-blocking
+    hipEvent_t hev[2] = {nullptr, nullptr};     // start of the classes' work; end of set 1's work
+    double *htot = nullptr, *hHext = nullptr;
     int *hmap_ptr = nullptr, *hmap_x = nullptr;   // fit parameter -> extended entries (CSR)
     int* hperm = nullptr;                         // rows kernel's summed columns -> [Z | u] order
     double* hmap_r = nullptr;
@@ -350,6 +362,10 @@ struct ctpwa_handle {
         int prev = 0;
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(device);
+        if (hs2) (void)hipStreamSynchronize(hs2);
+        if (hs2) (void)hipStreamDestroy(hs2);
+        for (hipEvent_t ev : hev)
+            if (ev) (void)hipEventDestroy(ev);
         // the modules belong to the process-wide compile cache (shared by handles of one
         // model), not to the handle
         for (void* p : allocs) (void)hipFree(p);
@@ -1035,13 +1051,23 @@ static long long hess_chunk(const Model_& M, long long maxn) {
     // a multiple of 4 events: the Gram's 16-byte slab loads stay aligned
     return (std::max<long long>(1, std::min(maxn, c)) + 3) / 4 * 4;
 }
-static long long hess_workspace_bytes(const Model_& M, long long maxn, long long ntot) {
+// events per chunk of each buffer set: set 0 the phase space, set 1 data and background
+static void hess_chunks(const Model_& M, const long long n3[3], long long ch[2]) {
+    ch[0] = hess_chunk(M, n3[0]);
+    const long long n12 = std::max(n3[1], n3[2]);
+    ch[1] = n12 > 0 ? hess_chunk(M, n12) : 0;
+}
+static long long hess_workspace_bytes(const Model_& M, const long long n3[3], long long ntot) {
     HessDims d = hess_dims(M);
-    long long chunk = hess_chunk(M, maxn);
+    long long ch[2];
+    hess_chunks(M, n3, ch);
     long long nG = (long long)(2 * d.C) * (2 * d.C + 1) / 2, nUu = (long long)d.NU * (d.NU + 1) / 2;
-    long long w = chunk * (M.P.npol * 2LL * d.C + d.NU + 2) + (long long)(d.NZ + d.NU) * ctpwa_hess_rows_waves(chunk) +
-                  3 * (nG + nUu + d.NZ + d.NU) + (long long)d.NU * d.NU;
-    w += gram_part_doubles(2 * d.C) + gram_part_doubles(d.NU);
+    long long w = 3 * (nG + nUu + d.NZ + d.NU) + (long long)d.NU * d.NU;
+    for (int k = 0; k < 2; ++k) {
+        if (ch[k] == 0) continue;
+        w += ch[k] * (M.P.npol * 2LL * d.C + d.NU + 2) + (long long)(d.NZ + d.NU) * ctpwa_hess_rows_waves(ch[k]);
+        w += gram_part_doubles(2 * d.C) + gram_part_doubles(d.NU);
+    }
     w += (long long)M.customs.size() * M.ncf2() * ntot;   // second-order Custom factor rows
     return w * 8 + 64 * 1024;    // + the index tables and allocation slack
 }
@@ -1069,19 +1095,28 @@ static void hess_setup(ctpwa_handle* h) {
     h->hnG = 2 * hd.C * (2 * hd.C + 1) / 2;
     h->hnUu = hd.NU * (hd.NU + 1) / 2;
     if (P.ntheta > 64) throw std::runtime_error("Hessian: more than 64 free resonance parameters is not built");
-    const long long maxn = std::max(h->n[0], std::max(h->n[1], h->n[2]));
-    h->hchunk = hess_chunk(M, maxn);
+    long long ch[2];
+    hess_chunks(M, h->n, ch);
     const size_t es = M.mixed ? sizeof(float) : sizeof(double);
-    h->hJ = h->alloc<unsigned char>((size_t)h->hchunk * P.npol * 2 * h->hC * es);
-    h->hU = h->alloc<double>((size_t)h->hchunk * h->hNU);
-    h->hcs_ld = ctpwa_hess_rows_waves(h->hchunk);
-    h->hcs = h->alloc<double>((size_t)(h->hNZ + h->hNU) * h->hcs_ld);
-    h->hW = h->alloc<double>((size_t)h->hchunk * 2);
+    for (int k = 0; k < 2; ++k) {
+        HessWS& ws = h->hws[k];
+        ws.chunk = ch[k];
+        if (ch[k] == 0) continue;
+        ws.J = h->alloc<unsigned char>((size_t)ch[k] * P.npol * 2 * h->hC * es);
+        ws.U = h->alloc<double>((size_t)ch[k] * h->hNU);
+        ws.cs_ld = ctpwa_hess_rows_waves(ch[k]);
+        ws.cs = h->alloc<double>((size_t)(h->hNZ + h->hNU) * ws.cs_ld);
+        ws.W = h->alloc<double>((size_t)ch[k] * 2);
+        gram_setup(h, ws.gJ, 2 * h->hC);
+        gram_setup(h, ws.gU, h->hNU);
+    }
+    if (ch[1] > 0) {
+        HIPCHECK(hipStreamCreateWithFlags(&h->hs2, hipStreamNonBlocking));
+        for (hipEvent_t& ev : h->hev) HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    }
     // totals: per class G, Uu, Z, g
     h->htot = h->alloc<double>((size_t)3 * (h->hnG + h->hnUu + h->hNZ + h->hNU));
     h->hHext = h->alloc<double>((size_t)h->hNU * h->hNU);
-    gram_setup(h, h->hgJ, 2 * h->hC);
-    gram_setup(h, h->hgU, h->hNU);
     // fit parameter -> extended entries: [Re p_f] -> Re v_a, [Im p_f] -> Im v_a with
     // v_a = r_a p_f(a); theta_j -> theta_j
     {
@@ -1263,43 +1298,56 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         size_t per_class = (size_t)h->hnG + h->hnUu + h->hNZ + h->hNU;
         HIPCHECK(ctpwa_launch_zero(h->htot, (long long)(3 * per_class), s));
         HessAsm as{};
-        const long long ldJ = h->hchunk * P.npol;
         const int f32 = M.mixed;
+        // data and background run on the handle's second stream with buffer set 1, after
+        // the coefficient table and the zeroed totals on the caller's stream
+        const bool two = h->hs2 != nullptr && (h->n[1] > 0 || h->n[2] > 0);
+        if (two) {
+            HIPCHECK(hipEventRecord(h->hev[0], s));
+            HIPCHECK(hipStreamWaitEvent(h->hs2, h->hev[0], 0));
+        }
         for (int c = 0; c < 3; ++c) {
             double* base = h->htot + c * per_class;
             as.G[c] = base;
             as.Uu[c] = base + h->hnG;
             as.Zs[c] = base + h->hnG + h->hnUu;
             as.gs[c] = base + h->hnG + h->hnUu + h->hNZ;
+            HessWS& ws = h->hws[c == 0 || !two ? 0 : 1];
+            hipStream_t sc = (c == 0 || !two) ? s : h->hs2;
+            const long long ldJ = ws.chunk * P.npol;
             const double* wts = (c > 0 && h->pw[c]) ? h->wts : nullptr;
             double wconst = h->wc[c];
             int cls = c;
             int k = 0;
-            for (long long s0 = 0; s0 < h->n[c]; s0 += h->hchunk, ++k) {
-                long long cnt = std::min(h->hchunk, h->n[c] - s0);
+            for (long long s0 = 0; s0 < h->n[c]; s0 += ws.chunk, ++k) {
+                long long cnt = std::min(ws.chunk, h->n[c] - s0);
                 long long e0 = h->off[c] + s0;
                 // per-event rows of the chunk (generated kernel)
                 void* args[] = {&h->table, (void*)&h->slp, &h->inv, &h->invf, &h->ntot, (void*)&wts, &wconst, &cls, &e0,
-                                &cnt, &h->hJ, (void*)&ldJ, &h->hchunk, &h->hU, &h->hW, &h->hcs, &h->hcs_ld, &h->cfb2,
+                                &cnt, &ws.J, (void*)&ldJ, &ws.chunk, &ws.U, &ws.W, &ws.cs, &ws.cs_ld, &h->cfb2,
                                 &h->ntot};
                 const unsigned grid = (unsigned)((cnt + 255) / 256);
-                HIPCHECK(hipModuleLaunchKernel(h->fn_hrows, grid, 1, 1, 256, 1, 1, 0, s, args, nullptr));
+                HIPCHECK(hipModuleLaunchKernel(h->fn_hrows, grid, 1, 1, 256, 1, 1, 0, sc, args, nullptr));
                 // weighted Grams on the matrix cores, summed over the class's chunks
-                HIPCHECK(ctpwa_launch_gram_panels(h->hJ, f32, 2 * h->hC, ldJ, cnt * P.npol, h->hW, cnt, h->hgJ.pp,
-                                                  h->hgJ.npp, h->hgJ.ntp, h->hgJ.part, h->hgJ.nwg, k > 0, s));
+                HIPCHECK(ctpwa_launch_gram_panels(ws.J, f32, 2 * h->hC, ldJ, cnt * P.npol, ws.W, cnt, ws.gJ.pp,
+                                                  ws.gJ.npp, ws.gJ.ntp, ws.gJ.part, ws.gJ.nwg, k > 0, sc));
                 if (c > 0)
-                    HIPCHECK(ctpwa_launch_gram_panels(h->hU, 0, h->hNU, h->hchunk, cnt, h->hW + h->hchunk, cnt, h->hgU.pp,
-                                                      h->hgU.npp, h->hgU.ntp, h->hgU.part, h->hgU.nwg, k > 0, s));
+                    HIPCHECK(ctpwa_launch_gram_panels(ws.U, 0, h->hNU, ws.chunk, cnt, ws.W + ws.chunk, cnt, ws.gU.pp,
+                                                      ws.gU.npp, ws.gU.ntp, ws.gU.part, ws.gU.nwg, k > 0, sc));
                 // Zs[c] and gs[c] are adjacent: one pass adds both from the rows kernel's sums
-                HIPCHECK(ctpwa_launch_accum_cols(h->hcs, h->hcs_ld, cnt, h->hNZ + h->hNU, h->hperm, (double*)as.Zs[c], s));
+                HIPCHECK(ctpwa_launch_accum_cols(ws.cs, ws.cs_ld, cnt, h->hNZ + h->hNU, h->hperm, (double*)as.Zs[c], sc));
             }
             if (h->n[c] > 0) {
-                HIPCHECK(ctpwa_launch_accum_panels(h->hgJ.part, h->hgJ.nwg, h->hgJ.ntp, h->hgJ.tiles, 2 * h->hC,
-                                                   (double*)as.G[c], s));
+                HIPCHECK(ctpwa_launch_accum_panels(ws.gJ.part, ws.gJ.nwg, ws.gJ.ntp, ws.gJ.tiles, 2 * h->hC,
+                                                   (double*)as.G[c], sc));
                 if (c > 0)
-                    HIPCHECK(ctpwa_launch_accum_panels(h->hgU.part, h->hgU.nwg, h->hgU.ntp, h->hgU.tiles, h->hNU,
-                                                       (double*)as.Uu[c], s));
+                    HIPCHECK(ctpwa_launch_accum_panels(ws.gU.part, ws.gU.nwg, ws.gU.ntp, ws.gU.tiles, h->hNU,
+                                                       (double*)as.Uu[c], sc));
             }
+        }
+        if (two) {
+            HIPCHECK(hipEventRecord(h->hev[1], h->hs2));
+            HIPCHECK(hipStreamWaitEvent(s, h->hev[1], 0));
         }
         as.C = h->hC; as.NU = h->hNU; as.NZ = h->hNZ;
         as.namp = P.namp; as.nfree = P.nfree; as.ntheta = P.ntheta; as.nblock = (int)P.blocks.size();
@@ -1648,7 +1696,8 @@ extern "C" long long ctpwa_hessian_workspace_bytes(const char* plan_text, long l
                                                    long long n_bkg) {
     try {
         Model_ M = derive(parse_plan(plan_text));
-        return hess_workspace_bytes(M, std::max(n_phsp, std::max(n_data, n_bkg)), n_phsp + n_data + n_bkg);
+        const long long n3[3] = {n_phsp, n_data, n_bkg};
+        return hess_workspace_bytes(M, n3, n_phsp + n_data + n_bkg);
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_hessian_workspace_bytes: ") + e.what();
         return -1;

@@ -23,6 +23,7 @@
 //
 // Events are processed in chunks; every sum is a fixed-order reduction.
 #include <algorithm>
+#include <cstdlib>
 
 #include "device.hpp"
 
@@ -472,7 +473,13 @@ extern "C" __attribute__((visibility("hidden"))) int ctpwa_gram_tables(int ncol,
 
 // workgroups along the rows for a Gram of npp panel pairs (about 2048 in all)
 extern "C" __attribute__((visibility("hidden"))) int ctpwa_gram_nwg(int npp) {
-    return std::max(32, std::min(512, 2048 / std::max(1, npp)));
+    // environment CTPWA_GRAM_WGS overrides the per-launch workgroup budget (measurements)
+    static const int budget = [] {
+        const char* e = std::getenv("CTPWA_GRAM_WGS");
+        return e ? std::max(32, std::atoi(e)) : 2048;
+    }();
+    const int cap = std::max(512, budget / 4);
+    return std::max(32, std::min(cap, budget / std::max(1, npp)));
 }
 
 // weighted Gram on the matrix cores: partials [nwg][ntp][256] written (accumulate = 0)
