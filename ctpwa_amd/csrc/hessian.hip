@@ -112,14 +112,16 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
     constexpr int NLD = (FULL ? 128 : NT * 16) / CPI;
     typedef T vec_t __attribute__((ext_vector_type(VW)));
     const int rr0 = (lane % LPC_) * VW, ch = lane / LPC_;
-    vec_t pre[NLD];
-    double wpre = 0.0;
     const int rw = lane % R;
+    // event of row t + rw is (t + rw) mod nev: one division for the wave's first slab, then
+    // the base of the next slab to load advances by R with subtractions (no 64-bit division
+    // per slab)
+    long long eload = r0 < r1 ? r0 % nev : 0;
     auto gcol = [&](int c) -> int {           // global column of LDS column c, or -1
         if (c < 64) return c < wP ? 64 * P + c : -1;
         return (c - 64) < wQ ? 64 * Q + (c - 64) : -1;
     };
-    auto load = [&](long long t0) {
+    auto load = [&](vec_t (&pre)[NLD], double& wpre, long long t0) {
         const int nt = (int)((r1 - t0) < R ? (r1 - t0) : R);
 #pragma unroll
         for (int it = 0; it < NLD; ++it) {
@@ -133,10 +135,15 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
                 for (int v = 0; v < VW; ++v) pre[it][v] = (g >= 0 && rr0 + v < nt) ? src[v] : (T)0;
             }
         }
-        wpre = (lane < R && rw < nt) ? Wt[(t0 + rw) % nev] : 0.0;
+        long long ev = eload + rw;
+        while (ev >= nev) ev -= nev;              // rows that wrap to the next polarisation
+        wpre = (lane < R && rw < nt) ? Wt[ev] : 0.0;
+        eload += R;
+        while (eload >= nev) eload -= nev;        // once at most unless nev < R
     };
-    if (r0 < r1) load(r0);
-    for (long long t0 = r0; t0 < r1; t0 += R) {
+    // one slab: its registers to the wave's LDS region, the refill of those registers with
+    // the slab at tnext (loads in flight during this slab's products), then the products
+    auto slab = [&](vec_t (&pre)[NLD], double& wpre, long long tnext) {
 #pragma unroll
         for (int it = 0; it < NLD; ++it) {
             const int c = CPI * it + ch;
@@ -149,7 +156,7 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        if (t0 + R < r1) load(t0 + R);
+        if (tnext < r1) load(pre, wpre, tnext);
         const int li = lane & 15;
         if constexpr (sizeof(T) == 8) {
 #pragma unroll
@@ -173,11 +180,11 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
 #pragma unroll
             for (int k0 = 0; k0 < R; k0 += 4) {
                 const int kr = k0 + (lane >> 4);
-                const double w = ws[kr];
+                const float w = (float)ws[kr];
                 int q = 0;
 #pragma unroll
                 for (int I = 0; I < NT; ++I) {
-                    const float av = (float)(w * (double)xs[(I * 16 + li) * ST + kr]);
+                    const float av = w * xs[(I * 16 + li) * ST + kr];
 #pragma unroll
                     for (int J = FULL ? 0 : I; J < NT; ++J, ++q)
                         t[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, (float)xs[(boff0 + J * 16 + li) * ST + kr], t[q], 0, 0, 0);
@@ -195,7 +202,13 @@ k_gram_panels(const T* __restrict__ X, int ncol, long long ldx, long long nrows,
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
+    };
+    // one register buffer: the next slab's loads are in flight while this one is consumed
+    // (a second buffer, two slabs ahead, measured no faster at two waves per SIMD)
+    vec_t pre[NLD];
+    double wpre = 0.0;
+    if (r0 < r1) load(pre, wpre, r0);
+    for (long long t0 = r0; t0 < r1; t0 += R) slab(pre, wpre, t0 + R);
     // combine the waves in wave order in LDS (the slab regions are free once every wave
     // has finished), element e = 16 i + j of tile pair q at comb[q * 256 + e]
     double* comb = (double*)gp_smem;
