@@ -258,6 +258,14 @@ static long long gram_part_doubles(int ncol) {
 // serial numbers of handles: a handle created after another was destroyed may reuse its
 // address, never its serial
 static std::atomic<unsigned long long> g_handle_serial{0};
+// handles alive in the process (a handle's member counts itself in and out)
+static std::atomic<int> g_live_handles{0};
+struct LiveCount {
+    LiveCount() { ++g_live_handles; }
+    ~LiveCount() { --g_live_handles; }
+    LiveCount(const LiveCount&) = delete;
+    LiveCount& operator=(const LiveCount&) = delete;
+};
 
 // one set of Hessian buffers: a chunk of Jacobian rows (FP64, or FP32 in the mixed
 // form), u rows, weights, the rows kernel's per-wave column sums, and the matrix-core
@@ -270,6 +278,7 @@ struct HessWS {
 };
 
 struct ctpwa_handle {
+    LiveCount live;
     int device = 0;
     const unsigned long long serial = ++g_handle_serial;
     Model_ M;
@@ -379,10 +388,15 @@ static void apply_codegen_env(Model_& M) {
     // next-event loads in registers: measured 4 % faster on the mixed form of a model of
     // about ten amplitudes over four SL tensors (216 of 256 VGPRs without); a larger model
     // has no registers to spare (a 20-amplitude four-body model spills with it)
-    M.prefetch_sl = M.mixed && M.P.namp <= 12 && (int)M.us.size() * M.P.npol <= 8;
+    M.prefetch_sl = (M.mixed && M.P.namp <= 12 && (int)M.us.size() * M.P.npol <= 8) ? 1 : 0;
     if (const char* e = std::getenv("CTPWA_EVAL_WAVES")) M.min_waves = std::atoi(e);
     if (const char* e = std::getenv("CTPWA_EVAL_LDS")) M.lds_pipe = std::atoi(e) != 0;
-    if (const char* e = std::getenv("CTPWA_EVAL_PREFETCH")) M.prefetch_sl = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CTPWA_EVAL_INVTREE")) M.inv_tree = std::atoi(e) != 0;
+    if (const char* e = std::getenv("CTPWA_EVAL_PREFETCH")) {
+        int d = std::atoi(e);
+        if (d < 0 || d > 2) throw std::runtime_error("CTPWA_EVAL_PREFETCH must be 0, 1 or 2");
+        M.prefetch_sl = d;
+    }
     if (const char* e = std::getenv("CTPWA_EVAL_WGS")) {
         int w = std::atoi(e);
         if (w != 64 && w != 128 && w != 256) throw std::runtime_error("CTPWA_EVAL_WGS must be 64, 128 or 256");
@@ -477,15 +491,30 @@ static void check_q0_masses(const Model_& M) {
 static void setup_partition(ctpwa_handle* h, bool scan_phsp) {
     const long long nev = (scan_phsp ? h->n[0] : 0) + h->n[1] + h->n[2];
     long long target = std::min<long long>(1024, std::max<long long>(1, (nev + 255) / 256));
+    // relative cost of a data or background event: the workgroups of each class get events
+    // in inverse proportion, so that the classes finish together.  A phase-space event costs
+    // about 20 FP64 instructions per amplitude (sums and gradient accumulators), 10 per
+    // propagator and 8 per SL tensor and polarisation; a data event adds the logarithm, the
+    // reciprocal of I and the clamps, worth about 160 of them (from the fastest measured
+    // split of the 3-pion model, 10 amplitudes: 1.3-1.4; environment CTPWA_DATA_COST
+    // overrides it for measurements)
+    const Model_& M = h->M;
+    const double base = 20.0 * M.P.namp + 10.0 * (double)M.dens.size() + 8.0 * (double)M.us.size() * M.P.npol + 40.0;
+    double wcost = 1.0 + 160.0 / base;
+    if (const char* e = std::getenv("CTPWA_DATA_COST")) wcost = std::atof(e);
+    if (!(wcost >= 0.25 && wcost <= 4.0)) throw std::runtime_error("CTPWA_DATA_COST must lie in [0.25, 4]");
+    const double weff = (scan_phsp ? (double)h->n[0] : 0.0) + wcost * (double)(h->n[1] + h->n[2]);
     std::vector<int> cls;
     std::vector<long long> rng;
     for (int c = 0; c < 3; ++c) {
         h->g0[c] = (int)cls.size();
         long long n = (c == 0 && !scan_phsp) ? 0 : h->n[c];
         if (n > 0) {
-            long long want = std::max<long long>(1, (long long)std::llround((double)target * n / nev));
+            const double wn = (c == 0 ? 1.0 : wcost) * (double)n;
+            long long want = std::max<long long>(1, (long long)std::llround((double)target * wn / weff));
             long long chunk = (n + want - 1) / want;
-            chunk = ((chunk + 255) / 256) * 256;
+            const long long gw = h->M.wgs;          // whole iterations for every thread
+            chunk = ((chunk + gw - 1) / gw) * gw;
             long long nw = (n + chunk - 1) / chunk;
             for (long long g = 0; g < nw; ++g) {
                 cls.push_back(c);
@@ -1051,16 +1080,43 @@ static long long hess_chunk(const Model_& M, long long maxn) {
     // a multiple of 4 events: the Gram's 16-byte slab loads stay aligned
     return (std::max<long long>(1, std::min(maxn, c)) + 3) / 4 * 4;
 }
-// events per chunk of each buffer set: set 0 the phase space, set 1 data and background
-static void hess_chunks(const Model_& M, const long long n3[3], long long ch[2]) {
-    ch[0] = hess_chunk(M, n3[0]);
-    const long long n12 = std::max(n3[1], n3[2]);
-    ch[1] = n12 > 0 ? hess_chunk(M, n12) : 0;
+// The data and background classes run on a second stream of the handle, concurrently
+// with the phase space, when both sides are large enough for the overlap to outweigh the
+// two cross-stream waits, and only for a handle that is alone in the process (`others`:
+// the other live handles, counted when the buffers are set up and again at every call):
+// with several (a fit service) other handles' work fills the GPU, and a second stream
+// per handle, its events and its second buffer set cost more than they gain (measured:
+// Newton fits/hour of the service down by a quarter).  Environment CTPWA_HESS_STREAMS =
+// 1 or 2 forces one or two streams.
+static int hess_streams_env() {
+    if (const char* e = std::getenv("CTPWA_HESS_STREAMS")) {
+        const int k = std::atoi(e);
+        if (k != 1 && k != 2) throw std::runtime_error("CTPWA_HESS_STREAMS must be 1 or 2");
+        return k;
+    }
+    return 0;
 }
-static long long hess_workspace_bytes(const Model_& M, const long long n3[3], long long ntot) {
+static bool hess_two_streams(const long long n3[3], int others) {
+    const int k = hess_streams_env();
+    if (k) return k == 2 && n3[0] > 0 && n3[1] + n3[2] > 0;
+    return others == 0 && n3[0] >= (1LL << 20) && n3[1] + n3[2] >= (1LL << 17);
+}
+// events per chunk of each buffer set: with two streams set 0 takes the phase space and
+// set 1 data and background; with one stream set 0 takes every class
+static void hess_chunks(const Model_& M, const long long n3[3], int others, long long ch[2]) {
+    const long long n12 = std::max(n3[1], n3[2]);
+    if (!hess_two_streams(n3, others)) {
+        ch[0] = hess_chunk(M, std::max(n3[0], n12));
+        ch[1] = 0;
+        return;
+    }
+    ch[0] = hess_chunk(M, n3[0]);
+    ch[1] = hess_chunk(M, n12);
+}
+static long long hess_workspace_bytes(const Model_& M, const long long n3[3], long long ntot, int others) {
     HessDims d = hess_dims(M);
     long long ch[2];
-    hess_chunks(M, n3, ch);
+    hess_chunks(M, n3, others, ch);
     long long nG = (long long)(2 * d.C) * (2 * d.C + 1) / 2, nUu = (long long)d.NU * (d.NU + 1) / 2;
     long long w = 3 * (nG + nUu + d.NZ + d.NU) + (long long)d.NU * d.NU;
     for (int k = 0; k < 2; ++k) {
@@ -1096,7 +1152,7 @@ static void hess_setup(ctpwa_handle* h) {
     h->hnUu = hd.NU * (hd.NU + 1) / 2;
     if (P.ntheta > 64) throw std::runtime_error("Hessian: more than 64 free resonance parameters is not built");
     long long ch[2];
-    hess_chunks(M, h->n, ch);
+    hess_chunks(M, h->n, g_live_handles.load() - 1, ch);
     const size_t es = M.mixed ? sizeof(float) : sizeof(double);
     for (int k = 0; k < 2; ++k) {
         HessWS& ws = h->hws[k];
@@ -1301,7 +1357,9 @@ extern "C" int ctpwa_hessian(ctpwa_handle* h, const double* params_dev, double* 
         const int f32 = M.mixed;
         // data and background run on the handle's second stream with buffer set 1, after
         // the coefficient table and the zeroed totals on the caller's stream
-        const bool two = h->hs2 != nullptr && (h->n[1] > 0 || h->n[2] > 0);
+        const int forced = hess_streams_env();
+        const bool two = h->hs2 != nullptr && (h->n[1] > 0 || h->n[2] > 0) &&
+                         (forced == 2 || (forced == 0 && g_live_handles.load() == 1));
         if (two) {
             HIPCHECK(hipEventRecord(h->hev[0], s));
             HIPCHECK(hipStreamWaitEvent(h->hs2, h->hev[0], 0));
@@ -1697,7 +1755,8 @@ extern "C" long long ctpwa_hessian_workspace_bytes(const char* plan_text, long l
     try {
         Model_ M = derive(parse_plan(plan_text));
         const long long n3[3] = {n_phsp, n_data, n_bkg};
-        return hess_workspace_bytes(M, n3, n_phsp + n_data + n_bkg);
+        // the handle these buffers are for does not exist yet: every live handle is another
+        return hess_workspace_bytes(M, n3, n_phsp + n_data + n_bkg, g_live_handles.load());
     } catch (const std::exception& e) {
         g_err = std::string("ctpwa_hessian_workspace_bytes: ") + e.what();
         return -1;

@@ -22,6 +22,7 @@
 // by the DMA path while the previous tile is computed; a deterministic wave/LDS
 // reduction then writes one partial row per workgroup.
 #include <sstream>
+#include <functional>
 #include <string>
 
 #include "model.hpp"
@@ -712,21 +713,35 @@ std::string generate_eval_kernel(const Model_& M) {
         tile_bytes = tb;
     } else {
         if (M.prefetch_sl) {
-            // the SL values and invariants of the thread's next event are loaded one
-            // iteration ahead (registers, in their stored types), so their latency overlaps
-            // the current event's arithmetic
+            // the SL values and invariants of the thread's next event (depth 1) or next two
+            // events (depth 2: register sets n and m, rotated each iteration) are loaded
+            // ahead in registers, in their stored types, so their latency overlaps the
+            // current event's arithmetic
+            const int depth = M.prefetch_sl;
             auto inv_raw = [&](size_t i, const std::string& ev) {
                 std::ostringstream q;
                 if (imap[i] >= 0) q << "inv[" << imap[i] << "LL * ld + " << ev << "]";
                 else q << "invf[" << (-imap[i] - 1) << "LL * ld + " << ev << "]";
                 return q.str();
             };
+            auto decl_set = [&](const char* pre, const char* ev) {
+                for (int u = 0; u < NU; ++u)
+                    for (int p = 0; p < NPS; ++p)
+                        o << "  " << SLT << " " << pre << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + " << ev
+                          << "];\n";
+                for (size_t i = 0; i < M.inv.size(); ++i)
+                    if (used[i]) o << "  " << (imap[i] >= 0 ? "double" : "float") << " " << pre << "x" << i << " = " << inv_raw(i, ev) << ";\n";
+            };
+            // the event weight of a class with per-event weights is loaded ahead too: a load
+            // issued after the next-event loads would make its wait cover them as well
             o << "  long long e = e0 + threadIdx.x;\n  const long long ep = (e < e1) ? e : e0;\n";
-            for (int u = 0; u < NU; ++u)
-                for (int p = 0; p < NPS; ++p)
-                    o << "  " << SLT << " n" << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + ep];\n";
-            for (size_t i = 0; i < M.inv.size(); ++i)
-                if (used[i]) o << "  " << (imap[i] >= 0 ? "double" : "float") << " nx" << i << " = " << inv_raw(i, "ep") << ";\n";
+            decl_set("n", "ep");
+            o << "  double nwt = pw ? wts[ep] : wc;\n";
+            if (depth > 1) {
+                o << "  const long long ep2 = (e + " << WGS << " < e1) ? e + " << WGS << " : e0;\n";
+                decl_set("m", "ep2");
+                o << "  double mwt = pw ? wts[ep2] : wc;\n";
+            }
             o << "  for (; e < e1; e += " << WGS << ") {\n";
             for (int u = 0; u < NU; ++u)
                 for (int p = 0; p < NP; ++p) {
@@ -736,12 +751,22 @@ std::string generate_eval_kernel(const Model_& M) {
                 }
             for (size_t i = 0; i < M.inv.size(); ++i)
                 if (used[i]) o << "    const double x" << i << " = (double)nx" << i << ";\n";
-            o << "    { const long long en = (e + " << WGS << " < e1) ? e + " << WGS << " : e;\n";
+            o << "    const double wev = nwt;\n";
+            const long long ahead = (long long)depth * WGS;
+            o << "    { const long long en = (e + " << ahead << " < e1) ? e + " << ahead << " : e;\n";
+            const char* tgt = depth > 1 ? "m" : "n";
             for (int u = 0; u < NU; ++u)
-                for (int p = 0; p < NPS; ++p)
-                    o << "      n" << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + en];\n";
+                for (int p = 0; p < NPS; ++p) {
+                    if (depth > 1) o << "      n" << u << "_" << p << " = m" << u << "_" << p << ";\n";
+                    o << "      " << tgt << u << "_" << p << " = sl[" << (long long)u * NPS + p << "LL * ld + en];\n";
+                }
             for (size_t i = 0; i < M.inv.size(); ++i)
-                if (used[i]) o << "      nx" << i << " = " << inv_raw(i, "en") << ";\n";
+                if (used[i]) {
+                    if (depth > 1) o << "      nx" << i << " = mx" << i << ";\n";
+                    o << "      " << tgt << "x" << i << " = " << inv_raw(i, "en") << ";\n";
+                }
+            if (depth > 1) o << "      nwt = mwt;\n";
+            o << "      if (pw) " << tgt << "wt = wts[en];\n";
             o << "    }\n";
         } else {
             o << "  for (long long e = e0 + threadIdx.x; e < e1; e += " << WGS << ") {\n";
@@ -792,6 +817,39 @@ std::string generate_eval_kernel(const Model_& M) {
             if (g1 - g0 == 1) {
                 const int d = recip[g0];
                 o << "    const double q" << d << " = frcp(m" << d << ");\n";
+                continue;
+            }
+            if (M.inv_tree) {
+                // balanced product tree: the dependent chain is log2 of the group size deep
+                // instead of the group size (same number of multiplications)
+                auto nm = [&](size_t lo, size_t hi) {
+                    return hi - lo == 1 ? "m" + std::to_string(recip[lo])
+                                        : "u" + std::to_string(lo) + "_" + std::to_string(hi);
+                };
+                std::function<void(size_t, size_t)> prod = [&](size_t lo, size_t hi) {
+                    if (hi - lo == 1) return;
+                    const size_t mid = (lo + hi) / 2;
+                    prod(lo, mid);
+                    prod(mid, hi);
+                    o << "    const double " << nm(lo, hi) << " = " << nm(lo, mid) << " * " << nm(mid, hi) << ";\n";
+                };
+                std::function<void(size_t, size_t, const std::string&)> inv = [&](size_t lo, size_t hi, const std::string& iv) {
+                    if (hi - lo == 1) {
+                        o << "    const double q" << recip[lo] << " = " << iv << ";\n";
+                        return;
+                    }
+                    const size_t mid = (lo + hi) / 2;
+                    const std::string il = "v" + std::to_string(lo) + "_" + std::to_string(mid);
+                    const std::string ir = "v" + std::to_string(mid) + "_" + std::to_string(hi);
+                    o << "    const double " << il << " = " << iv << " * " << nm(mid, hi) << ";\n";
+                    o << "    const double " << ir << " = " << iv << " * " << nm(lo, mid) << ";\n";
+                    inv(lo, mid, il);
+                    inv(mid, hi, ir);
+                };
+                prod(g0, g1);
+                const std::string rt = "v" + std::to_string(g0) + "_" + std::to_string(g1);
+                o << "    const double " << rt << " = frcp(" << nm(g0, g1) << ");\n";
+                inv(g0, g1, rt);
                 continue;
             }
             for (size_t k = g0 + 1; k < g1; ++k) {
@@ -857,7 +915,7 @@ std::string generate_eval_kernel(const Model_& M) {
     for (int p = 0; p < NP; ++p) o << "    double w" << p << "r, w" << p << "i;\n";
     o << "    if (cls == 0) {\n      av += I;\n";
     for (int p = 0; p < NP; ++p) o << "      w" << p << "r = S" << p << "r; w" << p << "i = S" << p << "i;\n";
-    o << "    } else {\n      const double we = pw ? wts[e] : wc;\n"
+    o << "    } else {\n      const double we = " << (M.prefetch_sl && !M.lds_pipe ? "wev" : "pw ? wts[e] : wc") << ";\n"
          "      if (I == 0.0) {\n        av += LOG_FLOOR * we;\n";
     for (int p = 0; p < NP; ++p) o << "        w" << p << "r = 0.0; w" << p << "i = 0.0;\n";
     o << "      } else {\n        av += log(I) * we;\n        const double f = we * frcp(I);\n";

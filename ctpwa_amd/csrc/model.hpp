@@ -232,7 +232,8 @@ struct Model_ {
     int min_waves = 0;       // __launch_bounds__ minimum waves per SIMD (0: compiler choice)
     int wgs = 128;           // threads (= events per tile) of a workgroup: 64, 128 or 256
     bool lds_pipe = false;   // copy the next tile's inputs HBM -> LDS while computing the current one
-    bool prefetch_sl = false; // load the next event's SL values and invariants one iteration ahead (registers)
+    int prefetch_sl = 0;     // load the SL values and invariants of the next 1 or 2 events ahead (registers)
+    bool inv_tree = false;   // simultaneous inversion over a balanced product tree (else a prefix chain)
     // storage precision of the per-event inputs: 0 FP64; 1 mixed (FP32 SL tensors and
     // invariants, FP64 squared masses and Custom-expression m, q)
     int mixed = 0;

@@ -153,7 +153,10 @@ long long ctpwa_footprint_bytes(const char* plan_text, long long n_phsp, long lo
                                 int precision);
 
 /* device bytes a handle allocates on its first ctpwa_hessian call (the per-chunk
- * Jacobian rows and partial sums it keeps until ctpwa_destroy) */
+ * Jacobian rows and partial sums it keeps until ctpwa_destroy).  A handle that is alone
+ * in the process, with at least 2^20 phase-space and 2^17 data + background events, runs
+ * data and background on a second stream with a second buffer set; the estimate counts
+ * that set when no handle is live at the time of this call */
 long long ctpwa_hessian_workspace_bytes(const char* plan_text, long long n_phsp, long long n_data, long long n_bkg);
 
 /* number of events of a class, and algorithmic bytes the fused kernel must read per

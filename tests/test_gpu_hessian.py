@@ -40,14 +40,19 @@ CASES = [
 ]
 
 
+# streams: 1 runs every event class on the caller's stream; 2 runs data and background
+# on the handle's second stream, concurrently with the phase space (the default only for
+# large samples, forced here through CTPWA_HESS_STREAMS, read at the first Hessian)
+@pytest.mark.parametrize("streams", [1, 2], ids=["one-stream", "two-streams"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_hessian_parity(case, oracle_lib):
+def test_hessian_parity(case, streams, oracle_lib, monkeypatch):
     from ctpwa_amd import _native
+    monkeypatch.setenv("CTPWA_HESS_STREAMS", str(streams))
     name, n_p, n_d, n_b, w_b, seed = case
     plan = plan_of(name)
     ph, da, bk = samples(n_p, n_d, n_b, seed)
     wb = None if bk is None else np.full(bk.shape[0], w_b)
-    print("case", name, n_p, n_d, n_b, flush=True)
+    print("case", name, n_p, n_d, n_b, "streams", streams, flush=True)
     h = _native.Handle(plan.to_text(), ph, da, bk, None, None, wb)
     O = oracle_lib.Oracle(plan.to_text(), 3, ph, da, bk, w_bkg=wb)
     base = W.nominal_params(plan, None if name == "jpsi_rho" else 1)
@@ -59,9 +64,11 @@ def test_hessian_parity(case, oracle_lib):
         assert np.max(np.abs(Hg - Hg.T)) <= 1e-9 * np.max(np.abs(Hg))
 
 
-def test_hessian_gauss_and_event_weights(oracle_lib):
+@pytest.mark.parametrize("streams", [1, 2], ids=["one-stream", "two-streams"])
+def test_hessian_gauss_and_event_weights(streams, oracle_lib, monkeypatch):
     """Gaussian constraint on two masses, per-event data and background weights."""
     from ctpwa_amd import _native
+    monkeypatch.setenv("CTPWA_HESS_STREAMS", str(streams))
     text = W.JPSI_3PI + "Constraints:\n  gauss_constr: {rho770_mass: 0.01, f2_1270_width: 0.02}\n"
     plan = plan_of(text)
     assert len(plan.gauss) == 2
