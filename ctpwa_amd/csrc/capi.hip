@@ -494,13 +494,15 @@ static void setup_partition(ctpwa_handle* h, bool scan_phsp) {
     // relative cost of a data or background event: the workgroups of each class get events
     // in inverse proportion, so that the classes finish together.  A phase-space event costs
     // about 20 FP64 instructions per amplitude (sums and gradient accumulators), 10 per
-    // propagator and 8 per SL tensor and polarisation; a data event adds the logarithm, the
-    // reciprocal of I and the clamps, worth about 160 of them (from the fastest measured
-    // split of the 3-pion model, 10 amplitudes: 1.3-1.4; environment CTPWA_DATA_COST
-    // overrides it for measurements)
+    // propagator and 8 per SL tensor and polarisation; a data event of a constant-weight
+    // class adds the two frexp of the product of intensities, the reciprocal of I and the
+    // clamps, about 40 (a class with per-event weights takes a logarithm per event, about
+    // 100 more, as the first form of the kernel did for every data event: there the fastest
+    // split of the 3-pion model was at 1.3-1.4).  Environment CTPWA_DATA_COST overrides it.
     const Model_& M = h->M;
     const double base = 20.0 * M.P.namp + 10.0 * (double)M.dens.size() + 8.0 * (double)M.us.size() * M.P.npol + 40.0;
-    double wcost = 1.0 + 160.0 / base;
+    const bool any_pw = (h->n[1] > 0 && h->pw[1]) || (h->n[2] > 0 && h->pw[2]);
+    double wcost = 1.0 + (any_pw ? 140.0 : 40.0) / base;
     if (const char* e = std::getenv("CTPWA_DATA_COST")) wcost = std::atof(e);
     if (!(wcost >= 0.25 && wcost <= 4.0)) throw std::runtime_error("CTPWA_DATA_COST must lie in [0.25, 4]");
     const double weff = (scan_phsp ? (double)h->n[0] : 0.0) + wcost * (double)(h->n[1] + h->n[2]);

@@ -641,6 +641,10 @@ std::string generate_eval_kernel(const Model_& M) {
     };
     emit_coefs("  ", true);
     o << "  double av = 0.0;\n";
+    // constant-weight data and background classes: sum_e w ln I_e = w ln prod_e I_e, the
+    // product kept as a mantissa in [0.5, 1) and a binary exponent (two frexp per event
+    // instead of a logarithm; one logarithm per thread after the loop)
+    o << "  double lm = 1.0;\n  int le = 0;\n";
     for (int a = 0; a < NA; ++a) o << "  double ax" << a << "r = 0.0, ax" << a << "i = 0.0;\n";
     for (int t = 0; t < NTH; ++t) o << "  double at" << t << " = 0.0;\n";
     const int NCT = (int)M.cterms.size();
@@ -916,9 +920,14 @@ std::string generate_eval_kernel(const Model_& M) {
     o << "    if (cls == 0) {\n      av += I;\n";
     for (int p = 0; p < NP; ++p) o << "      w" << p << "r = S" << p << "r; w" << p << "i = S" << p << "i;\n";
     o << "    } else {\n      const double we = " << (M.prefetch_sl && !M.lds_pipe ? "wev" : "pw ? wts[e] : wc") << ";\n"
-         "      if (I == 0.0) {\n        av += LOG_FLOOR * we;\n";
+         "      if (pw) {\n        av += (I == 0.0 ? LOG_FLOOR : log(I)) * we;\n      } else {\n"
+         "        const double Il = (I == 0.0) ? 1e-10 : I;   // ln(1e-10): the I = 0 rule\n"
+         "        lm *= __builtin_amdgcn_frexp_mant(Il); le += __builtin_amdgcn_frexp_exp(Il);\n"
+         "        const int ex = __builtin_amdgcn_frexp_exp(lm); lm = __builtin_amdgcn_frexp_mant(lm); le += ex;\n"
+         "      }\n"
+         "      if (I == 0.0) {\n";
     for (int p = 0; p < NP; ++p) o << "        w" << p << "r = 0.0; w" << p << "i = 0.0;\n";
-    o << "      } else {\n        av += log(I) * we;\n        const double f = we * frcp(I);\n";
+    o << "      } else {\n        const double f = we * frcp(I);\n";
     for (int p = 0; p < NP; ++p)
         o << "        w" << p << "r = fmin(fmax(S" << p << "r * f, -1e10), 1e10); w" << p << "i = fmin(fmax(S" << p
           << "i * f, -1e10), 1e10);\n";
@@ -1014,6 +1023,11 @@ std::string generate_eval_kernel(const Model_& M) {
         o << "    }\n";
     }
     o << (M.lds_pipe ? "    }\n  }\n" : "  }\n");
+    // ln prod I = ln(mantissa) + exponent ln 2 (ln 2 split into a high part with trailing
+    // zero bits and the remainder, so that the exponent's product stays exact)
+    o << "  if (cls != 0 && !pw)\n"
+         "    av += wc * __builtin_fma((double)le, 1.90821492927058770002e-10,\n"
+         "                              __builtin_fma((double)le, 6.93147180369123816490e-01, log(lm)));\n";
     // reduction
     o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
     auto red = [&](const std::string& v, int k) {

@@ -388,34 +388,39 @@ __device__ __forceinline__ double class_share(const FinArgs& f, int c, int col) 
     return s;
 }
 
+// fixed-order sum over a workgroup of RED_THREADS threads, valid in thread 0: a butterfly
+// within each wave (lane 0 always adds in the same order), then the wave sums in wave order
+// through LDS (one barrier instead of one per tree level)
+__device__ __forceinline__ double wg_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        t = sh[0];
+        for (int k = 1; k < RED_THREADS / 64; ++k) t += sh[k];
+    }
+    return t;
+}
+
 // stage 1: one workgroup per (class, accumulator column); fixed summation order
 __global__ void __launch_bounds__(RED_THREADS) k_reduce(FinArgs f) {
-    __shared__ double sh[RED_THREADS];
+    __shared__ double sh[RED_THREADS / 64];
     int pair = blockIdx.x;
     int c = pair / f.nacc, col = pair - c * f.nacc;
-    sh[threadIdx.x] = class_share(f, c, col);
-    __syncthreads();
-    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) f.totals[pair] = sh[0];
+    const double tot = wg_sum(class_share(f, c, col), sh);
+    if (threadIdx.x == 0) f.totals[pair] = tot;
 }
 
 __global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __restrict__ fins, const int* __restrict__ hdr) {
     if (hdr && (int)blockIdx.y >= hdr[0]) return;
     const FinArgs& f = fins[blockIdx.y];
     if ((int)blockIdx.x >= 3 * f.nacc) return;      // the whole workgroup leaves together
-    __shared__ double sh[RED_THREADS];
+    __shared__ double sh[RED_THREADS / 64];
     int pair = blockIdx.x;
     int c = pair / f.nacc, col = pair - c * f.nacc;
-    sh[threadIdx.x] = class_share(f, c, col);
-    __syncthreads();
-    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) f.totals[pair] = sh[0];
+    const double tot = wg_sum(class_share(f, c, col), sh);
+    if (threadIdx.x == 0) f.totals[pair] = tot;
 }
 
 // stage 2: NLL and gradient from the class totals.  Every global read happens first, in
@@ -515,18 +520,13 @@ __global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) { assemble_
 // resets the counter for the next evaluation.  Results are those of the two kernels.
 static_assert(RED_THREADS == ASM_THREADS, "one workgroup size for both parts");
 __global__ void __launch_bounds__(ASM_THREADS) k_finalize(FinArgs f, unsigned int* __restrict__ done) {
-    __shared__ double sh[RED_THREADS];
+    __shared__ double sh[RED_THREADS / 64];
     __shared__ int last;
     const int pair = blockIdx.x;
     const int c = pair / f.nacc, col = pair - c * f.nacc;
-    sh[threadIdx.x] = class_share(f, c, col);
-    __syncthreads();
-    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
+    const double tot = wg_sum(class_share(f, c, col), sh);
     if (threadIdx.x == 0) {
-        f.totals[pair] = sh[0];
+        f.totals[pair] = tot;
         __threadfence();                                   // the total is visible before the count
         last = atomicAdd(done, 1u) == gridDim.x - 1;
     }
