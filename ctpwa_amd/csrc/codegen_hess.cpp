@@ -106,6 +106,10 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
     o << "  __shared__ double slab[4][64][17];\n"
          "  const long long r0 = (long long)blockIdx.x * 256 + threadIdx.x;\n"
          "  const bool live = r0 < n;\n"
+         // a thread past the chunk's end takes its last event and stores the same values as
+         // that event's own thread: its stores need no guard.  Guarded stores (a divergent
+         // branch each) split a large model's straight-line code into hundreds of blocks,
+         // around which the register allocator spilled: 17 % of the 149-amplitude Hessian
          "  const long long r = live ? r0 : n - 1;\n"
          "  const int wv = threadIdx.x >> 6;\n"
          "  WaveColSums cs_{slab[wv], cs + (long long)blockIdx.x * 4 + wv, cs_ld, (int)(threadIdx.x & 63), 0, 0};\n"
@@ -215,8 +219,8 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
                 std::string sn = "s" + std::to_string(u) + "_" + std::to_string(p);
                 o << "      { const double ar = Fr * " << sn << "x - Fi * " << sn << "y, ai = Fr * " << sn << "y + Fi * " << sn << "x;\n"
                   << "        S" << p << "r += vr * ar - vi * ai; S" << p << "i += vr * ai + vi * ar;\n"
-                  << "        if (live) { J[" << a << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")ar; J[" << C + a << "LL * ldJ + " << p
-                  << "LL * n + r] = (" << JT << ")ai; } }\n";
+                  << "        J[" << a << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")ar; J[" << C + a << "LL * ldJ + " << p
+                  << "LL * n + r] = (" << JT << ")ai; }\n";
             }
             o << "    }\n";
         }
@@ -229,8 +233,8 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
          "  if (cls == 0) { alpha = 1.0; beta = 0.0; }\n"
          "  else if (I == 0.0) { alpha = 0.0; beta = 0.0; }\n"
          "  else { const double iI = frcp(I); alpha = we * iI; beta = -alpha * iI; }\n"
-         "  if (live) { W[r] = alpha; W[ldc + r] = beta; }\n"
-         "  const bool ust = live && cls != 0;\n";
+         "  W[r] = alpha; W[ldc + r] = beta;\n"
+         "  const bool ust = cls != 0;   // wave-uniform: a scalar branch around each u store\n";
 
     // ---- phase 2: u, derivative columns and second-part scalars --------------------
     // canonical column of every summed value: Z in plan block order, then alpha I, then u
@@ -417,8 +421,8 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
         for (int j : vars)
             if (last[j] == (int)kb) {
                 for (int p = 0; p < NP; ++p)
-                    o << "  if (live) { J[" << NA + j << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")dS" << j << "_" << p << "r; J[" << C + NA + j
-                      << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")dS" << j << "_" << p << "i; }\n";
+                    o << "  J[" << NA + j << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")dS" << j << "_" << p << "r; J[" << C + NA + j
+                      << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")dS" << j << "_" << p << "i;\n";
                 ustore("ut" + std::to_string(j), 2 * NA + j, "  ");
             }
     }
@@ -426,8 +430,8 @@ std::string generate_hess_kernel(const Model_& M, std::vector<int>* perm) {
     for (int j = 0; j < NT; ++j)
         if (last[j] < 0) {
             for (int p = 0; p < NP; ++p)
-                o << "  if (live) { J[" << NA + j << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")0.0; J[" << C + NA + j << "LL * ldJ + "
-                  << p << "LL * n + r] = (" << JT << ")0.0; }\n";
+                o << "  J[" << NA + j << "LL * ldJ + " << p << "LL * n + r] = (" << JT << ")0.0; J[" << C + NA + j << "LL * ldJ + "
+                  << p << "LL * n + r] = (" << JT << ")0.0;\n";
             ustore("0.0", 2 * NA + j, "  ");
         }
     o << "  if (cs_.n > 0) cs_.flush();\n}\n";
