@@ -21,9 +21,12 @@
 // Inputs are read straight from HBM, or (pipelined form) copied tile by tile into LDS
 // by the DMA path while the previous tile is computed; a deterministic wave/LDS
 // reduction then writes one partial row per workgroup.
-#include <sstream>
+#include <algorithm>
 #include <functional>
+#include <sstream>
+#include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "model.hpp"
 
@@ -1028,17 +1031,33 @@ std::string generate_eval_kernel(const Model_& M) {
     o << "  if (cls != 0 && !pw)\n"
          "    av += wc * __builtin_fma((double)le, 1.90821492927058770002e-10,\n"
          "                              __builtin_fma((double)le, 6.93147180369123816490e-01, log(lm)));\n";
-    // reduction
-    o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n";
-    auto red = [&](const std::string& v, int k) {
-        o << "  { double s = " << v << "; for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);"
-             " if (lane == 0) red[wave][" << k << "] = s; }\n";
-    };
-    red("av", 0);
-    for (int a = 0; a < NA; ++a) { red("ax" + std::to_string(a) + "r", 1 + 2 * a); red("ax" + std::to_string(a) + "i", 2 + 2 * a); }
-    for (int t = 0; t < NTH; ++t) red("at" + std::to_string(t), 1 + 2 * NA + t);
-    for (int t = 0; t < NCT; ++t) red("ct" + std::to_string(t), 1 + 2 * NA + NTH + t);
-    o << "  __syncthreads();\n";
+    // reduction over the wave: 16 accumulators at a time are written to the wave's LDS slab
+    // (rows padded to 17 doubles), lane 4k + q adds column k over lanes 16q .. 16q + 15 in
+    // lane order, and two exchanges join the quarters (a fixed order); per 16 accumulators
+    // that is 16 LDS writes, 16 reads and 2 exchanges instead of 6 exchanges each
+    std::vector<std::string> accs;
+    accs.push_back("av");
+    for (int a = 0; a < NA; ++a) { accs.push_back("ax" + std::to_string(a) + "r"); accs.push_back("ax" + std::to_string(a) + "i"); }
+    for (int t = 0; t < NTH; ++t) accs.push_back("at" + std::to_string(t));
+    for (int t = 0; t < NCT; ++t) accs.push_back("ct" + std::to_string(t));
+    if ((int)accs.size() != NACC) throw std::runtime_error("fused kernel generator: accumulator count mismatch");
+    if (M.lds_pipe) o << "  __syncthreads();                 // every wave is done with the tile buffers\n";
+    o << "  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;\n"
+         "  {\n    double (*tl)[17] = (double (*)[17])(lbuf + wave * " << 64 * 17 * 8 << ");\n"
+         "    const int k4 = lane >> 2, q4 = lane & 3;\n";
+    for (int c0 = 0; c0 < NACC; c0 += 16) {
+        const int m = std::min(16, NACC - c0);
+        for (int k = 0; k < m; ++k) o << "    tl[lane][" << k << "] = " << accs[c0 + k] << ";\n";
+        o << "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\"); __builtin_amdgcn_wave_barrier();\n"
+             "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");\n"
+             "    { double s = tl[16 * q4][k4];\n"
+             "      for (int t = 1; t < 16; ++t) s += tl[16 * q4 + t][k4];\n"
+             "      s += __shfl_xor(s, 1, 64); s += __shfl_xor(s, 2, 64);\n"
+             "      if (q4 == 0 && k4 < " << m << ") red[wave][" << c0 << " + k4] = s; }\n"
+             "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\"); __builtin_amdgcn_wave_barrier();\n"
+             "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");\n";
+    }
+    o << "  }\n  __syncthreads();\n";
     // fixed-order combination of the wave partials
     std::string comb;
     if (NW == 1) comb = "red[0][k]";
@@ -1050,7 +1069,8 @@ std::string generate_eval_kernel(const Model_& M) {
     std::string lb = "__launch_bounds__(" + std::to_string(WGS) + (M.min_waves > 0 ? ", " + std::to_string(M.min_waves) : std::string()) + ")";
     std::ostringstream decl;
     decl << "  __shared__ double red[" << NW << "][" << NACC << "];\n"
-         << "  __shared__ __attribute__((aligned(16))) unsigned char lbuf[" << (M.lds_pipe ? 2 * tile_bytes : 16) << "];\n";
+         << "  __shared__ __attribute__((aligned(16))) unsigned char lbuf["
+         << std::max<long long>(M.lds_pipe ? 2 * tile_bytes : 0, (long long)NW * 64 * 17 * 8) << "];\n";
     o << "extern \"C\" __global__ void " << lb << " ctpwa_eval(\n"
          "    const double* __restrict__ tab, const void* __restrict__ slv, const double* __restrict__ inv,\n"
          "    const float* __restrict__ invf, const double* __restrict__ wts, double* __restrict__ part,\n"
