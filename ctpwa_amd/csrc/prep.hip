@@ -424,21 +424,67 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce_batch(const FinArgs* __r
 }
 
 // stage 2: NLL and gradient from the class totals.  Every global read happens first, in
-// parallel over the threads: the class-combined accumulators, then each gradient entry's
-// weight (scale times its table coefficient) and its accumulator indices.  The gradient
-// rows are then fixed-order sums over LDS (one wave per theta with a butterfly sum), so
-// no thread waits on a chain of dependent global loads.
+// parallel over the threads: the gradient entries' weights (scale times their table
+// coefficient), accumulator indices and row pointers (assemble_prefill, which needs no
+// totals and so runs in k_finalize before a workgroup's reduction, off the last
+// workgroup's path), then the class-combined accumulators.  The gradient rows are then
+// fixed-order sums over LDS (one wave per theta with a butterfly sum), so no thread waits
+// on a chain of dependent global loads.
 // LDS layout: assemble_lds_bytes (device.hpp)
-__device__ __forceinline__ void assemble_body(const FinArgs& f) {
-    extern __shared__ double C[];            // [nacc] = -data + bkg + cp * phsp
-    double* wcp = C + f.nacc;
-    double* wth = wcp + f.ncpe;
-    double* vre = wth + f.nthe;
-    double* vim = vre + f.nthe;
-    int* icp = (int*)(vim + f.nthe);
-    int* ith = icp + f.ncpe;
-    int* cpp = ith + f.nthe;
-    int* thp = cpp + f.nfree + 1;
+struct AsmLds {
+    double *C, *wcp, *wth, *vre, *vim;
+    int *icp, *ith, *cpp, *thp;
+};
+__device__ __forceinline__ AsmLds asm_lds(const FinArgs& f) {
+    extern __shared__ double smem_asm[];
+    AsmLds L;
+    L.C = smem_asm;                          // [nacc] = -data + bkg + cp * phsp
+    L.wcp = L.C + f.nacc;
+    L.wth = L.wcp + f.ncpe;
+    L.vre = L.wth + f.nthe;
+    L.vim = L.vre + f.nthe;
+    L.icp = (int*)(L.vim + f.nthe);
+    L.ith = L.icp + f.ncpe;
+    L.cpp = L.ith + f.nthe;
+    L.thp = L.cpp + f.nfree + 1;
+    return L;
+}
+__device__ __forceinline__ void assemble_prefill(const FinArgs& f) {
+    const AsmLds L = asm_lds(f);
+    const int tid = threadIdx.x;
+    // coupling entries: weight r_a K_a, accumulator pair (1 + 2a, 2 + 2a)
+    for (int k = tid; k < f.ncpe; k += ASM_THREADS) {
+        DGEnt e = f.cpe[k];
+        L.wcp[k] = e.scale * f.table[f.off_rec + e.rec];
+        L.icp[k] = e.idx;
+    }
+    // theta entries: idx >= 0 -> w C[idx]; idx < 0 -> w (vr C[1 + 2a] + vi C[2 + 2a])
+    for (int k = tid; k < f.nthe; k += ASM_THREADS) {
+        DGEnt e = f.the[k];
+        double d = (e.rec < 0) ? 1.0 : f.table[f.off_rec + e.rec];
+        L.wth[k] = e.scale * d;
+        L.ith[k] = e.idx;
+        if (e.idx < 0) {
+            int a = -e.idx - 1;
+            L.vre[k] = f.table[f.off_v + 2 * a];
+            L.vim[k] = f.table[f.off_v + 2 * a + 1];
+        }
+    }
+    for (int k = tid; k <= f.nfree; k += ASM_THREADS) L.cpp[k] = f.cpp[k];
+    for (int k = tid; k <= f.ntheta; k += ASM_THREADS) L.thp[k] = f.thp[k];
+}
+// the assembly proper, after assemble_prefill and a barrier
+__device__ __forceinline__ void assemble_main(const FinArgs& f) {
+    const AsmLds L = asm_lds(f);
+    double* C = L.C;
+    const double* wcp = L.wcp;
+    const double* wth = L.wth;
+    const double* vre = L.vre;
+    const double* vim = L.vim;
+    const int* icp = L.icp;
+    const int* ith = L.ith;
+    const int* cpp = L.cpp;
+    const int* thp = L.thp;
     int tid = threadIdx.x;
     const double* tot = f.totals;            // tot[c * nacc + col]
     double Np = f.Np, Neff = f.Wd - f.Wb;
@@ -448,26 +494,6 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
     const double* theta = f.params + 2 * f.nfree;
     for (int col = tid; col < f.nacc; col += ASM_THREADS)
         C[col] = -tot[f.nacc + col] + tot[2 * f.nacc + col] + cp * tot[col];
-    // coupling entries: weight r_a K_a, accumulator pair (1 + 2a, 2 + 2a)
-    for (int k = tid; k < f.ncpe; k += ASM_THREADS) {
-        DGEnt e = f.cpe[k];
-        wcp[k] = e.scale * f.table[f.off_rec + e.rec];
-        icp[k] = e.idx;
-    }
-    // theta entries: idx >= 0 -> w C[idx]; idx < 0 -> w (vr C[1 + 2a] + vi C[2 + 2a])
-    for (int k = tid; k < f.nthe; k += ASM_THREADS) {
-        DGEnt e = f.the[k];
-        double d = (e.rec < 0) ? 1.0 : f.table[f.off_rec + e.rec];
-        wth[k] = e.scale * d;
-        ith[k] = e.idx;
-        if (e.idx < 0) {
-            int a = -e.idx - 1;
-            vre[k] = f.table[f.off_v + 2 * a];
-            vim[k] = f.table[f.off_v + 2 * a + 1];
-        }
-    }
-    for (int k = tid; k <= f.nfree; k += ASM_THREADS) cpp[k] = f.cpp[k];
-    for (int k = tid; k <= f.ntheta; k += ASM_THREADS) thp[k] = f.thp[k];
     double loss = -tot[f.nacc] + tot[2 * f.nacc] + Neff * log(pf);
     for (int g = 0; g < f.ngauss; ++g) {
         double d = theta[f.g_slot[g]] - f.g_mu[g];
@@ -512,6 +538,12 @@ __device__ __forceinline__ void assemble_body(const FinArgs& f) {
     }
 }
 
+__device__ __forceinline__ void assemble_body(const FinArgs& f) {
+    assemble_prefill(f);
+    __syncthreads();
+    assemble_main(f);
+}
+
 __global__ void __launch_bounds__(ASM_THREADS) k_assemble(FinArgs f) { assemble_body(f); }
 
 // k_reduce and k_assemble in one launch: every workgroup sums one (class, accumulator)
@@ -524,6 +556,9 @@ __global__ void __launch_bounds__(ASM_THREADS) k_finalize(FinArgs f, unsigned in
     __shared__ int last;
     const int pair = blockIdx.x;
     const int c = pair / f.nacc, col = pair - c * f.nacc;
+    // the assembly's inputs that need no totals, loaded by every workgroup while its
+    // reduction loads are in flight (any workgroup may turn out to be the last)
+    assemble_prefill(f);
     const double tot = wg_sum(class_share(f, c, col), sh);
     if (threadIdx.x == 0) {
         f.totals[pair] = tot;
@@ -533,7 +568,7 @@ __global__ void __launch_bounds__(ASM_THREADS) k_finalize(FinArgs f, unsigned in
     __syncthreads();
     if (!last) return;
     __threadfence();                                       // every total is visible to this workgroup
-    assemble_body(f);
+    assemble_main(f);
     if (threadIdx.x == 0) *done = 0u;
 }
 
