@@ -57,19 +57,15 @@ def _read_weights(entry, base_dir: str, n: int) -> Optional[np.ndarray]:
     return w[:n]
 
 
-class NLLFunction(torch.autograd.Function):
-    """Forward runs the fused NLL+gradient; backward scales the stored gradient."""
+class NLLFunction:
+    """The reference's NLLFunction (src/Analysis.cu:625-1261): ``apply(params, ana)`` gives
+    the NLL as a 0-dim tensor whose backward is the analytic gradient.  The autograd
+    function itself is C++ (``ctpwa_amd/csrc/torch_ops.cpp``): one fused NLL+gradient call
+    in forward, gradient times grad_out in backward."""
 
     @staticmethod
-    def forward(ctx, params, ana):
-        out = ana._nll_grad_tensor(params)
-        ctx.save_for_backward(out[1:])
-        return out[0]
-
-    @staticmethod
-    def backward(ctx, grad_out):
-        (g,) = ctx.saved_tensors
-        return g * grad_out, None
+    def apply(params, ana):
+        return ana.getNLL(params)
 
 
 class analysis:
@@ -131,6 +127,10 @@ class analysis:
         self._theta_on_host = True
         self._h = _native.Handle(self.plan.to_text(), s["phsp"], s["data"], s.get("bkg"),
                                  w["phsp"], w["data"], w["bkg"], device=device, precision=precision)
+        self._ops = _native.torch_ops()
+        self._hptr = self._h.h.value
+        self._dev_index = device
+        self._n_params = self.plan.n_params()
         self._valid = True
 
     # ---- reference API -------------------------------------------------------
@@ -139,7 +139,8 @@ class analysis:
 
     def getNLL(self, params: torch.Tensor) -> torch.Tensor:
         self._check_params(params)
-        return NLLFunction.apply(params, self)
+        self._remember_theta(params)
+        return self._ops.nll(params, self._hptr, _native.current_stream(self._dev_index), self._n_params)
 
     def getHessian(self, params: torch.Tensor) -> torch.Tensor:
         self._check_params(params)
@@ -147,7 +148,7 @@ class analysis:
         self._remember_theta(p)
         n = self.getNParams()
         H = torch.empty((n, n), dtype=torch.float64, device=self.device)
-        self._h.hessian(p.data_ptr(), H.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
+        self._h.hessian(p.data_ptr(), H.data_ptr(), _native.current_stream(self._dev_index))
         return H
 
     def getNVector(self) -> int:
@@ -334,7 +335,7 @@ class analysis:
         out = torch.empty((len(self.plan.waves) + 1, n), dtype=torch.float64, device=self.device)
         if n:
             self._h.partial_intensities(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
-                                        torch.cuda.current_stream(self.device).cuda_stream)
+                                        _native.current_stream(self._dev_index))
         return out
 
     def getPartialAmplitudes(self, params: torch.Tensor, cls: int = 0) -> torch.Tensor:
@@ -345,7 +346,7 @@ class analysis:
         out = torch.empty((len(self.plan.waves), n, self.plan.npol, 2), dtype=torch.float64, device=self.device)
         if n:
             self._h.partial_amplitudes(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
-                                       torch.cuda.current_stream(self.device).cuda_stream)
+                                       _native.current_stream(self._dev_index))
         return torch.view_as_complex(out)
 
     def getAmplitudeGram(self, params: torch.Tensor, cls: int = 0) -> np.ndarray:
@@ -360,7 +361,7 @@ class analysis:
         out = torch.empty(n, dtype=torch.float64, device=self.device)
         if n:
             self._h.intensity(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
-                              torch.cuda.current_stream(self.device).cuda_stream)
+                              _native.current_stream(self._dev_index))
         return out
 
     def getAmplitudes(self, params: torch.Tensor, cls: int = 1) -> torch.Tensor:
@@ -370,13 +371,13 @@ class analysis:
         out = torch.empty((n, self.plan.npol, self.plan.n_amp, 2), dtype=torch.float64, device=self.device)
         if n:
             self._h.amplitudes(params.detach().contiguous().data_ptr(), cls, out.data_ptr(),
-                               torch.cuda.current_stream(self.device).cuda_stream)
+                               _native.current_stream(self._dev_index))
         return torch.view_as_complex(out)
 
     def kernelTimeMs(self, params: torch.Tensor, iters: int = 20) -> float:
         self._check_params(params)
         return self._h.time_eval_kernel(params.detach().contiguous().data_ptr(), iters,
-                                        torch.cuda.current_stream(self.device).cuda_stream)
+                                        _native.current_stream(self._dev_index))
 
     def layout(self):
         return self._h.layout()
@@ -389,8 +390,8 @@ class analysis:
             raise RuntimeError("params must be float64")
         if not params.is_cuda or params.device != self.device:
             raise RuntimeError("params must be on %s" % self.device)
-        if params.dim() != 1 or params.numel() != self.getNParams():
-            raise RuntimeError("params must be 1-D with %d entries" % self.getNParams())
+        if params.dim() != 1 or params.numel() != self._n_params:
+            raise RuntimeError("params must be 1-D with %d entries" % self._n_params)
 
     def _stored_amplitudes(self, cls: int) -> torch.Tensor:
         plan = self.plan
@@ -417,7 +418,7 @@ class analysis:
     @property
     def _theta(self) -> np.ndarray:
         if not self._theta_on_host:
-            self._theta_host = self._h.last_theta(torch.cuda.current_stream(self.device).cuda_stream)
+            self._theta_host = self._h.last_theta(_native.current_stream(self._dev_index))
             self._theta_on_host = True
         return self._theta_host
 
@@ -436,7 +437,7 @@ class analysis:
     def _gram(self, handle, p: torch.Tensor, cls: int) -> np.ndarray:
         na = self.plan.n_amp
         out = torch.empty((na, na, 2), dtype=torch.float64, device=self.device)
-        handle.amplitude_gram(p.data_ptr(), cls, out.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
+        handle.amplitude_gram(p.data_ptr(), cls, out.data_ptr(), _native.current_stream(self._dev_index))
         o = out.cpu().numpy()
         return o[..., 0] + 1j * o[..., 1]
 
@@ -454,9 +455,3 @@ class analysis:
                                          precision=self.precision)
         return self._gram(self._truth, p, 0)
 
-    def _nll_grad_tensor(self, params: torch.Tensor) -> torch.Tensor:
-        p = params.detach().contiguous()
-        self._remember_theta(p)
-        out = torch.empty(1 + self.getNParams(), dtype=torch.float64, device=self.device)
-        self._h.nll_grad(p.data_ptr(), out.data_ptr(), torch.cuda.current_stream(self.device).cuda_stream)
-        return out

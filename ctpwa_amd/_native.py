@@ -10,6 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libctpwa_amd.so")
+TORCH_EXT_PATH = os.path.join(HERE, "_ctpwa_torch.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -28,10 +29,33 @@ EXPORTS = [
 ]
 
 _lib = None
+_ops = None
 
 
 class NativeError(RuntimeError):
     pass
+
+
+def torch_ops():
+    """The PyTorch extension over the C ABI (``_ctpwa_torch.so``): ``nll(params, handle,
+    stream, n_params)``, the likelihood as a C++ autograd function.  It links the same
+    ``libctpwa_amd.so`` that :func:`lib` loads (one copy in the process)."""
+    global _ops
+    if _ops is not None:
+        return _ops
+    if not os.path.exists(TORCH_EXT_PATH):
+        raise NativeError("torch extension %s is not built; run `python -m ctpwa_amd.build`" % TORCH_EXT_PATH)
+    import torch  # noqa: F401  (the extension resolves torch's libraries)
+    lib()
+    from . import _ctpwa_torch
+    _ops = _ctpwa_torch
+    return _ops
+
+
+def current_stream(device_index: int) -> int:
+    """The current torch stream of a device, as an integer handle for the C ABI."""
+    import torch
+    return torch._C._cuda_getCurrentRawStream(device_index)
 
 
 def lib():

@@ -71,6 +71,38 @@ def test_analysis_api_matches_oracle(tmp_path, oracle_lib):
         ana.getNLL(p.detach()[:-1])
 
 
+def test_getnll_is_the_c_abi_result_on_the_current_stream(tmp_path):
+    """getNLL (the C++ autograd function) returns exactly what ctpwa_nll_grad writes, for
+    a non-contiguous parameter view and on a side stream made current; its backward
+    scales the gradient by the incoming gradient."""
+    import ctpwa_amd as ctpwa
+    ph, da, bk = samples(20000, 4000, 1000, 53)
+    cfg = _write_cfg(tmp_path, "jpsi_3pi", ph, da, bk)
+    print("case getNLL vs C ABI", flush=True)
+    ana = ctpwa.analysis(cfg)
+    n = ana.getNParams()
+    p0 = perturbed(W.nominal_params(ana.plan, 1), ana.plan, 14)
+    t = torch.tensor(p0, dtype=torch.float64, device="cuda")
+    out = torch.empty(1 + n, dtype=torch.float64, device="cuda")
+    ana._h.nll_grad(t.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ref = out.cpu().numpy()
+    big = torch.zeros(2 * n, dtype=torch.float64, device="cuda")
+    big[::2] = t
+    big.requires_grad_(True)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        nll = ana.getNLL(big[::2])
+        (g,) = torch.autograd.grad(2.5 * nll, big)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    assert nll.item() == ref[0]
+    gg = g.cpu().numpy()
+    assert np.array_equal(gg[::2], 2.5 * ref[1:])
+    assert np.all(gg[1::2] == 0.0)
+
+
 def test_edge_samples(oracle_lib):
     """One data event, no background; a data sample whose weights are all zero."""
     from ctpwa_amd import _native

@@ -29,6 +29,19 @@ def test_library_exports_header_symbols():
     assert L.ctpwa_version().decode().startswith("ctpwa_amd")
 
 
+def test_torch_extension_loads():
+    """The PyTorch extension over the C ABI loads beside the library and exposes the
+    likelihood's autograd function; a call with a bad handle reports the library's error
+    instead of crashing (the handle is checked before any device work)."""
+    import torch
+    from ctpwa_amd import _native
+    ops = _native.torch_ops()
+    assert callable(ops.nll)
+    p = torch.zeros(3, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(RuntimeError, match="ctpwa_nll_grad"):
+        ops.nll(p, 0, 0, 3)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("name", ["jpsi_rho", "jpsi_3pi", "jpsi_kkpipi"])
 def test_generated_kernel_compiles_for_gfx950(name, precision):
