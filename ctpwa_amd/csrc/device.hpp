@@ -82,6 +82,11 @@ struct PrepArgs {
     int o_blocks, o_res, o_nodes, o_topos, o_slcL, o_slc0, o_dens, o_amps;
 };
 
+// dynamic LDS of the prep kernels: the staged model tables, then the parameter vector
+inline size_t prep_lds_bytes(const PrepArgs& a) {
+    return (size_t)a.meta_bytes + sizeof(double) * (size_t)(2 * a.nfree + a.ntheta);
+}
+
 // one term of a gradient contraction in the final assembly
 //   theta entries: idx >= 0  -> scale * dcoef * C[idx]          (propagator coefficient)
 //                  idx <  0  -> scale * dK * Re(v_a conj(X_a)), a = -idx - 1

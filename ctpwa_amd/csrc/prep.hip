@@ -200,14 +200,18 @@ __device__ inline void store(double* t, const DJ<NV, H>& x) {
 }
 
 template <int NV, bool H>
-__device__ __forceinline__ void prep_body(const double* __restrict__ params, double* __restrict__ table, PrepArgs a) {
+__device__ __forceinline__ void prep_body(const double* __restrict__ gparams, double* __restrict__ table, PrepArgs a) {
     using J = DJ<NV, H>;
     extern __shared__ uint4 smeta[];
+    // the parameter vector goes to LDS behind the model tables (meta_bytes is a multiple of
+    // 16), loaded in the same phase as the tables, so the item's chain starts with both
+    // in LDS instead of waiting for a dependent global load of its parameters
+    double* params = (double*)((char*)smeta + a.meta_bytes);
+    for (int i = threadIdx.x; i < 2 * a.nfree + a.ntheta; i += blockDim.x) params[i] = gparams[i];
     if (a.meta_bytes > 0) {
         // the model tables are read many times with dependent indices: stage them in LDS
         const uint4* src = (const uint4*)a.meta;
         for (int i = threadIdx.x; i < a.meta_bytes / 16; i += blockDim.x) smeta[i] = src[i];
-        __syncthreads();
         const char* base = (const char*)smeta;
         a.blocks = (const DBlock*)(base + a.o_blocks);
         a.res = (const DRes*)(base + a.o_res);
@@ -218,6 +222,7 @@ __device__ __forceinline__ void prep_body(const double* __restrict__ params, dou
         a.dens = (const DDen*)(base + a.o_dens);
         a.amps = (const DAmp*)(base + a.o_amps);
     }
+    __syncthreads();
     // one item (denominator or amplitude record) per workgroup: the items' chains of
     // dependent jet arithmetic then run on different compute units at the same time instead
     // of one after another in a single divergent wave
@@ -756,7 +761,7 @@ static hipError_t launch_prep_t(const double* params, double* table, const ctpwa
     int items = a->nden + a->namp;
     ctpwa::PrepArgs b = *a;
     b.remember = remember;
-    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3(items), dim3(64), (size_t)a->meta_bytes, stream, params, table, b);
+    hipLaunchKernelGGL((ctpwa::k_prep<NV, H>), dim3(items), dim3(64), ctpwa::prep_lds_bytes(*a), stream, params, table, b);
     return hipGetLastError();
 }
 
@@ -777,7 +782,7 @@ template <int NV, bool H>
 static hipError_t launch_prep_batch_t(const double* const* params, double* const* tables, const ctpwa::PrepArgs* a,
                                       const ctpwa::PrepArgs* args, int rows, const int* hdr, hipStream_t stream) {
     int items = a->nden + a->namp;
-    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3(items, rows), dim3(64), (size_t)a->meta_bytes, stream, params,
+    hipLaunchKernelGGL((ctpwa::k_prep_batch<NV, H>), dim3(items, rows), dim3(64), ctpwa::prep_lds_bytes(*a), stream, params,
                        tables, *a, args, hdr);
     return hipGetLastError();
 }
