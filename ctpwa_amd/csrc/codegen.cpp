@@ -911,11 +911,42 @@ std::string generate_eval_kernel(const Model_& M) {
         o << "    y" << u << "r += c" << a << "r * br" << b << "; y" << u << "r -= c" << a << "i * bi" << b << "; y" << u
           << "i += c" << a << "r * bi" << b << "; y" << u << "i += c" << a << "i * br" << b << ";\n";
     }
-    for (int p = 0; p < NP; ++p) {
-        o << "    double S" << p << "r = 0.0, S" << p << "i = 0.0;\n";
-        for (int u = 0; u < NU; ++u)
-            o << "    S" << p << "r += y" << u << "r * s" << u << "_" << p << ".x; S" << p << "r -= y" << u << "i * s" << u << "_" << p
-              << ".y; S" << p << "i += y" << u << "r * s" << u << "_" << p << ".y; S" << p << "i += y" << u << "i * s" << u << "_" << p << ".x;\n";
+    // mirror-image polarisations (one stored, sl_u,1 = g_u conj(sl_u,0)): with U_G = sum over
+    // the SL tensors of sign g_u = G of Re(y_u) sl_u,0 and V_G the same with Im(y_u),
+    // S_0 = U + i V and S_1 = conj(U') + i conj(V') (U = U_+ + U_-, U' = U_+ - U_-): 4
+    // products per SL tensor for both polarisations instead of 8
+    const bool mirror_pair = (NP == 2 && NPS == 1);
+    bool gpos = false, gneg = false;
+    for (int u = 0; u < NU && mirror_pair; ++u) (M.psign[u] > 0 ? gpos : gneg) = true;
+    if (mirror_pair) {
+        for (int G = 0; G < 2; ++G) {
+            if (!(G == 0 ? gpos : gneg)) continue;
+            const char* g = G == 0 ? "P" : "N";
+            o << "    double U" << g << "r = 0.0, U" << g << "i = 0.0, V" << g << "r = 0.0, V" << g << "i = 0.0;\n";
+            for (int u = 0; u < NU; ++u) {
+                if ((M.psign[u] > 0) != (G == 0)) continue;
+                o << "    U" << g << "r += y" << u << "r * s" << u << "_0.x; U" << g << "i += y" << u << "r * s" << u << "_0.y; V" << g
+                  << "r += y" << u << "i * s" << u << "_0.x; V" << g << "i += y" << u << "i * s" << u << "_0.y;\n";
+            }
+        }
+        if (gpos && gneg)
+            o << "    const double Ur = UPr + UNr, Ui = UPi + UNi, Vr = VPr + VNr, Vi = VPi + VNi;\n"
+                 "    const double Ur_ = UPr - UNr, Ui_ = UPi - UNi, Vr_ = VPr - VNr, Vi_ = VPi - VNi;\n";
+        else if (gpos)
+            o << "    const double Ur = UPr, Ui = UPi, Vr = VPr, Vi = VPi;\n"
+                 "    const double Ur_ = UPr, Ui_ = UPi, Vr_ = VPr, Vi_ = VPi;\n";
+        else
+            o << "    const double Ur = UNr, Ui = UNi, Vr = VNr, Vi = VNi;\n"
+                 "    const double Ur_ = -UNr, Ui_ = -UNi, Vr_ = -VNr, Vi_ = -VNi;\n";
+        o << "    const double S0r = Ur - Vi, S0i = Ui + Vr;\n"
+             "    const double S1r = Ur_ + Vi_, S1i = Vr_ - Ui_;\n";
+    } else {
+        for (int p = 0; p < NP; ++p) {
+            o << "    double S" << p << "r = 0.0, S" << p << "i = 0.0;\n";
+            for (int u = 0; u < NU; ++u)
+                o << "    S" << p << "r += y" << u << "r * s" << u << "_" << p << ".x; S" << p << "r -= y" << u << "i * s" << u << "_" << p
+                  << ".y; S" << p << "i += y" << u << "r * s" << u << "_" << p << ".y; S" << p << "i += y" << u << "i * s" << u << "_" << p << ".x;\n";
+        }
     }
     o << "    double I = 0.0;\n";
     for (int p = 0; p < NP; ++p) o << "    I += S" << p << "r * S" << p << "r; I += S" << p << "i * S" << p << "i;\n";
@@ -935,11 +966,23 @@ std::string generate_eval_kernel(const Model_& M) {
         o << "        w" << p << "r = fmin(fmax(S" << p << "r * f, -1e10), 1e10); w" << p << "i = fmin(fmax(S" << p
           << "i * f, -1e10), 1e10);\n";
     o << "      }\n    }\n";
-    for (int u = 0; u < NU; ++u) {
-        o << "    double t" << u << "r = 0.0, t" << u << "i = 0.0;\n";
-        for (int p = 0; p < NP; ++p)
-            o << "    t" << u << "r += s" << u << "_" << p << ".x * w" << p << "r; t" << u << "r += s" << u << "_" << p << ".y * w" << p
-              << "i; t" << u << "i += s" << u << "_" << p << ".x * w" << p << "i; t" << u << "i -= s" << u << "_" << p << ".y * w" << p << "r;\n";
+    if (mirror_pair) {
+        // T_u = conj(sl_u,0) w_0 + g_u sl_u,0 w_1 from the sums and differences of w_0, w_1:
+        // g = +1: (a sr + b di) + i (a si - b dr); g = -1: (a dr + b si) + i (a di - b sr)
+        o << "    const double wsr = w0r + w1r, wsi = w0i + w1i, wdr = w0r - w1r, wdi = w0i - w1i;\n";
+        for (int u = 0; u < NU; ++u) {
+            const bool pos = M.psign[u] > 0;
+            o << "    const double t" << u << "r = s" << u << "_0.x * " << (pos ? "wsr" : "wdr") << " + s" << u << "_0.y * "
+              << (pos ? "wdi" : "wsi") << ", t" << u << "i = s" << u << "_0.x * " << (pos ? "wsi" : "wdi") << " - s" << u
+              << "_0.y * " << (pos ? "wdr" : "wsr") << ";\n";
+        }
+    } else {
+        for (int u = 0; u < NU; ++u) {
+            o << "    double t" << u << "r = 0.0, t" << u << "i = 0.0;\n";
+            for (int p = 0; p < NP; ++p)
+                o << "    t" << u << "r += s" << u << "_" << p << ".x * w" << p << "r; t" << u << "r += s" << u << "_" << p << ".y * w" << p
+                  << "i; t" << u << "i += s" << u << "_" << p << ".x * w" << p << "i; t" << u << "i -= s" << u << "_" << p << ".y * w" << p << "r;\n";
+        }
     }
     // per block: the coupling sums P_a = conj(Psi_b) T_u (X_a += P_a), then the theta
     // terms R_d = Z_b Psi_b Psi_d with Z_b Psi_b = sum_{a in b} c_a conj(P_a)
