@@ -146,6 +146,23 @@ def run_form(ana, args, spec, torch, W, prec, sizes, label=None, hessian=True):
     sync = {"value": round(args.steps / dt_sync, 3), "ms_per_step": round(dt_sync / args.steps * 1e3, 4),
             "def": "the same steps with the loss copied to the host (nll.item()) after every step"}
     log("[bench] %s synchronous: %.4f ms/step" % (label, sync["ms_per_step"]))
+    # the same points through the C ABI alone (ctpwa_nll_grad into preallocated outputs):
+    # the evaluation without autograd's two kernels (gradient seed, gradient x grad_out)
+    h = ana._h
+    outs = [torch.empty(1 + ana.getNParams(), dtype=torch.float64, device="cuda") for _ in range(4)]
+    stream = torch.cuda.current_stream().cuda_stream
+    raw = [p.detach() for p in pts]
+    for k in range(args.warmup):
+        h.nll_grad(raw[k].data_ptr(), outs[k % 4].data_ptr(), stream)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(args.warmup, nsteps):
+        h.nll_grad(raw[k].data_ptr(), outs[k % 4].data_ptr(), stream)
+    torch.cuda.synchronize()
+    dt_c = time.perf_counter() - t
+    c_abi = {"value": round(args.steps / dt_c, 3), "ms_per_step": round(dt_c / args.steps * 1e3, 4),
+             "def": "the same points through ctpwa_nll_grad (the C ABI) into preallocated outputs, no autograd"}
+    log("[bench] %s through the C ABI: %.4f ms/step" % (label, c_abi["ms_per_step"]))
 
     # dominant kernel: the fused per-event kernel, timed with HIP events on the stream it runs on
     lay = ana.layout()
@@ -193,7 +210,7 @@ def run_form(ana, args, spec, torch, W, prec, sizes, label=None, hessian=True):
         hess_ms = (time.perf_counter() - t) / args.hessian_reps * 1e3
         log("[bench] %s getHessian %.3f ms" % (prec, hess_ms))
     return {"value": value, "ms_per_step": ms_per_step, "nll": nll_last, "roofline": roof, "hessian_ms": hess_ms,
-            "sync": sync}
+            "sync": sync, "c_abi": c_abi}
 
 
 def run_fixed_theta(args, cfg, samples, spec, torch, W, precisions, sizes):
@@ -411,7 +428,8 @@ def main():
     if len(precisions) == 2:
         other = results[precisions[1]]
         compare = {"precision": precisions[1], "value": round(other["value"], 3),
-                   "ms_per_step": round(other["ms_per_step"], 4), "sync": other["sync"], "roofline": other["roofline"],
+                   "ms_per_step": round(other["ms_per_step"], 4), "sync": other["sync"], "c_abi": other["c_abi"],
+                   "roofline": other["roofline"],
                    "nll": other["nll"],
                    "speedup_mixed_vs_fp64": round(r["value"] / other["value"], 3),
                    "delta_nll_mixed_minus_fp64": r["nll"] - other["nll"],
@@ -456,6 +474,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if primary == "fp64" else "f64 arithmetic, f32 amplitude storage",
         "sync": r["sync"],
+        "c_abi": r["c_abi"],
         "data": "synthetic",
         "config": {"workload": args.workload + ": J/psi -> pi+ pi- pi0, 3 chains x (rho770, rho1450, "
                    "rho1700) + f0(980) + f2(1270), %d amplitudes, %d free resonance parameters, "
