@@ -23,6 +23,7 @@
 // reduction then writes one partial row per workgroup.
 #include <algorithm>
 #include <functional>
+#include <map>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -555,6 +556,57 @@ std::string generate_custom2_kernel(const Model_& M) {
     return o.str();
 }
 
+// Denominator coefficients that the prep kernel computes by the same operations from the
+// same inputs, so that their values are equal bit for bit: m0^2 (BWR, BW, Flatte) of the
+// same mass source (theta slot, or fixed value), the BW width term m0 g0 of the same mass
+// and width, and the BWR width term g0 m0^2 N_L(q0 d) / q0^(2L+1) of the same mass, width,
+// L, barrier radius and fixed daughter masses (the rho states of the pi+- pi0 chains of a
+// three-pion model).  The fused kernel loads each distinct one once: fewer scalar
+// registers, which otherwise run out and spill to vector-register lanes that every event
+// iteration reads back.  alias[d][c] = (d' << 8) | c' of the first equal coefficient, or -1.
+static std::vector<std::vector<int>> shared_coefficients(const Model_& M) {
+    const Plan& P = M.P;
+    std::vector<std::vector<int>> alias(M.dens.size());
+    std::map<std::string, int> first;
+    auto src = [](const PRes& R, int k) {
+        std::ostringstream q;
+        q.precision(17);
+        if ((int)R.params.size() <= k) q << "none";
+        else if ((int)R.slots.size() > k && R.slots[k] >= 0) q << "slot" << R.slots[k];
+        else q << "val" << R.params[k];
+        return q.str();
+    };
+    for (size_t d = 0; d < M.dens.size(); ++d) {
+        const DenSpec& D = M.dens[d];
+        alias[d].assign(D.ncoef, -1);
+        if (D.stat || D.ncoef > 255) continue;
+        const PRes& R = P.blocks[D.block].res[D.res];
+        const PNode& n = P.topos[P.blocks[D.block].topo].nodes[R.node];
+        if (R.model != M_BWR && R.model != M_BW && R.model != M_FLATTE) continue;
+        for (int c = 0; c < D.ncoef && c < 2; ++c) {
+            std::ostringstream key;
+            key.precision(17);
+            if (c == 0) {
+                key << "m2|" << (R.params.empty() ? std::string("one") : src(R, 0));
+            } else if (R.model == M_BW) {
+                key << "bw1|" << src(R, 0) << "|" << src(R, 1);
+            } else if (R.model == M_BWR) {
+                // q0 from fixed daughter masses only (a cascade's daughter mass parameter
+                // makes q0 depend on the block)
+                if (!(n.md1 > 0 || n.d1ref >= 0) || !(n.md2 > 0 || n.d2ref >= 0)) continue;
+                key << "bwr1|" << src(R, 0) << "|" << src(R, 1) << "|" << n.lmin << "|" << (n.has_bf ? n.bf_d : 0.0) << "|"
+                    << (n.md1 > 0 ? n.md1 : 0.0) << "|" << (n.md2 > 0 ? n.md2 : 0.0);
+            } else {
+                continue;
+            }
+            auto it = first.find(key.str());
+            if (it == first.end()) first[key.str()] = ((int)d << 8) | c;
+            else alias[d][c] = it->second;
+        }
+    }
+    return alias;
+}
+
 std::string generate_eval_kernel(const Model_& M) {
     const Plan& P = M.P;
     int NU = (int)M.us.size(), NP = P.npol, NA = P.namp;
@@ -627,6 +679,7 @@ std::string generate_eval_kernel(const Model_& M) {
          "  const double LOG_FLOOR = log(1e-10);\n";
     // theta-only coefficients (wave-uniform)
     o << "  const double* tb = tab;\n";
+    const std::vector<std::vector<int>> coef_alias = shared_coefficients(M);
     // ul: inside eval_body, whose UL instantiation (the grouped kernels, where the table
     // pointer comes from a job descriptor in memory rather than a kernel argument) reads
     // the coefficients through uld() so that they stay in scalar registers
@@ -638,9 +691,15 @@ std::string generate_eval_kernel(const Model_& M) {
             o << ind << "const double c" << a << "r = " << ld("OFFC + " + std::to_string(2 * a)) << ", c" << a
               << "i = " << ld("OFFC + " + std::to_string(2 * a + 1)) << ";\n";
         for (size_t d = 0; d < M.dens.size(); ++d)
-            for (int c = 0; c < M.dens[d].ncoef; ++c)
+            for (int c = 0; c < M.dens[d].ncoef; ++c) {
+                const int src = coef_alias[d][c];
+                if (src >= 0) {
+                    o << ind << "const double k" << d << "_" << c << " = k" << (src >> 8) << "_" << (src & 255) << ";\n";
+                    continue;
+                }
                 o << ind << "const double k" << d << "_" << c << " = "
                   << ld("OFFR + " + std::to_string((long long)(M.dens[d].coef_base + c)) + "LL * REC") << ";\n";
+            }
     };
     emit_coefs("  ", true);
     o << "  double av = 0.0;\n";
