@@ -53,5 +53,8 @@ at::Tensor nll(const at::Tensor& params, int64_t handle, int64_t stream, int64_t
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "ctpwa likelihood as a C++ autograd function over libctpwa_amd.so";
-    m.def("nll", &nll, "NLL with the analytic gradient as its backward");
+    // the GIL is released while the function records its graph node and launches the
+    // evaluation (other Python threads, such as a service's admission thread, run meanwhile)
+    m.def("nll", &nll, pybind11::call_guard<pybind11::gil_scoped_release>(),
+          "NLL with the analytic gradient as its backward");
 }
