@@ -57,6 +57,28 @@ def test_generated_kernel_compiles_for_gfx950(name, precision):
     assert _native.compile_check(plan.to_text(), precision) > 1000
 
 
+def test_equal_denominator_coefficients_are_loaded_once():
+    """The three-pion model carries rho(770), rho(1450) and rho(1700) in each of its three
+    chains: every rho mass term m0^2 is loaded once (6 aliases), and the BWR width terms of
+    the two pi+- pi0 chains (equal daughter masses) are shared (3 aliases) while those of
+    the pi+ pi- chain (other daughter masses) are not.  An alias always names a coefficient
+    loaded earlier.  The single-chain model has nothing to share."""
+    import re
+    from ctpwa_amd import _native
+    src = _native.generate_source(plan_of("jpsi_3pi").to_text(), "mixed")
+    body = src[src.index("eval_body("):]
+    aliases = re.findall(r"const double (k\d+_\d+) = (k\d+_\d+);", body)
+    loads = re.findall(r"const double (k\d+_\d+) = \(UL \? uld", body)
+    assert len(aliases) == 9 and len(loads) == 11
+    for dst, srcv in aliases:
+        assert srcv in loads and dst not in loads
+        assert srcv.endswith(dst[-2:])            # same coefficient index (m0^2 or width term)
+    masses = {d: s_ for d, s_ in aliases if d.endswith("_0")}
+    assert sorted(set(masses.values())) == ["k0_0", "k1_0", "k2_0"]
+    one = _native.generate_source(plan_of("jpsi_rho").to_text(), "fp64")
+    assert not re.findall(r"const double k\d+_\d+ = k\d+_\d+;", one)
+
+
 def test_footprint_and_layout_formula():
     from ctpwa_amd import _native
     plan = plan_of("jpsi_3pi")
