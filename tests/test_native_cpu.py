@@ -136,3 +136,13 @@ def test_wide_cascade_plan_generates():
     src = _native.generate_source(plan.to_text(), "fp64")
     hsrc = _native.generate_hessian_source(plan.to_text(), "mixed")
     assert "ctpwa_eval" in src and "ctpwa_hess_rows" in hsrc and "float* __restrict__ J" in hsrc
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus above 1 exits with a message before any device work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
